@@ -1,0 +1,88 @@
+// Device-side data layout shared by the HIP kernels and the C-ABI layer.
+//
+// One instance lives in HBM as row-major n x P float32 matrices (P = n
+// rounded up to a multiple of 4 so rows are 16-byte aligned for float4
+// passes): dist, heur (= eta^beta, times an optional prior), tau and
+// total_info.  Candidate rows (n x k ids and their total_info) are stored
+// contiguously so one construction step reads k*8 bytes.  Tours are
+// int32[(m+1) x T], T = 2n, slot m holds the best-so-far tour; sizes and
+// lengths are separate arrays (the reference packs the size into the last
+// tour slot, g_aco.cpp:39,640-653).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cvrp {
+
+constexpr int kLanes = 64;             // wavefront = one ant
+constexpr float kEps = 0.001f;         // utilities.h:19
+constexpr int kMaxNodes = 4096;        // 12-bit node ids in reduction keys
+constexpr int kMaxRas = 16;
+constexpr int kMaxMovesPerRoute = 1 << 14;
+constexpr int kMaxMovesFactor = 64;    // swap/relocate cap = 64*n per call
+
+enum { kSelIRoulette = 0, kSelKnn = 1 };
+enum { kRngParkMiller = 0, kRngPhilox = 1 };
+enum { kLs2Opt = 1, kLsSwap = 2, kLsRelocate = 4 };
+
+// control words (DevState::ctrl)
+enum { kCtrlIter = 0, kCtrlNbsf = 1, kCtrlIterBest = 2, kCtrlFlag = 3, kCtrlOverflow = 4,
+       kCtrlWords = 8 };
+// running totals (DevState::totals)
+enum { kTotEvals = 0, kTotMoves = 1, kTotFallbacks = 2, kTotIters = 3, kTotWords = 4 };
+
+struct BsfRecord {
+    int32_t iter;
+    float length;
+    float time;
+};
+
+struct DevProblem {
+    int n, P, k, nn_ls, m, T, ras, Q;
+    float D, svc, alpha, rho;
+    int has_d;                // max_dist is finite
+    const float *dist;        // n*P
+    const float *d0;          // n, distance to the depot (column 0)
+    const int *dem;           // n
+    const float *heur;        // n*P
+    const int *nnk;           // n*k
+    const int *nnls;          // n*nn_ls
+    const int *rank;          // n, priority order of the I-Roulette reduction
+    const int *rank64;        // 64, same for best_solution_phase_0 groups
+    const int *rank64inv;     // 64
+};
+
+struct DevState {
+    float *tau, *tinfo, *tiknn;
+    int *tours, *sizes;
+    float *lens;
+    int *pm;                  // m*64 Park-Miller lane states
+    int *ctrl;                // kCtrlWords
+    BsfRecord *bsf;
+    int bsf_cap;
+    int *elite;               // ras
+    int *succ;                // ras*n: successor of customer j in elite g
+    unsigned char *firstf;    // ras*P: h directly follows the depot in elite g
+    float *delta;             // ras
+    long long *ant_evals;     // m
+    int *ant_moves, *ant_fb;  // m
+    long long *totals;        // kTotWords
+    unsigned long long *clock0;
+};
+
+// ---- host launchers (aco_kernels.hip) ----
+hipError_t launch_construct(const DevProblem &p, const DevState &s, int selection, int rng,
+                            uint64_t seed, hipStream_t st);
+hipError_t launch_local_search(const DevProblem &p, const DevState &s, int flags, hipStream_t st);
+hipError_t launch_statistics(const DevProblem &p, const DevState &s, float tick_to_s,
+                             hipStream_t st);
+hipError_t launch_pheromone_update(const DevProblem &p, const DevState &s, hipStream_t st);
+hipError_t launch_gather(const DevProblem &p, const DevState &s, hipStream_t st);
+hipError_t launch_pheromone_init(const DevProblem &p, const DevState &s, float tau0, int from_bsf,
+                                 hipStream_t st);
+hipError_t launch_stamp_clock(const DevState &s, hipStream_t st);
+size_t construct_lds_bytes(int n, int selection);
+size_t local_search_lds_bytes(int n);
+
+}  // namespace cvrp

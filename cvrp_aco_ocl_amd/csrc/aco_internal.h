@@ -1,0 +1,37 @@
+// Internal declarations shared by the host-side translation units of
+// libcvrpaco.so (not part of the public C ABI).
+#pragma once
+
+#include <math.h>
+
+#include <string>
+#include <vector>
+
+#include "cvrp_aco.h"
+
+namespace cvrp {
+
+enum { kDistFloat = 0, kDistNint = 1, kDistCeil = 2, kDistGeo = 3, kDistAtt = 4 };
+
+struct VrpInstance {
+    std::string name;
+    int n = 0;
+    std::vector<double> x, y;
+    std::vector<int> demand;
+    int capacity = 0;
+    float max_dist = INFINITY;
+    float service = 0.0f;
+    float optimum = 0.0f;
+    int edge_type = 0;
+};
+
+int set_error(int code, const std::string &msg);
+float distance_of(int mode, double xi, double yi, double xj, double yj);
+void nn_lists(int n, const float *dist, int nn, int tie, int *out);
+int check_solution(int n, const float *dist, const int *dem, int Q, float D, float svc,
+                   const int *tour, int size);
+float nn_tour_length(int n, const float *dist, const int *dem, int Q, float D, float svc);
+void tree_rank(int N, int *rank);
+int read_vrplib(const char *path, VrpInstance &inst, std::string &err);
+
+}  // namespace cvrp

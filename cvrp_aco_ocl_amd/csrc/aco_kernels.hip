@@ -1,0 +1,602 @@
+// HIP kernels (gfx950) for one ACO iteration: solution construction,
+// iteration statistics, the fused pheromone pass and candidate-row gather.
+// Local search lives in aco_local_search.hip.
+//
+// Arithmetic contract: compiled with -ffp-contract=off, IEEE division and
+// denormals kept, so every float result equals the CPU restatement's.
+#include "aco_device.h"
+
+namespace cvrp {
+
+using u32 = uint32_t;
+using u64 = unsigned long long;
+
+__device__ __forceinline__ u32 f2u(float f) { return __float_as_uint(f); }
+
+// ---------------------------------------------------------------- RNG ----
+
+// Park-Miller minimal standard (kernel.cl:42-52): z <- 16807 z mod (2^31-1)
+// by Schrage's method; ran01 = float(z) * 2^-31.
+__device__ __forceinline__ float pm_ran01(int &z)
+{
+    int k = z / 127773;
+    z = 16807 * (z - k * 127773) - 2836 * k;
+    if (z < 0) z += 2147483647;
+    return (float)z * 0x1p-31f;
+}
+
+// Philox-4x32-10 (Salmon et al., SC'11), word 0 of ctr=(iter, ant, step, 0),
+// top 24 bits -> [0,1).
+__device__ __forceinline__ float philox_u01(u64 seed, u32 iter, u32 ant, u32 step)
+{
+    u32 c0 = iter, c1 = ant, c2 = step, c3 = 0u;
+    u32 k0 = (u32)seed, k1 = (u32)(seed >> 32);
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        const u32 hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
+        const u32 hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+        const u32 n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+    }
+    return (float)(c0 >> 8) * 0x1p-24f;
+}
+
+// ------------------------------------------------------- wave helpers ----
+
+__device__ __forceinline__ u64 wave_max_u64(u64 v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const u64 w = __shfl_xor(v, o);
+        v = w > v ? w : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ u64 wave_min_u64(u64 v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const u64 w = __shfl_xor(v, o);
+        v = w < v ? w : v;
+    }
+    return v;
+}
+
+// Inclusive Hillis-Steele scan, offsets 1..32 (the association the CPU
+// restatement emulates exactly).
+__device__ __forceinline__ float wave_scan_f32(float x, int lane)
+{
+#pragma unroll
+    for (int d = 1; d < kLanes; d <<= 1) {
+        const float y = __shfl_up(x, d);
+        if (lane >= d) x = x + y;
+    }
+    return x;
+}
+
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// ------------------------------------------------------- construction ----
+
+// Shared per-ant LDS: demands, depot distances, (I-Roulette) reduction rank
+// and the visited flags.
+struct AntLds {
+    int *dem;
+    float *d0;
+    int *rank;
+    unsigned char *vis;
+};
+
+__device__ __forceinline__ AntLds ant_lds(unsigned char *smem, int n, bool with_rank)
+{
+    AntLds L;
+    L.dem = reinterpret_cast<int *>(smem);
+    L.d0 = reinterpret_cast<float *>(L.dem + n);
+    L.rank = reinterpret_cast<int *>(L.d0 + n);
+    L.vis = reinterpret_cast<unsigned char *>(with_rank ? L.rank + n : L.rank);
+    return L;
+}
+
+size_t construct_lds_bytes(int n, int selection)
+{
+    return (size_t)n * (4 + 4 + (selection == kSelIRoulette ? 4 : 0) + 1) + 16;
+}
+
+// kernel.cl:445-452 mask: not visited, load + d_i <= Q and, with a route
+// length limit, rdist + (d(cur,i) + svc) + d(i,0) <= D.
+template <bool HAS_D>
+__device__ __forceinline__ bool feasible(const DevProblem &p, const AntLds &L, int i, float dci,
+                                         int load, float rdist)
+{
+    if (L.vis[i]) return false;
+    if (load + L.dem[i] > p.Q) return false;
+    if (HAS_D) {
+        float t = rdist + (dci + p.svc);
+        t = t + L.d0[i];
+        return t <= p.D;
+    }
+    return true;
+}
+
+// Full-row greedy pick when the roulette has no positive mass: largest
+// total_info among feasible nodes, ties to the lowest index; 0 = none.
+template <bool HAS_D>
+__device__ int greedy_pick(const DevProblem &p, const AntLds &L, const float *tirow,
+                           const float *drow, int load, float rdist, int lane)
+{
+    u64 best = 0;
+    for (int i = lane; i < p.n; i += kLanes) {
+        const float dci = HAS_D ? drow[i] : 0.0f;
+        if (feasible<HAS_D>(p, L, i, dci, load, rdist)) {
+            const u64 key = ((u64)(f2u(tirow[i]) + 1u) << 32) | (u64)(0xFFFFFFFFu - (u32)i);
+            best = key > best ? key : best;
+        }
+    }
+    best = wave_max_u64(best);
+    return best ? (int)(0xFFFFFFFFu - (u32)(best & 0xFFFFFFFFull)) : 0;
+}
+
+// Tour positions are buffered one per lane (position q lives in lane q&63)
+// and written 64 at a time.
+__device__ __forceinline__ void tour_put(int *tour, int &tbuf, int step, int w, int lane)
+{
+    if (lane == (step & 63)) tbuf = w;
+    if ((step & 63) == 63) tour[step - 63 + lane] = tbuf;
+}
+
+__device__ __forceinline__ void tour_close(int *tour, int &tbuf, int step, int lane)
+{
+    if (lane == (step & 63)) tbuf = 0;
+    if (lane <= (step & 63)) tour[(step & ~63) + lane] = tbuf;
+}
+
+// construct_solution with I-Roulette (kernel.cl:325-373, 382-493): one
+// wavefront per ant, lane l draws for nodes l, l+64, ... from its own
+// Park-Miller stream; the reduction's tie order is reproduced by taking the
+// max of (prob, 4095-rank, node) keys.
+template <bool HAS_D>
+__global__ __launch_bounds__(64) void k_construct_iroulette(DevProblem p, DevState s)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int n = p.n, lane = threadIdx.x, a = blockIdx.x;
+    AntLds L = ant_lds(smem, n, true);
+    for (int i = lane; i < n; i += kLanes) {
+        L.dem[i] = p.dem[i];
+        L.d0[i] = p.d0[i];
+        L.rank[i] = p.rank[i];
+        L.vis[i] = i == 0;
+    }
+    __syncthreads();
+    int z = s.pm[a * kLanes + lane];
+    int *tour = s.tours + (size_t)a * p.T;
+    int cur = 0, load = 0, vcnt = 1, step = 0, fb = 0, tbuf = 0;
+    float rdist = 0.0f, tdist = 0.0f;
+    while (vcnt < n) {
+        const float *tirow = s.tinfo + (size_t)cur * p.P;
+        const float *drow = p.dist + (size_t)cur * p.P;
+        u64 best = 0;
+        for (int i = lane; i < n; i += kLanes) {
+            const float r = pm_ran01(z) + 0.001f;
+            const float pp = tirow[i] * 100.0f;
+            const float dci = HAS_D ? drow[i] : 0.0f;
+            const bool c = feasible<HAS_D>(p, L, i, dci, load, rdist);
+            const float prob = c ? r * pp : 0.0f;
+            const u64 key = ((u64)f2u(prob) << 32) | ((u64)(4095 - L.rank[i]) << 12) | (u64)i;
+            best = key > best ? key : best;
+        }
+        best = wave_max_u64(best);
+        step++;
+        int w;
+        if ((best >> 32) == 0) {
+            if (cur == 0) {
+                w = greedy_pick<HAS_D>(p, L, tirow, drow, load, rdist, lane);
+                fb++;
+            } else {
+                w = 0;
+            }
+        } else {
+            w = (int)(best & 0xFFFull);
+        }
+        w = uniform(w);
+        const float dcw = drow[w];
+        if (w == 0) {
+            load = 0;
+            rdist = 0.0f;
+        } else {
+            load += L.dem[w];
+            rdist = rdist + (dcw + p.svc);
+            vcnt++;
+        }
+        if (lane == 0) L.vis[w] = 1;
+        tdist = tdist + dcw;
+        tour_put(tour, tbuf, step, w, lane);
+        cur = w;
+        __syncthreads();
+    }
+    tdist = tdist + L.d0[cur];
+    step++;
+    tour_close(tour, tbuf, step, lane);
+    if (lane == 0) {
+        s.sizes[a] = step + 1;
+        s.lens[a] = tdist;
+        s.ant_fb[a] = fb;
+        s.ant_evals[a] = 0;
+        s.ant_moves[a] = 0;
+    }
+    s.pm[a * kLanes + lane] = z;
+}
+
+// Roulette over the k-nearest candidate row: lane c holds candidate c's
+// total_info masked by feasibility, a wavefront scan gives the wheel,
+// r = u * wheel[k-1] and the first lane whose prefix exceeds r wins (the
+// last lane with mass when rounding leaves none).  No mass -> full-row
+// greedy pick, and the depot when nothing is feasible.
+template <bool HAS_D, int RNG>
+__global__ __launch_bounds__(64) void k_construct_knn(DevProblem p, DevState s, u64 seed)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int n = p.n, k = p.k, lane = threadIdx.x, a = blockIdx.x;
+    AntLds L = ant_lds(smem, n, false);
+    for (int i = lane; i < n; i += kLanes) {
+        L.dem[i] = p.dem[i];
+        L.d0[i] = p.d0[i];
+        L.vis[i] = i == 0;
+    }
+    __syncthreads();
+    const u32 iter = (u32)s.ctrl[kCtrlIter];
+    int z = RNG == kRngParkMiller ? s.pm[a * kLanes] : 0;  // lane 0's stream, held by all lanes
+    int *tour = s.tours + (size_t)a * p.T;
+    int cur = 0, load = 0, vcnt = 1, step = 0, fb = 0, tbuf = 0;
+    float rdist = 0.0f, tdist = 0.0f;
+    while (vcnt < n) {
+        step++;
+        const float *drow = p.dist + (size_t)cur * p.P;
+        int j = 0;
+        float pj = 0.0f;
+        if (lane < k) {
+            j = p.nnk[cur * k + lane];
+            const float tj = s.tiknn[cur * k + lane];
+            const float dcj = HAS_D ? drow[j] : 0.0f;
+            pj = feasible<HAS_D>(p, L, j, dcj, load, rdist) ? tj : 0.0f;
+        }
+        const float x = wave_scan_f32(pj, lane);
+        const float total = __shfl(x, k - 1);
+        int w;
+        if (total > 0.0f) {
+            const float u = RNG == kRngPhilox ? philox_u01(seed, iter, (u32)a, (u32)step)
+                                              : pm_ran01(z);
+            const float r = u * total;
+            const u64 m1 = __ballot(lane < k && x > r);
+            int wc;
+            if (m1) {
+                wc = __ffsll((unsigned long long)m1) - 1;
+            } else {
+                const u64 m2 = __ballot(pj > 0.0f);
+                wc = 63 - __clzll((long long)m2);
+            }
+            w = __shfl(j, wc);
+        } else {
+            w = greedy_pick<HAS_D>(p, L, s.tinfo + (size_t)cur * p.P, drow, load, rdist, lane);
+            fb++;
+        }
+        w = uniform(w);
+        const float dcw = drow[w];
+        if (w == 0) {
+            load = 0;
+            rdist = 0.0f;
+        } else {
+            load += L.dem[w];
+            rdist = rdist + (dcw + p.svc);
+            vcnt++;
+        }
+        if (lane == 0) L.vis[w] = 1;
+        tdist = tdist + dcw;
+        tour_put(tour, tbuf, step, w, lane);
+        cur = w;
+        __syncthreads();
+    }
+    tdist = tdist + L.d0[cur];
+    step++;
+    tour_close(tour, tbuf, step, lane);
+    if (lane == 0) {
+        s.sizes[a] = step + 1;
+        s.lens[a] = tdist;
+        s.ant_fb[a] = fb;
+        s.ant_evals[a] = 0;
+        s.ant_moves[a] = 0;
+        if (RNG == kRngParkMiller) s.pm[a * kLanes] = z;
+    }
+}
+
+hipError_t launch_construct(const DevProblem &p, const DevState &s, int selection, int rng,
+                            uint64_t seed, hipStream_t st)
+{
+    const size_t lds = construct_lds_bytes(p.n, selection);
+    const dim3 grid(p.m), block(kLanes);
+    if (selection == kSelIRoulette) {
+        if (p.has_d) hipLaunchKernelGGL(k_construct_iroulette<true>, grid, block, lds, st, p, s);
+        else hipLaunchKernelGGL(k_construct_iroulette<false>, grid, block, lds, st, p, s);
+    } else if (rng == kRngPhilox) {
+        if (p.has_d) hipLaunchKernelGGL((k_construct_knn<true, kRngPhilox>), grid, block, lds, st, p, s, (u64)seed);
+        else hipLaunchKernelGGL((k_construct_knn<false, kRngPhilox>), grid, block, lds, st, p, s, (u64)seed);
+    } else {
+        if (p.has_d) hipLaunchKernelGGL((k_construct_knn<true, kRngParkMiller>), grid, block, lds, st, p, s, (u64)seed);
+        else hipLaunchKernelGGL((k_construct_knn<false, kRngParkMiller>), grid, block, lds, st, p, s, (u64)seed);
+    }
+    return hipGetLastError();
+}
+
+// --------------------------------------------------------- statistics ----
+
+constexpr int kStatThreads = 1024;
+constexpr int kStatWaves = kStatThreads / kLanes;
+
+__device__ u64 block_min_u64(u64 v, u64 *red)
+{
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    v = wave_min_u64(v);
+    if (lane == 0) red[wv] = v;
+    __syncthreads();
+    if (wv == 0) {
+        u64 x = lane < kStatWaves ? red[lane] : ~0ull;
+        x = wave_min_u64(x);
+        if (lane == 0) red[kStatWaves] = x;
+    }
+    __syncthreads();
+    const u64 r = red[kStatWaves];
+    __syncthreads();
+    return r;
+}
+
+__device__ long long block_sum_i64(long long v, long long *red)
+{
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if (lane == 0) red[wv] = v;
+    __syncthreads();
+    long long r = 0;
+    for (int w = 0; w < kStatWaves; ++w) r += red[w];
+    __syncthreads();
+    return r;
+}
+
+// update_statistics (g_aco.cpp:407-522): iteration best by the tie order of
+// best_solution_phase_0/1 (kernel.cl:594-665), best-so-far copy and record
+// (kernel.cl:546-586) stamped with the device clock; then get_elites
+// (kernel.cl:717-741), the deposit amounts (kernel.cl:767-768) and the
+// elite edge tables the pheromone pass reads; finally the iteration count.
+__global__ __launch_bounds__(kStatThreads) void k_statistics(DevProblem p, DevState s,
+                                                            float tick_to_s)
+{
+    __shared__ u64 red[kStatWaves + 1];
+    __shared__ long long red64[kStatWaves];
+    __shared__ int sh_elite[kMaxRas];
+    const int tid = threadIdx.x, m = p.m, T = p.T;
+
+    // 1. iteration best: key (length bits, group, rank-in-group)
+    u64 kmin = ~0ull;
+    for (int a = tid; a < m; a += kStatThreads) {
+        const u64 key = ((u64)f2u(s.lens[a]) << 32) | (u64)((a & ~63) | p.rank64[a & 63]);
+        kmin = key < kmin ? key : kmin;
+    }
+    kmin = block_min_u64(kmin, red);
+    const u32 lo = (u32)(kmin & 0xFFFFFFFFull);
+    const int best = (int)((lo & ~63u) | (u32)p.rank64inv[lo & 63]);
+    const float best_len = s.lens[best];
+    const float bsf_len = s.lens[m];
+    const bool flag = best_len - bsf_len < -kEps;
+    __syncthreads();
+
+    // 2. best-so-far copy and record
+    if (flag) {
+        const int sz = s.sizes[best];
+        const int *src = s.tours + (size_t)best * T;
+        int *dst = s.tours + (size_t)m * T;
+        for (int q = tid; q < sz; q += kStatThreads) dst[q] = src[q];
+        if (tid == 0) {
+            s.sizes[m] = sz;
+            s.lens[m] = best_len;
+            const int pos = s.ctrl[kCtrlNbsf];
+            if (pos < s.bsf_cap) {
+                BsfRecord r;
+                r.iter = s.ctrl[kCtrlIter];
+                r.length = best_len;
+                r.time = (float)((double)(wall_clock64() - *s.clock0) * (double)tick_to_s);
+                s.bsf[pos] = r;
+                s.ctrl[kCtrlNbsf] = pos + 1;
+            } else {
+                s.ctrl[kCtrlOverflow] += 1;
+            }
+        }
+    }
+    if (tid == 0) {
+        s.ctrl[kCtrlIterBest] = best;
+        s.ctrl[kCtrlFlag] = flag ? 1 : 0;
+    }
+
+    // 3. elites: best-so-far slot, then ras-1 iteration ranks, ties to the
+    //    lowest index (repeated strict '<' scans of get_elites)
+    if (tid == 0) sh_elite[0] = m;
+    __syncthreads();
+    for (int g = 1; g < p.ras; ++g) {
+        u64 km = ~0ull;
+        for (int a = tid; a < m; a += kStatThreads) {
+            bool taken = false;
+            for (int q = 1; q < g; ++q) taken |= sh_elite[q] == a;
+            const float v = taken ? __int_as_float(0x7f800000) : s.lens[a];
+            const u64 key = ((u64)f2u(v) << 32) | (u64)a;
+            km = key < km ? key : km;
+        }
+        km = block_min_u64(km, red);
+        if (tid == 0) sh_elite[g] = (km >> 32) == 0x7f800000ull ? 0 : (int)(km & 0xFFFFFFFFull);
+        __syncthreads();
+    }
+
+    // 4. deposit amounts and elite edge tables
+    for (int g = tid; g < p.ras; g += kStatThreads) {
+        s.elite[g] = sh_elite[g];
+        s.delta[g] = (float)(p.ras - g) / s.lens[sh_elite[g]];
+    }
+    for (int q = tid; q < p.ras * p.P; q += kStatThreads) s.firstf[q] = 0;
+    __syncthreads();
+    for (int g = 0; g < p.ras; ++g) {
+        const int e = sh_elite[g];
+        const int sz = s.sizes[e];
+        const int *t = s.tours + (size_t)e * T;
+        for (int q = tid; q < sz - 1; q += kStatThreads) {
+            const int u = t[q], v = t[q + 1];
+            if (u != 0) s.succ[g * p.n + u] = v;
+            else if (v != 0) s.firstf[(size_t)g * p.P + v] = 1;
+        }
+    }
+
+    // 5. counters and iteration
+    long long ev = 0, mv = 0, fb = 0;
+    for (int a = tid; a < m; a += kStatThreads) {
+        ev += s.ant_evals[a];
+        mv += s.ant_moves[a];
+        fb += s.ant_fb[a];
+    }
+    ev = block_sum_i64(ev, red64);
+    mv = block_sum_i64(mv, red64);
+    fb = block_sum_i64(fb, red64);
+    if (tid == 0) {
+        s.totals[kTotEvals] += ev;
+        s.totals[kTotMoves] += mv;
+        s.totals[kTotFallbacks] += fb;
+        s.totals[kTotIters] += 1;
+        s.ctrl[kCtrlIter] += 1;
+    }
+}
+
+hipError_t launch_statistics(const DevProblem &p, const DevState &s, float tick_to_s,
+                             hipStream_t st)
+{
+    hipLaunchKernelGGL(k_statistics, dim3(1), dim3(kStatThreads), 0, st, p, s, tick_to_s);
+    return hipGetLastError();
+}
+
+// --------------------------------------------------------- pheromone ----
+
+// Fused pheromone_evaporation + pheromone_deposit + compute_total_info
+// (kernel.cl:685-692, 747-776, 782-789) in one pass over n x P.  Each
+// wavefront owns 64 float4 of one row j, so the row's elite successors
+// (succ[g][j], <= ras values) are wave-uniform.  Per element:
+//   t = (1-rho)*tau;  for g in order: if edge (j,h) is in elite g: t += d_g
+//   tau = t;  total_info = t^alpha * heur
+// The elite-order sum is deterministic (the reference's CAS atomics are not).
+__global__ __launch_bounds__(256) void k_pheromone_update(DevProblem p, DevState s, float evap)
+{
+    const int lane = threadIdx.x & 63;
+    const int chunks = (p.P / 4 + 63) / 64;
+    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int j = wave / chunks;
+    if (j >= p.n) return;
+    const int c4 = (wave - j * chunks) * 64 + lane;
+    if (c4 * 4 >= p.P) return;
+    int sc[kMaxRas];
+    float dl[kMaxRas];
+#pragma unroll
+    for (int g = 0; g < kMaxRas; ++g) {
+        sc[g] = -1;
+        dl[g] = 0.0f;
+        if (g < p.ras) {
+            sc[g] = j != 0 ? s.succ[g * p.n + j] : -1;
+            dl[g] = s.delta[g];
+        }
+    }
+    const size_t off = (size_t)j * p.P + (size_t)c4 * 4;
+    float4 t = *reinterpret_cast<const float4 *>(s.tau + off);
+    const float4 h = *reinterpret_cast<const float4 *>(p.heur + off);
+    u32 ff[kMaxRas];
+#pragma unroll
+    for (int g = 0; g < kMaxRas; ++g) {
+        ff[g] = 0;
+        if (g < p.ras && j == 0) ff[g] = *reinterpret_cast<const u32 *>(s.firstf + (size_t)g * p.P + c4 * 4);
+    }
+    float v[4] = {t.x, t.y, t.z, t.w};
+    const float hv[4] = {h.x, h.y, h.z, h.w};
+    float ti[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const int col = c4 * 4 + c;
+        float x = evap * v[c];
+#pragma unroll
+        for (int g = 0; g < kMaxRas; ++g) {
+            if (g < p.ras) {
+                const bool hit = j != 0 ? sc[g] == col : ((ff[g] >> (8 * c)) & 0xFFu) != 0;
+                if (hit) x = x + dl[g];
+            }
+        }
+        v[c] = x;
+        ti[c] = (p.alpha == 1.0f ? x : powf(x, p.alpha)) * hv[c];
+    }
+    *reinterpret_cast<float4 *>(s.tau + off) = make_float4(v[0], v[1], v[2], v[3]);
+    *reinterpret_cast<float4 *>(s.tinfo + off) = make_float4(ti[0], ti[1], ti[2], ti[3]);
+}
+
+hipError_t launch_pheromone_update(const DevProblem &p, const DevState &s, hipStream_t st)
+{
+    const int chunks = (p.P / 4 + 63) / 64;
+    const int waves = p.n * chunks;
+    hipLaunchKernelGGL(k_pheromone_update, dim3((waves + 3) / 4), dim3(256), 0, st, p, s,
+                       1.0f - p.rho);
+    return hipGetLastError();
+}
+
+// total_info of each candidate row, contiguous (n x k)
+__global__ __launch_bounds__(256) void k_gather_knn(DevProblem p, DevState s)
+{
+    const int idx = blockIdx.x * 256 + threadIdx.x;
+    if (idx >= p.n * p.k) return;
+    const int i = idx / p.k;
+    s.tiknn[idx] = s.tinfo[(size_t)i * p.P + p.nnk[idx]];
+}
+
+hipError_t launch_gather(const DevProblem &p, const DevState &s, hipStream_t st)
+{
+    if (p.k <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gather_knn, dim3((p.n * p.k + 255) / 256), dim3(256), 0, st, p, s);
+    return hipGetLastError();
+}
+
+// pheromone_init (kernel.cl:672-679); from_bsf takes tau0 = 1/(rho * L_bsf)
+// on the device (g_aco.cpp:89) so initialisation needs no host round trip.
+__global__ __launch_bounds__(256) void k_pheromone_init(DevProblem p, DevState s, float tau0,
+                                                        int from_bsf)
+{
+    const float t0 = from_bsf ? 1.0f / (p.rho * s.lens[p.m]) : tau0;
+    const float ta = p.alpha == 1.0f ? t0 : powf(t0, p.alpha);
+    const size_t total = (size_t)p.n * p.P / 4;
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < total; q += (size_t)gridDim.x * 256) {
+        const float4 h = reinterpret_cast<const float4 *>(p.heur)[q];
+        reinterpret_cast<float4 *>(s.tau)[q] = make_float4(t0, t0, t0, t0);
+        reinterpret_cast<float4 *>(s.tinfo)[q] = make_float4(ta * h.x, ta * h.y, ta * h.z, ta * h.w);
+    }
+}
+
+hipError_t launch_pheromone_init(const DevProblem &p, const DevState &s, float tau0, int from_bsf,
+                                 hipStream_t st)
+{
+    const size_t total = (size_t)p.n * p.P / 4;
+    int blocks = (int)((total + 255) / 256);
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_pheromone_init, dim3(blocks), dim3(256), 0, st, p, s, tau0, from_bsf);
+    return hipGetLastError();
+}
+
+__global__ void k_stamp_clock(DevState s)
+{
+    if (threadIdx.x == 0) *s.clock0 = wall_clock64();
+}
+
+hipError_t launch_stamp_clock(const DevState &s, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_stamp_clock, dim3(1), dim3(64), 0, st, s);
+    return hipGetLastError();
+}
+
+}  // namespace cvrp

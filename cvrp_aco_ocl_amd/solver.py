@@ -1,0 +1,247 @@
+"""Python mirror of the reference's host structure over the C ABI.
+
+``Instance`` holds what ``Problem`` (cvrp_ocl/problem.h:82-140) carries into
+the loop; ``GpuACO`` mirrors ``g_ACO`` (cvrp_ocl/gpu/g_aco.h:16-37) method
+for method and drives libcvrpaco.so.  Every computation of the loop runs in
+the HIP library; this module only marshals arrays.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import lib, check
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data
+
+
+@dataclass
+class Instance:
+    """One CVRP instance: node 0 is the depot."""
+
+    n: int
+    x: np.ndarray
+    y: np.ndarray
+    demand: np.ndarray
+    capacity: int
+    dist: np.ndarray
+    max_dist: float = math.inf
+    service_time: float = 0.0
+    optimum: float = 0.0
+    name: str = ""
+    dist_mode: int = _lib.DIST_NINT
+    meta: dict = field(default_factory=dict)
+
+    @classmethod
+    def from_coords(cls, x, y, demand, capacity, dist_mode=_lib.DIST_NINT, max_dist=math.inf,
+                    service_time=0.0, name="", optimum=0.0, meta=None):
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.ascontiguousarray(y, np.float64)
+        n = len(x)
+        dist = np.empty((n, n), np.float32)
+        check(lib.aco_compute_distances(n, _ptr(x), _ptr(y), dist_mode, _ptr(dist)),
+              "aco_compute_distances")
+        return cls(n=n, x=x, y=y, demand=np.ascontiguousarray(demand, np.int32),
+                   capacity=int(capacity), dist=dist, max_dist=float(max_dist),
+                   service_time=float(service_time), optimum=float(optimum), name=name,
+                   dist_mode=dist_mode, meta=dict(meta or {}))
+
+    @classmethod
+    def from_vrplib(cls, path: str, dist_mode: int | None = None):
+        """read_instance_file (io.cpp:334-517).  dist_mode None keeps the
+        file's EDGE_WEIGHT_TYPE with EUC_2D as the reference's float sqrt."""
+        n = C.c_int32()
+        check(lib.aco_read_vrplib(path.encode(), 0, C.byref(n), None, None, None, None, None,
+                                  None, None, None), f"aco_read_vrplib({path})")
+        n = n.value
+        x = np.empty(n, np.float64)
+        y = np.empty(n, np.float64)
+        dem = np.empty(n, np.int32)
+        Q = C.c_int32()
+        D = C.c_float()
+        svc = C.c_float()
+        opt = C.c_float()
+        ew = C.c_int32()
+        nn = C.c_int32()
+        check(lib.aco_read_vrplib(path.encode(), n, C.byref(nn), _ptr(x), _ptr(y), _ptr(dem),
+                                  C.byref(Q), C.byref(D), C.byref(svc), C.byref(opt),
+                                  C.byref(ew)), f"aco_read_vrplib({path})")
+        if dist_mode is None:
+            dist_mode = {0: _lib.DIST_FLOAT, 1: _lib.DIST_CEIL, 2: _lib.DIST_GEO,
+                         3: _lib.DIST_ATT}[ew.value]
+        import os
+        return cls.from_coords(x, y, dem, Q.value, dist_mode, D.value, svc.value,
+                               name=os.path.basename(path), optimum=opt.value)
+
+    def nn_lists(self, nn: int, tie: int = 0) -> np.ndarray:
+        out = np.empty((self.n, nn), np.int32)
+        d = np.ascontiguousarray(self.dist, np.float32)
+        check(lib.aco_compute_nn_lists(self.n, _ptr(d), nn, tie, _ptr(out)), "aco_compute_nn_lists")
+        return out
+
+    def check_solution(self, tour) -> bool:
+        t = np.ascontiguousarray(tour, np.int32)
+        d = np.ascontiguousarray(self.dist, np.float32)
+        rc = lib.aco_check_solution(self.n, _ptr(d), _ptr(self.demand), self.capacity,
+                                    self.max_dist, self.service_time, _ptr(t), len(t))
+        if rc < 0:
+            check(rc, "aco_check_solution")
+        return bool(rc)
+
+    def tour_length(self, tour) -> float:
+        t = np.asarray(tour, np.int64)
+        length = np.float32(0.0)
+        for a, b in zip(t[:-1], t[1:]):
+            length = np.float32(length + self.dist[a, b])
+        return float(length)
+
+
+class GpuACO:
+    """g_ACO (cvrp_ocl/gpu/g_aco.h:16-37) on libcvrpaco.so."""
+
+    def __init__(self, inst: Instance, *, alpha=1.0, beta=2.0, rho=0.1, n_ants=0, ras_ranks=6,
+                 k=0, nn_ls=0, selection=_lib.SEL_IROULETTE, rng=_lib.RNG_PARK_MILLER,
+                 ls_flags=0, tau0_rule=_lib.TAU0_FIRST_COLONY, seed=1, use_graph=True,
+                 bsf_capacity=4096, prior=None, nn_list=None):
+        self.inst = inst
+        p = _lib.AcoParams()
+        lib.aco_default_params(C.byref(p))
+        p.alpha, p.beta, p.rho = alpha, beta, rho
+        p.n_ants, p.ras_ranks, p.k, p.nn_ls = n_ants, ras_ranks, k, nn_ls
+        p.selection, p.rng, p.ls_flags, p.tau0_rule = selection, rng, ls_flags, tau0_rule
+        p.seed, p.use_graph, p.bsf_capacity = seed, int(bool(use_graph)), bsf_capacity
+        self.params = p
+        self._dist = np.ascontiguousarray(inst.dist, np.float32)
+        self._dem = np.ascontiguousarray(inst.demand, np.int32)
+        self._prior = None if prior is None else np.ascontiguousarray(prior, np.float32)
+        self._nn = None if nn_list is None else np.ascontiguousarray(nn_list, np.int32)
+        prob = _lib.AcoProblem(inst.n, _ptr(self._dist), _ptr(self._dem), inst.capacity,
+                               inst.max_dist, inst.service_time, _ptr(self._nn),
+                               0 if self._nn is None else self._nn.shape[1], _ptr(self._prior))
+        h = C.c_void_p()
+        check(lib.aco_create(C.byref(prob), C.byref(p), C.byref(h)), "aco_create")
+        self.h = h
+        self.n = inst.n
+        self.m = n_ants if n_ants > 0 else inst.n
+        self.T = 2 * inst.n
+
+    # -- lifetime ----------------------------------------------------------
+    def close(self):
+        if getattr(self, "h", None):
+            lib.aco_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- the reference loop (g_aco.cpp:63-134) -------------------------------
+    def init_aco(self):
+        check(lib.aco_init(self.h), "aco_init")
+
+    def run_aco_iteration(self):
+        check(lib.aco_iteration(self.h), "aco_iteration")
+
+    def run(self, iterations: int):
+        check(lib.aco_run(self.h, iterations), "aco_run")
+
+    def exit_aco(self):
+        """g_aco.cpp:95-117: best-so-far ant and its record trajectory."""
+        tour, length = self.best()
+        return tour, length, self.bsf_records()
+
+    def construct_solutions(self):
+        check(lib.aco_construct(self.h), "aco_construct")
+
+    def local_search(self):
+        check(lib.aco_local_search(self.h), "aco_local_search")
+
+    def update_statistics(self):
+        check(lib.aco_update_statistics(self.h), "aco_update_statistics")
+
+    def pheromone_update(self):
+        check(lib.aco_pheromone_update(self.h), "aco_pheromone_update")
+
+    def pheromone_init(self, tau0: float):
+        check(lib.aco_pheromone_init(self.h, tau0), "aco_pheromone_init")
+
+    def synchronize(self):
+        check(lib.aco_synchronize(self.h), "aco_synchronize")
+
+    # -- state readback ----------------------------------------------------
+    @property
+    def iteration(self) -> int:
+        it = C.c_int32()
+        check(lib.aco_get_iteration(self.h, C.byref(it)), "aco_get_iteration")
+        return it.value
+
+    def best(self):
+        t = np.empty(self.T, np.int32)
+        sz = C.c_int32()
+        L = C.c_float()
+        check(lib.aco_get_best(self.h, _ptr(t), self.T, C.byref(sz), C.byref(L)), "aco_get_best")
+        return t[: sz.value].copy(), L.value
+
+    def ants(self):
+        tours = np.zeros((self.m + 1, self.T), np.int32)
+        sizes = np.zeros(self.m + 1, np.int32)
+        lens = np.zeros(self.m + 1, np.float32)
+        check(lib.aco_get_ants(self.h, _ptr(tours), self.T, _ptr(sizes), _ptr(lens)), "aco_get_ants")
+        return tours, sizes, lens
+
+    def set_ants(self, tours, sizes, lens):
+        tours = np.ascontiguousarray(tours, np.int32)
+        sizes = np.ascontiguousarray(sizes, np.int32)
+        lens = np.ascontiguousarray(lens, np.float32)
+        check(lib.aco_set_ants(self.h, _ptr(tours), tours.shape[1], _ptr(sizes), _ptr(lens)),
+              "aco_set_ants")
+
+    def bsf_records(self):
+        cnt = C.c_int32()
+        check(lib.aco_get_bsf(self.h, None, 0, C.byref(cnt)), "aco_get_bsf")
+        rec = (_lib.AcoBsfRecord * max(cnt.value, 1))()
+        check(lib.aco_get_bsf(self.h, C.cast(rec, C.c_void_p), cnt.value, C.byref(cnt)), "aco_get_bsf")
+        return [(r.iter, r.length, r.time) for r in rec[: cnt.value]]
+
+    def pheromone(self):
+        tau = np.empty((self.n, self.n), np.float32)
+        ti = np.empty((self.n, self.n), np.float32)
+        check(lib.aco_get_pheromone(self.h, _ptr(tau), _ptr(ti)), "aco_get_pheromone")
+        return tau, ti
+
+    def set_pheromone(self, tau, ti):
+        tau = np.ascontiguousarray(tau, np.float32)
+        ti = np.ascontiguousarray(ti, np.float32)
+        check(lib.aco_set_pheromone(self.h, _ptr(tau), _ptr(ti)), "aco_set_pheromone")
+
+    def rng_state(self):
+        out = np.empty((self.m, 64), np.int32)
+        check(lib.aco_get_rng_state(self.h, _ptr(out)), "aco_get_rng_state")
+        return out
+
+    def counters(self) -> dict:
+        c = _lib.AcoCounters()
+        check(lib.aco_get_counters(self.h, C.byref(c)), "aco_get_counters")
+        return {"ls_evals": c.ls_evals, "ls_moves": c.ls_moves, "fallbacks": c.fallbacks,
+                "iterations": c.iterations, "bsf_overflow": c.bsf_overflow}
+
+    def profile_enable(self, on: bool = True):
+        check(lib.aco_profile_enable(self.h, int(on)), "aco_profile_enable")
+
+    def profile(self, name: str):
+        ms = C.c_double()
+        cnt = C.c_int64()
+        check(lib.aco_profile_get(self.h, name.encode(), C.byref(ms), C.byref(cnt)),
+              "aco_profile_get")
+        return ms.value, cnt.value

@@ -1,0 +1,177 @@
+/*
+ * cvrp_aco.h -- C ABI of the MI355X ant-colony CVRP solver (libcvrpaco.so).
+ *
+ * This is the drop-in boundary for the reference's per-iteration loop
+ * (class g_ACO, cvrp_ocl/gpu/g_aco.h:16-37, driven from
+ * cvrp_ocl/main.cpp:80-88).  Plain C types only: pointers, sizes, scalars.
+ * Every function returns ACO_OK (0) or a negative ACO_ERR_* code; the text
+ * of the last error of the calling thread is aco_last_error().  The
+ * reference instead calls exit(1) on any device error
+ * (g_aco.cpp:607-614); the C++ host wrapper restores that behaviour.
+ *
+ * A context owns all device memory and one HIP stream.  Calls are
+ * asynchronous on that stream; aco_get_* synchronise.  A context is not
+ * thread-safe.  Inputs are copied at aco_create; no host pointer is kept.
+ */
+#ifndef CVRP_ACO_H
+#define CVRP_ACO_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ACO_OK               0
+#define ACO_ERR_ARG         -1   /* invalid argument or size               */
+#define ACO_ERR_HIP         -2   /* HIP runtime error                      */
+#define ACO_ERR_STATE       -3   /* call out of order (e.g. before init)   */
+#define ACO_ERR_INFEASIBLE  -4   /* instance has an unservable customer    */
+#define ACO_ERR_NODEV       -5   /* no HIP device                          */
+#define ACO_ERR_IO          -6   /* file could not be read or parsed       */
+
+/* next-node rule */
+#define ACO_SEL_IROULETTE    0   /* I-Roulette over the full row (kernel.cl:325-373) */
+#define ACO_SEL_KNN          1   /* roulette over the k-nearest candidate row        */
+/* random numbers */
+#define ACO_RNG_PARK_MILLER  0   /* per-lane minimal-standard streams (kernel.cl:42-52) */
+#define ACO_RNG_PHILOX       1   /* counter-based Philox-4x32-10                     */
+/* local search flags, applied in this order */
+#define ACO_LS_2OPT          1   /* intra-route 2-opt (kernel.cl:126-319)            */
+#define ACO_LS_SWAP          2   /* swap (localSearch.cpp:301-410)                   */
+#define ACO_LS_RELOCATE      4   /* relocate (neighbourSearch.cpp:227-365)           */
+/* initial trail */
+#define ACO_TAU0_FIRST_COLONY 0  /* 0.5, one colony, then 1/(rho*L) (g_aco.cpp:79-90) */
+#define ACO_TAU0_NN_TOUR      1  /* 1/(n*L_nn) of a nearest-neighbour tour           */
+
+/* Instance (replaces Problem, cvrp_ocl/problem.h:82-140).  Node 0 is the
+ * depot.  dist is n*n row-major and symmetric; with integer (nint)
+ * distances every cost and move delta is exact. */
+typedef struct aco_problem {
+    int32_t        n;             /* nodes including the depot                 */
+    const float   *dist;          /* n*n                                       */
+    const int32_t *demand;        /* n                                         */
+    int32_t        capacity;      /* vehicle_capacity                          */
+    float          max_dist;      /* max_distance, INFINITY when unlimited     */
+    float          service_time;  /* service_time                              */
+    const int32_t *nn_list;       /* optional n*nn_width sorted neighbour ids  */
+    int32_t        nn_width;      /* >= max(k, nn_ls) when nn_list != NULL     */
+    const float   *prior;         /* optional n*n multiplier of eta^beta       */
+} aco_problem;
+
+/* Parameters (problem.cpp:186-220 defaults, runtime instead of macros). */
+typedef struct aco_params {
+    float    alpha, beta, rho;    /* 1, 2, 0.1                                 */
+    int32_t  n_ants;              /* m; 0 = n (the reference's default)        */
+    int32_t  ras_ranks;           /* 6: best-so-far + 5 iteration elites        */
+    int32_t  k;                   /* candidate row width for ACO_SEL_KNN (<=64) */
+    int32_t  nn_ls;               /* neighbour depth used by 2-opt; 0 = n-2     */
+    int32_t  selection;           /* ACO_SEL_*                                 */
+    int32_t  rng;                 /* ACO_RNG_*                                 */
+    int32_t  ls_flags;            /* ACO_LS_* bit set                          */
+    int32_t  tau0_rule;           /* ACO_TAU0_*                                */
+    uint64_t seed;                /* colony seed                               */
+    int32_t  bsf_capacity;        /* best-so-far record slots (4096)           */
+    int32_t  use_graph;           /* 1: aco_iteration replays a hipGraph        */
+} aco_params;
+
+/* BestSolutionInfo (problem.h:62-66); time is seconds since aco_init,
+ * taken from the device clock so a captured iteration needs no host. */
+typedef struct aco_bsf_record {
+    int32_t iter;
+    float   length;
+    float   time;
+} aco_bsf_record;
+
+typedef struct aco_counters {
+    int64_t ls_evals;             /* local-search move evaluations            */
+    int64_t ls_moves;             /* applied local-search moves               */
+    int64_t fallbacks;            /* full-row greedy picks in construction     */
+    int64_t iterations;           /* completed colony iterations              */
+    int32_t bsf_overflow;         /* records dropped because the log was full */
+} aco_counters;
+
+typedef struct aco_ctx aco_ctx;
+
+/* Fills the reference defaults (problem.cpp:186-220, g_aco.cpp:79-90). */
+void aco_default_params(aco_params *params);
+
+/* g_ACO::g_ACO + create_memory_objects (g_aco.cpp:35-43, 640-761):
+ * validates the instance (every customer servable alone: demand <= Q and
+ * d(0,i)+svc+d(i,0) <= max_dist), preprocesses it and allocates device
+ * memory. */
+int aco_create(const aco_problem *problem, const aco_params *params, aco_ctx **ctx);
+/* g_ACO::~g_ACO (g_aco.cpp:45-61) */
+int aco_destroy(aco_ctx *ctx);
+
+/* g_ACO::init_aco (g_aco.cpp:63-93) */
+int aco_init(aco_ctx *ctx);
+/* g_ACO::run_aco_iteration (g_aco.cpp:122-134): construct -> local search
+ * -> statistics -> pheromone update; replayed as one hipGraph when
+ * params.use_graph */
+int aco_iteration(aco_ctx *ctx);
+/* iterations x aco_iteration, without host synchronisation */
+int aco_run(aco_ctx *ctx, int32_t iterations);
+
+/* the steps of one iteration (g_aco.cpp:141-177, 182-228, 407-417, 233-243) */
+int aco_construct(aco_ctx *ctx);
+int aco_local_search(aco_ctx *ctx);
+int aco_update_statistics(aco_ctx *ctx);
+int aco_pheromone_update(aco_ctx *ctx);
+/* g_ACO::pheromone_init (g_aco.cpp:245-263) */
+int aco_pheromone_init(aco_ctx *ctx, float tau0);
+
+int aco_synchronize(aco_ctx *ctx);
+int aco_get_iteration(aco_ctx *ctx, int32_t *iteration);
+
+/* g_ACO::update_best_so_far_from_device (g_aco.cpp:527-552) */
+int aco_get_best(aco_ctx *ctx, int32_t *tour, int32_t capacity, int32_t *size, float *length);
+/* all m ants and the best-so-far slot m: tours[(m+1)*stride] */
+int aco_get_ants(aco_ctx *ctx, int32_t *tours, int32_t stride, int32_t *sizes, float *lengths);
+/* loads m+1 tours (slot m = best-so-far); used to replay a state */
+int aco_set_ants(aco_ctx *ctx, const int32_t *tours, int32_t stride, const int32_t *sizes,
+                 const float *lengths);
+/* best-so-far records (g_aco.cpp:95-117); *count = records written */
+int aco_get_bsf(aco_ctx *ctx, aco_bsf_record *records, int32_t capacity, int32_t *count);
+/* n*n pheromone and total_info, row-major */
+int aco_get_pheromone(aco_ctx *ctx, float *tau, float *total_info);
+int aco_set_pheromone(aco_ctx *ctx, const float *tau, const float *total_info);
+/* m*64 Park-Miller lane states */
+int aco_get_rng_state(aco_ctx *ctx, int32_t *state);
+int aco_get_counters(aco_ctx *ctx, aco_counters *counters);
+/* per-step device time: enable = 1 records a HIP event pair around each
+ * launch (iterations then run without the graph) */
+int aco_profile_enable(aco_ctx *ctx, int32_t enable);
+/* names: "construct", "local_search", "statistics", "pheromone", "gather";
+ * returns total milliseconds and launch count since enabling */
+int aco_profile_get(aco_ctx *ctx, const char *name, double *total_ms, int64_t *launches);
+
+const char *aco_last_error(void);
+
+/* ---- host-side instance preparation (no device needed) ---- */
+
+/* compute_distances (vrpHelper.cpp:160-184).  mode: ACO_DIST_* */
+#define ACO_DIST_FLOAT  0   /* float sqrt, unrounded: round_distance (vrpHelper.cpp:67-81) */
+#define ACO_DIST_NINT   1   /* VRPLIB nint(euclid), the X-set convention                 */
+#define ACO_DIST_CEIL   2   /* ceil_distance (vrpHelper.cpp:83-97)                       */
+#define ACO_DIST_GEO    3   /* geo_distance (vrpHelper.cpp:99-136)                       */
+#define ACO_DIST_ATT    4   /* att_distance (vrpHelper.cpp:138-158)                      */
+int aco_compute_distances(int32_t n, const double *x, const double *y, int32_t mode, float *dist);
+/* compute_nn_lists (vrpHelper.cpp:188-236): self and depot excluded;
+ * tie 0 = the reference's quicksort order, tie 1 = (distance, index) */
+int aco_compute_nn_lists(int32_t n, const float *dist, int32_t nn, int32_t tie, int32_t *out);
+/* read_instance_file (io.cpp:334-517).  First call with capacity 0 to get
+ * *n; then call with capacity >= *n to fill the arrays.  max_dist is
+ * INFINITY and service_time 0 when the file has none; edge_weight_type is
+ * 0 EUC_2D, 1 CEIL_2D, 2 GEO, 3 ATT. */
+int aco_read_vrplib(const char *path, int32_t capacity, int32_t *n, double *x, double *y,
+                    int32_t *demand, int32_t *vehicle_capacity, float *max_dist,
+                    float *service_time, float *optimum, int32_t *edge_weight_type);
+/* check_solution (problem.cpp:226-319); returns 1 valid, 0 invalid */
+int aco_check_solution(int32_t n, const float *dist, const int32_t *demand, int32_t capacity,
+                       float max_dist, float service_time, const int32_t *tour, int32_t size);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CVRP_ACO_H */
