@@ -1,0 +1,235 @@
+"""Benchmark: ACO iterations per second on one MI355X, as ant-solutions/s.
+
+Workload (the reference's metric configuration that fits one GPU): the
+481-node clustered instance -- 480 customers in 8 Gaussian clusters
+(sigma 6) in [0,1000]^2, random depot, demand U{1..100}, capacity for 30
+routes, nint int32 distances -- with 480 ants, alpha 1, beta 2, rho 0.1,
+6 ranks, roulette over the k=30 nearest candidates (Philox draws), and
+2-opt + swap + relocate local search.  One step = one whole iteration
+(construct -> local search -> statistics -> fused pheromone pass), replayed
+as one hipGraph, inputs resident in HBM.
+
+Prints ONE JSON line.  Extra legs: per-kernel device time from HIP event
+pairs on the library's stream (roofline of the dominant kernel), the
+best-so-far trajectory of a full run (time to a 2% gap), and the CPU
+oracle timed on this host (cpu_baseline).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "ant-solutions/s + time-to-2%-gap, learned-prior vs plain ACO, 1 MI355X"
+HBM_PEAK_GBS = 8000.0
+WORKLOAD = dict(customers=480, clusters=8, sigma=6.0, routes=30, m=480, k=30, ls=7,
+                iterations_full=1000)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--full-iterations", type=int, default=WORKLOAD["iterations_full"],
+                    help="length of the best-so-far run for time-to-gap (0 = skip)")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0,
+                    help="CPU baseline sample budget (0 = skip)")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cores)")
+    ap.add_argument("--seed", type=int, default=0)
+    return ap.parse_args()
+
+
+def make_solver(ca, inst, seed):
+    return ca.GpuACO(inst, n_ants=WORKLOAD["m"], k=WORKLOAD["k"], seed=seed + 1,
+                     selection=ca.SEL_KNN, rng=ca.RNG_PHILOX, ls_flags=WORKLOAD["ls"],
+                     use_graph=True)
+
+
+def construct_bytes(inst, sizes, m, k, fallbacks):
+    """Algorithmic bytes of one construction launch: per step k candidate
+    ids + k total_info + the chosen distance + the tour write (8k+8 B), per
+    ant the demand and depot-distance rows (8n B), per full-row fallback
+    the total_info row (4n B)."""
+    steps = int(np.sum(sizes[:m] - 1))
+    n = inst.n
+    return steps * (8 * k + 8) + m * 8 * n + fallbacks * 4 * n
+
+
+def pheromone_bytes(inst):
+    """Fused pass: read tau and heur, write tau and total_info (16 n^2 B)."""
+    return 16 * inst.n * inst.n
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design is "
+              "described in DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+    import cvrp_aco_ocl_amd as ca
+    from cvrp_aco_ocl_amd import instances
+
+    inst = instances.clustered(args.seed, customers=WORKLOAD["customers"],
+                               nclust=WORKLOAD["clusters"], sigma=WORKLOAD["sigma"],
+                               routes=WORKLOAD["routes"])
+    m, k = WORKLOAD["m"], WORKLOAD["k"]
+    print(f"# workload {inst.name}: n={inst.n} Q={inst.capacity} m={m} k={k}", file=sys.stderr,
+          flush=True)
+
+    # ---- timed region: K iterations, graph replay ----
+    g = make_solver(ca, inst, args.seed)
+    g.init_aco()
+    g.run(args.warmup)
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.run(args.steps)
+    g.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = m * args.steps / elapsed
+    cnt0 = g.counters()
+    print(f"# timed {args.steps} iterations: {ms_per_step:.3f} ms/iter, {value:.0f} ant-sol/s",
+          file=sys.stderr, flush=True)
+
+    # ---- per-kernel device time (HIP events on the library stream) ----
+    g.profile_enable(True)
+    fb_before = g.counters()["fallbacks"]
+    ev_before = g.counters()["ls_evals"]
+    size_sum = 0
+    cons_bytes = 0
+    for _ in range(args.profile_steps):
+        fb0 = g.counters()["fallbacks"]
+        g.run_aco_iteration()
+        _, sizes, _ = g.ants()
+        # construct bytes need this iteration's tours before local search moved
+        # nodes around: sizes do not change under local search
+        cons_bytes += construct_bytes(inst, sizes, m, k, g.counters()["fallbacks"] - fb0)
+        size_sum += int(np.sum(sizes[:m]))
+    kern = {}
+    for name in ("construct", "local_search", "statistics", "pheromone", "gather"):
+        ms, launches = g.profile(name)
+        if launches:
+            kern[name] = {"avg_ms": ms / launches, "launches": launches}
+    cnt1 = g.counters()
+    ls_evals = cnt1["ls_evals"] - ev_before
+    g.profile_enable(False)
+    g.close()
+
+    avg = {n: v["avg_ms"] for n, v in kern.items()}
+    dominant = max(avg, key=avg.get)
+    cons_bpl = cons_bytes / args.profile_steps
+    ph_bpl = pheromone_bytes(inst)
+    per_kernel = {}
+    for name, ms in avg.items():
+        entry = {"avg_ms": round(ms, 4)}
+        if name == "construct":
+            entry["algorithmic_bytes"] = int(cons_bpl)
+            entry["GBps"] = round(cons_bpl / (ms * 1e-3) / 1e9, 2)
+        if name == "pheromone":
+            entry["algorithmic_bytes"] = ph_bpl
+            entry["GBps"] = round(ph_bpl / (ms * 1e-3) / 1e9, 2)
+        if name == "local_search":
+            entry["move_evals_per_s"] = round(ls_evals / args.profile_steps / (ms * 1e-3), 1)
+        per_kernel[name] = entry
+    if dominant == "construct":
+        achieved = cons_bpl / (avg["construct"] * 1e-3) / 1e9
+        traffic_alg = int(cons_bpl)
+    elif dominant == "pheromone":
+        achieved = ph_bpl / (avg["pheromone"] * 1e-3) / 1e9
+        traffic_alg = ph_bpl
+    else:
+        # local search: HBM bytes = tours in/out + distance rows staged per
+        # scanned position (3 rows for swap, 1 for relocate) -- the
+        # evaluation itself runs from LDS
+        steps = size_sum / args.profile_steps
+        traffic_alg = int(8 * steps + 4 * 4 * inst.n * steps)
+        achieved = traffic_alg / (avg[dominant] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 2),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                "traffic": None, "algorithmic_bytes_per_launch": traffic_alg}
+
+    # ---- best-so-far trajectory: time to 2% of the run's final best ----
+    quality = None
+    if args.full_iterations > 0:
+        g = make_solver(ca, inst, args.seed)
+        t0 = time.perf_counter()
+        g.init_aco()
+        g.run(args.full_iterations)
+        g.synchronize()
+        wall = time.perf_counter() - t0
+        rec = g.bsf_records()
+        g.close()
+        best = rec[-1][1]
+        target = 1.02 * best
+        hit = next(r for r in rec if r[1] <= target)
+        quality = {"iterations": args.full_iterations, "best": best, "wall_s": round(wall, 3),
+                   "time_to_2pct_gap_s": round(hit[2], 4), "iter_to_2pct_gap": hit[0],
+                   "records": len(rec)}
+        print(f"# full run {args.full_iterations} it: best {best} in {wall:.2f}s, 2% gap at "
+              f"it {hit[0]} ({hit[2]:.3f}s)", file=sys.stderr, flush=True)
+
+    # ---- CPU baseline: the oracle on this host's cores ----
+    cpu = None
+    if args.cpu_seconds > 0:
+        import oracle
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        P = oracle.Problem(inst.dist, inst.demand, inst.capacity, inst.max_dist,
+                           inst.service_time, k=k, tie=0)
+        col = oracle.Colony(P, m, selection=oracle.SEL_KNN, rng=oracle.RNG_PHILOX,
+                            ls_flags=WORKLOAD["ls"], seed=args.seed + 1, threads=threads)
+        col.init()
+        its = 0
+        t0 = time.perf_counter()
+        while True:
+            col.iteration()
+            its += 1
+            el = time.perf_counter() - t0
+            if el >= args.cpu_seconds or its >= 200:
+                break
+        cpu = {"value": round(m * its / el, 2), "unit": "ant-solutions/s", "cores": threads,
+               "kind": "port",
+               "sample": f"{its} whole iterations (480 ants each, same workload, OpenMP over ants) "
+                         f"in {el:.2f}s"}
+        print(f"# cpu baseline: {cpu['value']} ant-sol/s on {threads} threads", file=sys.stderr,
+              flush=True)
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "ant-solutions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"clustered-481: {WORKLOAD['customers']} customers, 8 Gaussian clusters "
+                        f"sigma 6, Q for 30 routes, nint int32 distances; m=480, k=30 roulette_knn "
+                        f"(Philox), 2-opt+swap+relocate, alpha 1 beta 2 rho 0.1, 6 ranks",
+            "n": inst.n, "ants": m, "k": k, "capacity": inst.capacity, "seed": args.seed,
+            "step": "one ACO iteration (hipGraph replay)",
+        },
+        "roofline": roofline,
+        "kernels": per_kernel,
+        "ls_move_evals_per_iter": int(ls_evals / max(args.profile_steps, 1)),
+        "quality": quality,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()
