@@ -61,6 +61,16 @@ _SIGNATURES = {
     "aco_default_params": (None, [C.POINTER(AcoParams)]),
     "aco_create": (C.c_int, [C.POINTER(AcoProblem), C.POINTER(AcoParams), C.POINTER(C.c_void_p)]),
     "aco_destroy": (C.c_int, [_P]),
+    "aco_create_batch": (C.c_int, [C.POINTER(AcoProblem), C.c_int32, C.POINTER(AcoParams),
+                                   C.POINTER(C.c_void_p)]),
+    "aco_batch_size": (C.c_int32, [_P]),
+    "aco_get_best_at": (C.c_int, [_P, C.c_int32, _P, C.c_int32, C.POINTER(C.c_int32),
+                                  C.POINTER(C.c_float)]),
+    "aco_get_ants_at": (C.c_int, [_P, C.c_int32, _P, C.c_int32, _P, _P]),
+    "aco_set_ants_at": (C.c_int, [_P, C.c_int32, _P, C.c_int32, _P, _P]),
+    "aco_get_bsf_at": (C.c_int, [_P, C.c_int32, _P, C.c_int32, C.POINTER(C.c_int32)]),
+    "aco_get_pheromone_at": (C.c_int, [_P, C.c_int32, _P, _P]),
+    "aco_set_pheromone_at": (C.c_int, [_P, C.c_int32, _P, _P]),
     "aco_init": (C.c_int, [_P]),
     "aco_iteration": (C.c_int, [_P]),
     "aco_run": (C.c_int, [_P, C.c_int32]),

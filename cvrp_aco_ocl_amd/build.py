@@ -1,7 +1,8 @@
 """Build libcvrpaco.so (HIP kernels + C ABI, gfx950) and the cvrp_aco CLI in-tree.
 
 Invoked by ``__graft_entry__.build()``; also runnable as
-``python -m cvrp_aco_ocl_amd.build``.  Plain hipcc invocations, no CMake.
+``python cvrp_aco_ocl_amd/build.py`` (not through the package, whose import
+loads the library this script builds).  Plain hipcc invocations, no CMake.
 """
 from __future__ import annotations
 

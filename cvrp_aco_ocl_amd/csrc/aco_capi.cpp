@@ -1,8 +1,9 @@
 // C ABI of libcvrpaco.so: context lifetime, validation, device memory,
 // launch sequencing (eager or one hipGraph per iteration), readback.
 //
-// Mirrors the reference host driver g_ACO (cvrp_ocl/gpu/g_aco.cpp); see
-// include/cvrp_aco.h for the per-function mapping.
+// A context holds one or more instances (aco_create_batch); every launch
+// covers all of them.  Mirrors the reference host driver g_ACO
+// (cvrp_ocl/gpu/g_aco.cpp); include/cvrp_aco.h maps each entry point.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
@@ -53,21 +54,31 @@ struct ProfSlot {
     int64_t launches = 0;
 };
 
-struct aco_ctx {
-    aco_params prm{};
-    int n = 0, P = 0, m = 0, T = 0, k = 0, nn_ls = 0, device = 0;
+// One instance of a context: host copies needed after creation, and the
+// device problem/state structures pointing into the instance's arena.
+struct InstanceData {
+    int n = 0, P = 0, T = 0;
+    std::vector<float> h_dist;
+    std::vector<int> h_dem;
+    int Q = 0;
+    float D = INFINITY, svc = 0.0f;
+    uint64_t seed = 0;
     DevProblem dp{};
     DevState ds{};
+    void *arena = nullptr;
+};
+
+struct aco_ctx {
+    aco_params prm{};
+    int m = 0, device = 0;
+    std::vector<InstanceData> inst;
+    DevBatch bt{};
     std::vector<void *> allocs;
     hipStream_t stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     bool inited = false;
     float tick_to_s = 0.0f;
-    std::vector<float> h_dist;
-    std::vector<int> h_dem;
-    int Q = 0;
-    float D = INFINITY, svc = 0.0f;
     bool profiling = false;
     std::map<std::string, ProfSlot> prof;
 };
@@ -79,29 +90,61 @@ struct aco_ctx {
             return set_error(ACO_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-template <typename T>
-static hipError_t dev_alloc(aco_ctx *c, T **p, size_t count)
-{
-    void *q = nullptr;
-    hipError_t e = hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T));
-    if (e == hipSuccess) {
-        c->allocs.push_back(q);
-        *p = static_cast<T *>(q);
+// Bump allocator over one device allocation (256-byte aligned pieces).
+struct Carver {
+    unsigned char *base;
+    size_t off = 0;
+    template <typename T>
+    T *take(size_t count)
+    {
+        T *p = base ? reinterpret_cast<T *>(base + off) : nullptr;
+        off = (off + std::max<size_t>(count, 1) * sizeof(T) + 255) & ~size_t(255);
+        return p;
     }
-    return e;
-}
+};
 
-// upload an n x n host matrix into an n x P device matrix
-static hipError_t upload_pitched(aco_ctx *c, float *dst, const float *src)
+// Lays out one instance's device buffers; with base == nullptr only sizes.
+static size_t carve_instance(InstanceData &I, int m, int k, int nnP, int ras, int bsf_cap,
+                             unsigned char *base)
 {
-    return hipMemcpy2DAsync(dst, (size_t)c->P * 4, src, (size_t)c->n * 4, (size_t)c->n * 4, c->n,
-                            hipMemcpyHostToDevice, c->stream);
-}
-
-static hipError_t download_pitched(aco_ctx *c, float *dst, const float *src)
-{
-    return hipMemcpy2DAsync(dst, (size_t)c->n * 4, src, (size_t)c->P * 4, (size_t)c->n * 4, c->n,
-                            hipMemcpyDeviceToHost, c->stream);
+    Carver c{base};
+    const int n = I.n, P = I.P, T = I.T;
+    DevProblem &dp = I.dp;
+    DevState &ds = I.ds;
+    float *dist = c.take<float>((size_t)n * P);
+    float *heur = c.take<float>((size_t)n * P);
+    float *d0 = c.take<float>(n);
+    int *dem = c.take<int>(n);
+    int *nnk = c.take<int>((size_t)n * std::max(k, 1));
+    float *nnkd = c.take<float>((size_t)n * std::max(k, 1));
+    int *nnls = c.take<int>((size_t)n * nnP);
+    float *nnd = c.take<float>((size_t)n * nnP);
+    int *rank = c.take<int>(n);
+    unsigned short *nnpos = c.take<unsigned short>((size_t)n * n);
+    int *r64 = c.take<int>(kLanes);
+    int *r64i = c.take<int>(kLanes);
+    ds.tau = c.take<float>((size_t)n * P);
+    ds.tinfo = c.take<float>((size_t)n * P);
+    ds.tiknn = c.take<float>((size_t)n * std::max(k, 1));
+    ds.tours = c.take<int>((size_t)(m + 1) * T);
+    ds.sizes = c.take<int>(m + 1);
+    ds.lens = c.take<float>(m + 1);
+    ds.pm = c.take<int>((size_t)m * kLanes);
+    ds.ctrl = c.take<int>(kCtrlWords);
+    ds.bsf = c.take<BsfRecord>(bsf_cap);
+    ds.elite = c.take<int>(ras);
+    ds.succ = c.take<int>((size_t)ras * n);
+    ds.firstf = c.take<unsigned char>((size_t)ras * P);
+    ds.delta = c.take<float>(ras);
+    ds.ant_evals = c.take<long long>(m);
+    ds.ant_moves = c.take<int>(m);
+    ds.ant_fb = c.take<int>(m);
+    ds.totals = c.take<long long>(kTotWords);
+    ds.bsf_cap = bsf_cap;
+    dp.dist = dist; dp.heur = heur; dp.d0 = d0; dp.dem = dem; dp.nnk = nnk; dp.nnkd = nnkd;
+    dp.nnls = nnls; dp.nnd = nnd; dp.rank = rank; dp.rank64 = r64; dp.rank64inv = r64i;
+    dp.nnpos = nnpos;
+    return c.off;
 }
 
 template <typename F>
@@ -122,37 +165,34 @@ static hipError_t timed(aco_ctx *c, const char *name, F &&fn)
 
 static int run_construct(aco_ctx *c)
 {
-    HIP_TRY(timed(c, "construct", [&] {
-        return launch_construct(c->dp, c->ds, c->prm.selection, c->prm.rng, c->prm.seed, c->stream);
-    }));
+    HIP_TRY(timed(c, "construct",
+                  [&] { return launch_construct(c->bt, c->prm.selection, c->prm.rng, c->stream); }));
     return ACO_OK;
 }
 
 static int run_local_search(aco_ctx *c)
 {
     if (!c->prm.ls_flags) return ACO_OK;
-    HIP_TRY(timed(c, "local_search",
-                  [&] { return launch_local_search(c->dp, c->ds, c->prm.ls_flags, c->stream); }));
+    HIP_TRY(timed(c, "local_search", [&] { return launch_local_search(c->bt, c->prm.ls_flags, c->stream); }));
     return ACO_OK;
 }
 
 static int run_statistics(aco_ctx *c)
 {
-    HIP_TRY(timed(c, "statistics",
-                  [&] { return launch_statistics(c->dp, c->ds, c->tick_to_s, c->stream); }));
+    HIP_TRY(timed(c, "statistics", [&] { return launch_statistics(c->bt, c->tick_to_s, c->stream); }));
     return ACO_OK;
 }
 
 static int run_gather(aco_ctx *c)
 {
-    if (c->k <= 0) return ACO_OK;
-    HIP_TRY(timed(c, "gather", [&] { return launch_gather(c->dp, c->ds, c->stream); }));
+    if (c->bt.max_nk <= 0) return ACO_OK;
+    HIP_TRY(timed(c, "gather", [&] { return launch_gather(c->bt, c->stream); }));
     return ACO_OK;
 }
 
 static int run_pheromone(aco_ctx *c)
 {
-    HIP_TRY(timed(c, "pheromone", [&] { return launch_pheromone_update(c->dp, c->ds, c->stream); }));
+    HIP_TRY(timed(c, "pheromone", [&] { return launch_pheromone_update(c->bt, c->stream); }));
     return run_gather(c);
 }
 
@@ -170,6 +210,81 @@ static int need(aco_ctx *c, bool inited)
     if (!c) return set_error(ACO_ERR_ARG, "null context");
     if (inited && !c->inited) return set_error(ACO_ERR_STATE, "aco_init has not been called");
     HIP_TRY(hipSetDevice(c->device));
+    return ACO_OK;
+}
+
+static int need_inst(aco_ctx *c, int b, bool inited)
+{
+    int rc = need(c, inited);
+    if (rc) return rc;
+    if (b < 0 || b >= (int)c->inst.size()) return set_error(ACO_ERR_ARG, "instance index out of range");
+    return ACO_OK;
+}
+
+// Parameter checks shared by every instance of a context.
+static int check_params(aco_params &prm, int min_n)
+{
+    if (prm.bsf_capacity <= 0) prm.bsf_capacity = 4096;
+    if (prm.ras_ranks < 1 || prm.ras_ranks > kMaxRas) return set_error(ACO_ERR_ARG, "ras_ranks must be in [1,16]");
+    if (!(prm.rho > 0.0f && prm.rho <= 1.0f)) return set_error(ACO_ERR_ARG, "rho must be in (0,1]");
+    if (!isfinite(prm.alpha) || !isfinite(prm.beta)) return set_error(ACO_ERR_ARG, "alpha/beta not finite");
+    if (prm.selection != ACO_SEL_IROULETTE && prm.selection != ACO_SEL_KNN)
+        return set_error(ACO_ERR_ARG, "unknown selection rule");
+    if (prm.rng != ACO_RNG_PARK_MILLER && prm.rng != ACO_RNG_PHILOX) return set_error(ACO_ERR_ARG, "unknown rng");
+    if (prm.selection == ACO_SEL_IROULETTE && prm.rng != ACO_RNG_PARK_MILLER)
+        return set_error(ACO_ERR_ARG, "iroulette draws from Park-Miller lane streams");
+    if (prm.selection == ACO_SEL_KNN && (prm.k < 1 || prm.k > kLanes || prm.k > min_n - 2))
+        return set_error(ACO_ERR_ARG, "k must be in [1, min(64, n-2)] for roulette_knn");
+    if (prm.selection != ACO_SEL_KNN) prm.k = 0;
+    if (prm.nn_ls < 0 || prm.nn_ls > min_n - 2) return set_error(ACO_ERR_ARG, "nn_ls must be <= n-2");
+    if (prm.ls_flags & ~7) return set_error(ACO_ERR_ARG, "unknown local search flag");
+    if (prm.tau0_rule != ACO_TAU0_FIRST_COLONY && prm.tau0_rule != ACO_TAU0_NN_TOUR)
+        return set_error(ACO_ERR_ARG, "unknown tau0 rule");
+    return ACO_OK;
+}
+
+// Instance checks: finite, non-negative, symmetric distances; every
+// customer servable by a vehicle of its own (otherwise construction could
+// never visit it: kernel.cl:438-479 would not terminate).
+static int check_instance(const aco_problem *pb, int width, std::vector<int> &nnl)
+{
+    const int n = pb->n;
+    if (n < 3 || n > kMaxNodes) return set_error(ACO_ERR_ARG, "aco_create: need 3 <= n <= 4096");
+    if (!pb->dist || !pb->demand) return set_error(ACO_ERR_ARG, "aco_create: dist and demand required");
+    if (pb->capacity <= 0) return set_error(ACO_ERR_ARG, "capacity must be positive");
+    if (isnan(pb->max_dist) || !(pb->service_time >= 0.0f) || !isfinite(pb->service_time))
+        return set_error(ACO_ERR_ARG, "bad max_dist or service_time");
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            const float d = pb->dist[(size_t)i * n + j];
+            if (!isfinite(d) || d < 0.0f) return set_error(ACO_ERR_ARG, "distances must be finite and >= 0");
+            if (d != pb->dist[(size_t)j * n + i]) return set_error(ACO_ERR_ARG, "distance matrix is not symmetric");
+        }
+    for (int i = 0; i < n; ++i)
+        if (pb->demand[i] < 0) return set_error(ACO_ERR_ARG, "negative demand");
+    for (int i = 1; i < n; ++i) {
+        if (pb->demand[i] > pb->capacity)
+            return set_error(ACO_ERR_INFEASIBLE, "customer " + std::to_string(i) + " demand exceeds capacity");
+        float t = 0.0f + (pb->dist[i] + pb->service_time);
+        t = t + pb->dist[(size_t)i * n];
+        if (!(t <= pb->max_dist))
+            return set_error(ACO_ERR_INFEASIBLE, "customer " + std::to_string(i) + " beyond max_dist");
+    }
+    nnl.assign((size_t)n * width, 0);
+    if (pb->nn_list) {
+        if (pb->nn_width < width) return set_error(ACO_ERR_ARG, "nn_width smaller than max(k, nn_ls)");
+        for (int i = 0; i < n; ++i)
+            for (int h = 0; h < width; ++h) {
+                const int v = pb->nn_list[(size_t)i * pb->nn_width + h];
+                if (v < 1 || v >= n || v == i) return set_error(ACO_ERR_ARG, "nn_list entry out of range");
+                if (h > 0 && pb->dist[(size_t)i * n + v] <
+                                 pb->dist[(size_t)i * n + nnl[(size_t)i * width + h - 1]])
+                    return set_error(ACO_ERR_ARG, "nn_list rows must be sorted by distance");
+                nnl[(size_t)i * width + h] = v;
+            }
+    } else {
+        nn_lists(n, pb->dist, width, 0, nnl.data());
+    }
     return ACO_OK;
 }
 
@@ -197,72 +312,33 @@ void aco_default_params(aco_params *p)
     p->use_graph = 1;
 }
 
-int aco_create(const aco_problem *pb, const aco_params *prm_in, aco_ctx **out)
+int aco_create(const aco_problem *pb, const aco_params *prm, aco_ctx **out)
 {
-    if (!pb || !prm_in || !out) return set_error(ACO_ERR_ARG, "aco_create: null argument");
+    return aco_create_batch(pb, 1, prm, out);
+}
+
+int aco_create_batch(const aco_problem *pbs, int32_t count, const aco_params *prm_in, aco_ctx **out)
+{
+    if (!pbs || !prm_in || !out || count < 1) return set_error(ACO_ERR_ARG, "aco_create: null argument");
     *out = nullptr;
     aco_params prm = *prm_in;
-    const int n = pb->n;
-    if (n < 3 || n > kMaxNodes) return set_error(ACO_ERR_ARG, "aco_create: need 3 <= n <= 4096");
-    if (!pb->dist || !pb->demand) return set_error(ACO_ERR_ARG, "aco_create: dist and demand required");
-    if (prm.n_ants <= 0) prm.n_ants = n;
-    if (prm.nn_ls <= 0) prm.nn_ls = n - 2;
-    if (prm.bsf_capacity <= 0) prm.bsf_capacity = 4096;
-    if (prm.ras_ranks < 1 || prm.ras_ranks > kMaxRas) return set_error(ACO_ERR_ARG, "ras_ranks must be in [1,16]");
-    if (!(prm.rho > 0.0f && prm.rho <= 1.0f)) return set_error(ACO_ERR_ARG, "rho must be in (0,1]");
-    if (!isfinite(prm.alpha) || !isfinite(prm.beta)) return set_error(ACO_ERR_ARG, "alpha/beta not finite");
-    if (prm.selection != ACO_SEL_IROULETTE && prm.selection != ACO_SEL_KNN)
-        return set_error(ACO_ERR_ARG, "unknown selection rule");
-    if (prm.rng != ACO_RNG_PARK_MILLER && prm.rng != ACO_RNG_PHILOX) return set_error(ACO_ERR_ARG, "unknown rng");
-    if (prm.selection == ACO_SEL_IROULETTE && prm.rng != ACO_RNG_PARK_MILLER)
-        return set_error(ACO_ERR_ARG, "iroulette draws from Park-Miller lane streams");
-    if (prm.selection == ACO_SEL_KNN && (prm.k < 1 || prm.k > kLanes || prm.k > n - 2))
-        return set_error(ACO_ERR_ARG, "k must be in [1, min(64, n-2)] for roulette_knn");
-    if (prm.selection != ACO_SEL_KNN) prm.k = 0;
-    if (prm.nn_ls > n - 2) return set_error(ACO_ERR_ARG, "nn_ls must be <= n-2");
-    if (prm.ls_flags & ~7) return set_error(ACO_ERR_ARG, "unknown local search flag");
-    if (prm.tau0_rule != ACO_TAU0_FIRST_COLONY && prm.tau0_rule != ACO_TAU0_NN_TOUR)
-        return set_error(ACO_ERR_ARG, "unknown tau0 rule");
-    if (pb->capacity <= 0) return set_error(ACO_ERR_ARG, "capacity must be positive");
-    if (isnan(pb->max_dist) || !(pb->service_time >= 0.0f) || !isfinite(pb->service_time))
-        return set_error(ACO_ERR_ARG, "bad max_dist or service_time");
-
-    // instance checks: finite, non-negative, symmetric distances; every
-    // customer servable by a vehicle of its own (otherwise construction
-    // could never visit it: kernel.cl:438-479 would not terminate)
-    for (int i = 0; i < n; ++i)
-        for (int j = 0; j < n; ++j) {
-            const float d = pb->dist[(size_t)i * n + j];
-            if (!isfinite(d) || d < 0.0f) return set_error(ACO_ERR_ARG, "distances must be finite and >= 0");
-            if (d != pb->dist[(size_t)j * n + i]) return set_error(ACO_ERR_ARG, "distance matrix is not symmetric");
-        }
-    for (int i = 0; i < n; ++i)
-        if (pb->demand[i] < 0) return set_error(ACO_ERR_ARG, "negative demand");
-    for (int i = 1; i < n; ++i) {
-        if (pb->demand[i] > pb->capacity)
-            return set_error(ACO_ERR_INFEASIBLE, "customer " + std::to_string(i) + " demand exceeds capacity");
-        float t = 0.0f + (pb->dist[i] + pb->service_time);
-        t = t + pb->dist[(size_t)i * n];
-        if (!(t <= pb->max_dist))
-            return set_error(ACO_ERR_INFEASIBLE, "customer " + std::to_string(i) + " beyond max_dist");
+    int min_n = INT32_MAX, max_n = 0;
+    for (int b = 0; b < count; ++b) {
+        min_n = std::min(min_n, pbs[b].n);
+        max_n = std::max(max_n, pbs[b].n);
     }
-
-    // neighbour rows: given (checked sorted) or built with the reference order
-    const int width = std::max(prm.k, prm.nn_ls);
-    std::vector<int> nnl((size_t)n * width);
-    if (pb->nn_list) {
-        if (pb->nn_width < width) return set_error(ACO_ERR_ARG, "nn_width smaller than max(k, nn_ls)");
-        for (int i = 0; i < n; ++i)
-            for (int h = 0; h < width; ++h) {
-                const int v = pb->nn_list[(size_t)i * pb->nn_width + h];
-                if (v < 1 || v >= n || v == i) return set_error(ACO_ERR_ARG, "nn_list entry out of range");
-                if (h > 0 && pb->dist[(size_t)i * n + v] <
-                                 pb->dist[(size_t)i * n + nnl[(size_t)i * width + h - 1]])
-                    return set_error(ACO_ERR_ARG, "nn_list rows must be sorted by distance");
-                nnl[(size_t)i * width + h] = v;
-            }
-    } else {
-        nn_lists(n, pb->dist, width, 0, nnl.data());
+    if (prm.n_ants <= 0) {
+        if (count > 1) return set_error(ACO_ERR_ARG, "n_ants must be given for a batch");
+        prm.n_ants = pbs[0].n;
+    }
+    int rc = check_params(prm, min_n);
+    if (rc) return rc;
+    std::vector<std::vector<int>> nnl(count);
+    std::vector<int> nn_ls(count);
+    for (int b = 0; b < count; ++b) {
+        nn_ls[b] = prm.nn_ls > 0 ? prm.nn_ls : pbs[b].n - 2;
+        rc = check_instance(&pbs[b], std::max(prm.k, nn_ls[b]), nnl[b]);
+        if (rc) return rc;
     }
 
     int ndev = 0;
@@ -271,17 +347,7 @@ int aco_create(const aco_problem *pb, const aco_params *prm_in, aco_ctx **out)
 
     aco_ctx *c = new aco_ctx();
     c->prm = prm;
-    c->n = n;
-    c->P = (n + 3) & ~3;
     c->m = prm.n_ants;
-    c->T = 2 * n;
-    c->k = prm.k;
-    c->nn_ls = prm.nn_ls;
-    c->h_dist.assign(pb->dist, pb->dist + (size_t)n * n);
-    c->h_dem.assign(pb->demand, pb->demand + n);
-    c->Q = pb->capacity;
-    c->D = pb->max_dist;
-    c->svc = pb->service_time;
     auto fail = [&](int code) {
         aco_destroy(c);
         return code;
@@ -295,8 +361,8 @@ int aco_create(const aco_problem *pb, const aco_params *prm_in, aco_ctx **out)
     CHK(hipGetDevice(&c->device));
     hipDeviceProp_t prop;
     CHK(hipGetDeviceProperties(&prop, c->device));
-    const size_t lds_need = std::max(construct_lds_bytes(n, prm.selection),
-                                     prm.ls_flags ? local_search_lds_bytes(n) : 0);
+    const size_t lds_need = std::max(construct_lds_bytes(max_n, prm.selection),
+                                     prm.ls_flags ? local_search_lds_bytes(max_n) : 0);
     if (lds_need > prop.sharedMemPerBlock)
         return fail(set_error(ACO_ERR_ARG, "instance too large for LDS: needs " + std::to_string(lds_need) +
                                                " bytes, device has " + std::to_string(prop.sharedMemPerBlock)));
@@ -305,88 +371,140 @@ int aco_create(const aco_problem *pb, const aco_params *prm_in, aco_ctx **out)
     c->tick_to_s = rate_khz > 0 ? 1.0f / (1000.0f * (float)rate_khz) : 0.0f;
     CHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
 
-    const int m = c->m, P = c->P, T = c->T, ras = prm.ras_ranks;
-    // host-side derived arrays
-    std::vector<float> heur((size_t)n * n), d0(n);
-    for (size_t e = 0; e < (size_t)n * n; ++e) {
-        const float y = 1.0f / (pb->dist[e] + 0.1f);
-        float h = prm.beta == 2.0f ? y * y : prm.beta == 1.0f ? y : powf(y, prm.beta);
-        if (pb->prior) h = h * pb->prior[e];
-        heur[e] = h;
+    const int m = c->m, ras = prm.ras_ranks, k = prm.k;
+    unsigned long long *clock0 = nullptr;
+    {
+        void *q = nullptr;
+        CHK(hipMalloc(&q, 256));
+        c->allocs.push_back(q);
+        clock0 = static_cast<unsigned long long *>(q);
     }
-    for (int i = 0; i < n; ++i) d0[i] = pb->dist[(size_t)i * n];
-    std::vector<int> rank(n), r64(kLanes), r64inv(kLanes);
-    tree_rank(n, rank.data());
+    std::vector<int> r64(kLanes), r64inv(kLanes);
     tree_rank(kLanes, r64.data());
     for (int l = 0; l < kLanes; ++l) r64inv[r64[l]] = l;
-    std::vector<int> nnk((size_t)n * std::max(c->k, 1)), nnls((size_t)n * c->nn_ls);
-    for (int i = 0; i < n; ++i) {
-        for (int h = 0; h < c->k; ++h) nnk[(size_t)i * c->k + h] = nnl[(size_t)i * width + h];
-        for (int h = 0; h < c->nn_ls; ++h) nnls[(size_t)i * c->nn_ls + h] = nnl[(size_t)i * width + h];
+
+    c->inst.resize(count);
+    for (int b = 0; b < count; ++b) {
+        const aco_problem *pb = &pbs[b];
+        InstanceData &I = c->inst[b];
+        const int n = pb->n, width = std::max(k, nn_ls[b]), nl = nn_ls[b];
+        const int nnP = (nl + 3) & ~3;
+        I.n = n;
+        I.P = (n + 3) & ~3;
+        I.T = 2 * n;
+        I.h_dist.assign(pb->dist, pb->dist + (size_t)n * n);
+        I.h_dem.assign(pb->demand, pb->demand + n);
+        I.Q = pb->capacity;
+        I.D = pb->max_dist;
+        I.svc = pb->service_time;
+        I.seed = prm.seed + (uint64_t)b;
+        const int P = I.P;
+        const size_t bytes = carve_instance(I, m, k, nnP, ras, prm.bsf_capacity, nullptr);
+        CHK(hipMalloc(&I.arena, bytes));
+        c->allocs.push_back(I.arena);
+        CHK(hipMemsetAsync(I.arena, 0, bytes, c->stream));
+        carve_instance(I, m, k, nnP, ras, prm.bsf_capacity, static_cast<unsigned char *>(I.arena));
+        I.ds.clock0 = clock0;
+
+        // host-side derived arrays
+        std::vector<float> heur((size_t)n * n), d0(n);
+        for (size_t e = 0; e < (size_t)n * n; ++e) {
+            const float y = 1.0f / (pb->dist[e] + 0.1f);
+            float h = prm.beta == 2.0f ? y * y : prm.beta == 1.0f ? y : powf(y, prm.beta);
+            if (pb->prior) h = h * pb->prior[e];
+            heur[e] = h;
+        }
+        for (int i = 0; i < n; ++i) d0[i] = pb->dist[(size_t)i * n];
+        std::vector<int> rank(n);
+        tree_rank(n, rank.data());
+        std::vector<int> nnk((size_t)n * std::max(k, 1), 0), nnls((size_t)n * nnP, 0);
+        std::vector<float> nnkdist((size_t)n * std::max(k, 1), 0.0f), nnd((size_t)n * nnP, INFINITY);
+        std::vector<unsigned short> nnpos((size_t)n * n, 0xFFFF);
+        const std::vector<int> &L = nnl[b];
+        for (int i = 0; i < n; ++i) {
+            for (int h = 0; h < k; ++h) {
+                nnk[(size_t)i * k + h] = L[(size_t)i * width + h];
+                nnkdist[(size_t)i * k + h] = pb->dist[(size_t)i * n + L[(size_t)i * width + h]];
+            }
+            for (int h = 0; h < nl; ++h) {
+                const int v = L[(size_t)i * width + h];
+                nnls[(size_t)i * nnP + h] = v;
+                nnd[(size_t)i * nnP + h] = pb->dist[(size_t)i * n + v];
+                nnpos[(size_t)i * n + v] = (unsigned short)h;
+            }
+        }
+        DevProblem &dp = I.dp;
+        auto h2d = [&](void *dst, const void *src, size_t bytes_) {
+            return hipMemcpyAsync(dst, src, bytes_, hipMemcpyHostToDevice, c->stream);
+        };
+        CHK(hipMemcpy2DAsync((void *)dp.dist, (size_t)P * 4, pb->dist, (size_t)n * 4, (size_t)n * 4, n,
+                             hipMemcpyHostToDevice, c->stream));
+        CHK(hipMemcpy2DAsync((void *)dp.heur, (size_t)P * 4, heur.data(), (size_t)n * 4, (size_t)n * 4, n,
+                             hipMemcpyHostToDevice, c->stream));
+        CHK(h2d((void *)dp.d0, d0.data(), (size_t)n * 4));
+        CHK(h2d((void *)dp.dem, pb->demand, (size_t)n * 4));
+        CHK(h2d((void *)dp.nnk, nnk.data(), nnk.size() * 4));
+        CHK(h2d((void *)dp.nnkd, nnkdist.data(), nnkdist.size() * 4));
+        CHK(h2d((void *)dp.nnls, nnls.data(), nnls.size() * 4));
+        CHK(h2d((void *)dp.nnd, nnd.data(), nnd.size() * 4));
+        CHK(h2d((void *)dp.rank, rank.data(), (size_t)n * 4));
+        CHK(h2d((void *)dp.nnpos, nnpos.data(), nnpos.size() * 2));
+        CHK(h2d((void *)dp.rank64, r64.data(), kLanes * 4));
+        CHK(h2d((void *)dp.rank64inv, r64inv.data(), kLanes * 4));
+        CHK(hipStreamSynchronize(c->stream));  // host vectors die at the end of this iteration
+
+        dp.n = n; dp.P = P; dp.k = k; dp.nn_ls = nl; dp.nnP = nnP; dp.m = m; dp.T = I.T; dp.ras = ras;
+        dp.Q = pb->capacity; dp.D = pb->max_dist; dp.svc = pb->service_time;
+        dp.alpha = prm.alpha; dp.rho = prm.rho;
+        dp.has_d = isfinite(pb->max_dist) ? 1 : 0;
+        dp.seed = I.seed;
+        dp.tau0_nn = 0.0f;
+        if (prm.tau0_rule == ACO_TAU0_NN_TOUR) {
+            const float Lnn = nn_tour_length(n, pb->dist, pb->demand, pb->capacity, pb->max_dist,
+                                             pb->service_time);
+            if (!(Lnn > 0.0f)) return fail(set_error(ACO_ERR_INFEASIBLE, "nearest-neighbour tour failed"));
+            dp.tau0_nn = 1.0f / ((float)n * Lnn);
+        }
     }
 
-    DevProblem &dp = c->dp;
-    DevState &ds = c->ds;
-    float *dist = nullptr, *d0d = nullptr, *heurd = nullptr;
-    int *demd = nullptr, *nnkd = nullptr, *nnlsd = nullptr, *rankd = nullptr, *r64d = nullptr, *r64id = nullptr;
-    CHK(dev_alloc(c, &dist, (size_t)n * P));
-    CHK(dev_alloc(c, &heurd, (size_t)n * P));
-    CHK(dev_alloc(c, &d0d, n));
-    CHK(dev_alloc(c, &demd, n));
-    CHK(dev_alloc(c, &nnkd, (size_t)n * std::max(c->k, 1)));
-    CHK(dev_alloc(c, &nnlsd, (size_t)n * c->nn_ls));
-    CHK(dev_alloc(c, &rankd, n));
-    CHK(dev_alloc(c, &r64d, kLanes));
-    CHK(dev_alloc(c, &r64id, kLanes));
-    CHK(dev_alloc(c, &ds.tau, (size_t)n * P));
-    CHK(dev_alloc(c, &ds.tinfo, (size_t)n * P));
-    CHK(dev_alloc(c, &ds.tiknn, (size_t)n * std::max(c->k, 1)));
-    CHK(dev_alloc(c, &ds.tours, (size_t)(m + 1) * T));
-    CHK(dev_alloc(c, &ds.sizes, m + 1));
-    CHK(dev_alloc(c, &ds.lens, m + 1));
-    CHK(dev_alloc(c, &ds.pm, (size_t)m * kLanes));
-    CHK(dev_alloc(c, &ds.ctrl, kCtrlWords));
-    CHK(dev_alloc(c, &ds.bsf, prm.bsf_capacity));
-    CHK(dev_alloc(c, &ds.elite, ras));
-    CHK(dev_alloc(c, &ds.succ, (size_t)ras * n));
-    CHK(dev_alloc(c, &ds.firstf, (size_t)ras * P));
-    CHK(dev_alloc(c, &ds.delta, ras));
-    CHK(dev_alloc(c, &ds.ant_evals, m));
-    CHK(dev_alloc(c, &ds.ant_moves, m));
-    CHK(dev_alloc(c, &ds.ant_fb, m));
-    CHK(dev_alloc(c, &ds.totals, kTotWords));
-    CHK(dev_alloc(c, &ds.clock0, 1));
-    ds.bsf_cap = prm.bsf_capacity;
-
-    // zero everything that padding or first reads may touch
-    CHK(hipMemsetAsync(dist, 0, (size_t)n * P * 4, c->stream));
-    CHK(hipMemsetAsync(heurd, 0, (size_t)n * P * 4, c->stream));
-    CHK(hipMemsetAsync(ds.tau, 0, (size_t)n * P * 4, c->stream));
-    CHK(hipMemsetAsync(ds.tinfo, 0, (size_t)n * P * 4, c->stream));
-    CHK(hipMemsetAsync(ds.tours, 0, (size_t)(m + 1) * T * 4, c->stream));
-    CHK(hipMemsetAsync(ds.sizes, 0, (size_t)(m + 1) * 4, c->stream));
-    CHK(hipMemsetAsync(ds.succ, 0, (size_t)ras * n * 4, c->stream));
-    CHK(hipMemsetAsync(ds.firstf, 0, (size_t)ras * P, c->stream));
-    CHK(hipMemsetAsync(ds.ant_evals, 0, (size_t)m * 8, c->stream));
-    CHK(hipMemsetAsync(ds.ant_moves, 0, (size_t)m * 4, c->stream));
-    CHK(hipMemsetAsync(ds.ant_fb, 0, (size_t)m * 4, c->stream));
-    CHK(upload_pitched(c, dist, pb->dist));
-    CHK(upload_pitched(c, heurd, heur.data()));
-    CHK(hipMemcpyAsync(d0d, d0.data(), (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
-    CHK(hipMemcpyAsync(demd, pb->demand, (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
-    CHK(hipMemcpyAsync(nnkd, nnk.data(), nnk.size() * 4, hipMemcpyHostToDevice, c->stream));
-    CHK(hipMemcpyAsync(nnlsd, nnls.data(), nnls.size() * 4, hipMemcpyHostToDevice, c->stream));
-    CHK(hipMemcpyAsync(rankd, rank.data(), (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
-    CHK(hipMemcpyAsync(r64d, r64.data(), kLanes * 4, hipMemcpyHostToDevice, c->stream));
-    CHK(hipMemcpyAsync(r64id, r64inv.data(), kLanes * 4, hipMemcpyHostToDevice, c->stream));
+    // device tables of the batch
+    std::vector<DevProblem> hp(count);
+    std::vector<DevState> hs(count);
+    std::vector<int> ant_inst((size_t)count * m), ant_base(count);
+    DevBatch &bt = c->bt;
+    bt.B = count;
+    bt.ants = count * m;
+    bt.max_n = bt.max_P = bt.max_nk = 0;
+    bt.has_d = 0;
+    for (int b = 0; b < count; ++b) {
+        hp[b] = c->inst[b].dp;
+        hs[b] = c->inst[b].ds;
+        ant_base[b] = b * m;
+        for (int a = 0; a < m; ++a) ant_inst[(size_t)b * m + a] = b;
+        bt.max_n = std::max(bt.max_n, hp[b].n);
+        bt.max_P = std::max(bt.max_P, hp[b].P);
+        bt.max_nk = std::max(bt.max_nk, hp[b].n * hp[b].k);
+        bt.has_d |= hp[b].has_d;
+    }
+    void *q = nullptr;
+    const size_t tb = sizeof(DevProblem) * count + sizeof(DevState) * count + 4 * ant_inst.size() +
+                      4 * ant_base.size() + 1024;
+    CHK(hipMalloc(&q, tb));
+    c->allocs.push_back(q);
+    Carver cv{static_cast<unsigned char *>(q)};
+    DevProblem *dprobs = cv.take<DevProblem>(count);
+    DevState *dstates = cv.take<DevState>(count);
+    int *dinst = cv.take<int>(ant_inst.size());
+    int *dbase = cv.take<int>(ant_base.size());
+    CHK(hipMemcpyAsync(dprobs, hp.data(), sizeof(DevProblem) * count, hipMemcpyHostToDevice, c->stream));
+    CHK(hipMemcpyAsync(dstates, hs.data(), sizeof(DevState) * count, hipMemcpyHostToDevice, c->stream));
+    CHK(hipMemcpyAsync(dinst, ant_inst.data(), 4 * ant_inst.size(), hipMemcpyHostToDevice, c->stream));
+    CHK(hipMemcpyAsync(dbase, ant_base.data(), 4 * ant_base.size(), hipMemcpyHostToDevice, c->stream));
     CHK(hipStreamSynchronize(c->stream));
-
-    dp.n = n; dp.P = P; dp.k = c->k; dp.nn_ls = c->nn_ls; dp.m = m; dp.T = T; dp.ras = ras;
-    dp.Q = pb->capacity; dp.D = pb->max_dist; dp.svc = pb->service_time;
-    dp.alpha = prm.alpha; dp.rho = prm.rho;
-    dp.has_d = isfinite(pb->max_dist) ? 1 : 0;
-    dp.dist = dist; dp.d0 = d0d; dp.dem = demd; dp.heur = heurd; dp.nnk = nnkd; dp.nnls = nnlsd;
-    dp.rank = rankd; dp.rank64 = r64d; dp.rank64inv = r64id;
+    bt.probs = dprobs;
+    bt.states = dstates;
+    bt.ant_inst = dinst;
+    bt.ant_base = dbase;
 #undef CHK
     *out = c;
     return ACO_OK;
@@ -410,37 +528,43 @@ int aco_destroy(aco_ctx *c)
     return ACO_OK;
 }
 
+int32_t aco_batch_size(aco_ctx *c) { return c ? (int32_t)c->inst.size() : 0; }
+
 int aco_init(aco_ctx *c)
 {
     int rc = need(c, false);
     if (rc) return rc;
     const int m = c->m;
-    std::vector<int32_t> pm((size_t)m * kLanes);
-    for (int a = 0; a < m; ++a)
-        for (int l = 0; l < kLanes; ++l) pm[(size_t)a * kLanes + l] = pm_seed(c->prm.seed, a, l);
     static const int zeros[kCtrlWords] = {0};
     static const long long zeros64[kTotWords] = {0};
-    const float inf = INFINITY;
-    HIP_TRY(hipMemcpyAsync(c->ds.pm, pm.data(), pm.size() * 4, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipMemcpyAsync(c->ds.ctrl, zeros, sizeof(zeros), hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipMemcpyAsync(c->ds.totals, zeros64, sizeof(zeros64), hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipMemcpyAsync(c->ds.lens + m, &inf, 4, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipMemsetAsync(c->ds.sizes + m, 0, 4, c->stream));
-    HIP_TRY(launch_stamp_clock(c->ds, c->stream));
+    static const float inf = INFINITY;
+    std::vector<std::vector<int32_t>> pms(c->inst.size());
+    for (size_t b = 0; b < c->inst.size(); ++b) {
+        InstanceData &I = c->inst[b];
+        std::vector<int32_t> &pm = pms[b];
+        pm.resize((size_t)m * kLanes);
+        for (int a = 0; a < m; ++a)
+            for (int l = 0; l < kLanes; ++l) pm[(size_t)a * kLanes + l] = pm_seed(I.seed, a, l);
+        HIP_TRY(hipMemcpyAsync(I.ds.pm, pm.data(), pm.size() * 4, hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(hipMemcpyAsync(I.ds.ctrl, zeros, sizeof(zeros), hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(hipMemcpyAsync(I.ds.totals, zeros64, sizeof(zeros64), hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(hipMemcpyAsync(I.ds.lens + m, &inf, 4, hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(hipMemsetAsync(I.ds.sizes + m, 0, 4, c->stream));
+    }
+    HIP_TRY(launch_stamp_clock(c->bt, c->stream));
     if (c->prm.tau0_rule == ACO_TAU0_FIRST_COLONY) {
         // g_aco.cpp:79-92: colony on tau = 0.5, then tau0 = 1/(rho * L_bsf)
-        HIP_TRY(launch_pheromone_init(c->dp, c->ds, 0.5f, 0, c->stream));
+        HIP_TRY(launch_pheromone_init(c->bt, kTau0Arg, 0.5f, c->stream));
         if ((rc = run_gather(c))) return rc;
         if ((rc = run_construct(c))) return rc;
         if ((rc = run_statistics(c))) return rc;
-        HIP_TRY(launch_pheromone_init(c->dp, c->ds, 0.0f, 1, c->stream));
+        HIP_TRY(launch_pheromone_init(c->bt, kTau0Bsf, 0.0f, c->stream));
         if ((rc = run_gather(c))) return rc;
     } else {
-        const float L = nn_tour_length(c->n, c->h_dist.data(), c->h_dem.data(), c->Q, c->D, c->svc);
-        if (!(L > 0.0f)) return set_error(ACO_ERR_INFEASIBLE, "nearest-neighbour tour failed");
-        HIP_TRY(launch_pheromone_init(c->dp, c->ds, 1.0f / ((float)c->n * L), 0, c->stream));
+        HIP_TRY(launch_pheromone_init(c->bt, kTau0Nn, 0.0f, c->stream));
         if ((rc = run_gather(c))) return rc;
     }
+    HIP_TRY(hipStreamSynchronize(c->stream));  // pm staging vectors go out of scope
     c->inited = true;
     return ACO_OK;
 }
@@ -474,7 +598,7 @@ int aco_pheromone_init(aco_ctx *c, float tau0)
     int rc = need(c, false);
     if (rc) return rc;
     if (!(tau0 >= 0.0f) || !isfinite(tau0)) return set_error(ACO_ERR_ARG, "tau0 must be finite and >= 0");
-    HIP_TRY(launch_pheromone_init(c->dp, c->ds, tau0, 0, c->stream));
+    HIP_TRY(launch_pheromone_init(c->bt, kTau0Arg, tau0, c->stream));
     return run_gather(c);
 }
 
@@ -523,117 +647,151 @@ int aco_get_iteration(aco_ctx *c, int32_t *it)
     int rc = need(c, false);
     if (rc) return rc;
     if (!it) return set_error(ACO_ERR_ARG, "null output");
-    HIP_TRY(hipMemcpyAsync(it, c->ds.ctrl + kCtrlIter, 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(it, c->inst[0].ds.ctrl + kCtrlIter, 4, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return ACO_OK;
 }
 
-int aco_get_best(aco_ctx *c, int32_t *tour, int32_t cap, int32_t *size, float *len)
+int aco_get_best_at(aco_ctx *c, int32_t b, int32_t *tour, int32_t cap, int32_t *size, float *len)
 {
-    int rc = need(c, true);
+    int rc = need_inst(c, b, true);
     if (rc) return rc;
+    const InstanceData &I = c->inst[b];
     int sz = 0;
     float L = 0.0f;
-    HIP_TRY(hipMemcpyAsync(&sz, c->ds.sizes + c->m, 4, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipMemcpyAsync(&L, c->ds.lens + c->m, 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(&sz, I.ds.sizes + c->m, 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(&L, I.ds.lens + c->m, 4, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     if (size) *size = sz;
     if (len) *len = L;
     if (tour) {
         if (cap < sz) return set_error(ACO_ERR_ARG, "tour buffer too small");
-        HIP_TRY(hipMemcpyAsync(tour, c->ds.tours + (size_t)c->m * c->T, (size_t)sz * 4,
+        HIP_TRY(hipMemcpyAsync(tour, I.ds.tours + (size_t)c->m * I.T, (size_t)sz * 4,
                                hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(hipStreamSynchronize(c->stream));
     }
     return ACO_OK;
 }
 
-int aco_get_ants(aco_ctx *c, int32_t *tours, int32_t stride, int32_t *sizes, float *lens)
+int aco_get_best(aco_ctx *c, int32_t *tour, int32_t cap, int32_t *size, float *len)
 {
-    int rc = need(c, false);
+    return aco_get_best_at(c, 0, tour, cap, size, len);
+}
+
+int aco_get_ants_at(aco_ctx *c, int32_t b, int32_t *tours, int32_t stride, int32_t *sizes, float *lens)
+{
+    int rc = need_inst(c, b, false);
     if (rc) return rc;
+    const InstanceData &I = c->inst[b];
     const int m1 = c->m + 1;
     if (tours) {
-        if (stride < c->T) return set_error(ACO_ERR_ARG, "stride must be >= 2n");
-        HIP_TRY(hipMemcpy2DAsync(tours, (size_t)stride * 4, c->ds.tours, (size_t)c->T * 4,
-                                 (size_t)c->T * 4, m1, hipMemcpyDeviceToHost, c->stream));
+        if (stride < I.T) return set_error(ACO_ERR_ARG, "stride must be >= 2n");
+        HIP_TRY(hipMemcpy2DAsync(tours, (size_t)stride * 4, I.ds.tours, (size_t)I.T * 4, (size_t)I.T * 4, m1,
+                                 hipMemcpyDeviceToHost, c->stream));
     }
-    if (sizes) HIP_TRY(hipMemcpyAsync(sizes, c->ds.sizes, (size_t)m1 * 4, hipMemcpyDeviceToHost, c->stream));
-    if (lens) HIP_TRY(hipMemcpyAsync(lens, c->ds.lens, (size_t)m1 * 4, hipMemcpyDeviceToHost, c->stream));
+    if (sizes) HIP_TRY(hipMemcpyAsync(sizes, I.ds.sizes, (size_t)m1 * 4, hipMemcpyDeviceToHost, c->stream));
+    if (lens) HIP_TRY(hipMemcpyAsync(lens, I.ds.lens, (size_t)m1 * 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return ACO_OK;
+}
+
+int aco_get_ants(aco_ctx *c, int32_t *tours, int32_t stride, int32_t *sizes, float *lens)
+{
+    return aco_get_ants_at(c, 0, tours, stride, sizes, lens);
+}
+
+int aco_set_ants_at(aco_ctx *c, int32_t b, const int32_t *tours, int32_t stride, const int32_t *sizes,
+                    const float *lens)
+{
+    int rc = need_inst(c, b, false);
+    if (rc) return rc;
+    if (!tours || !sizes || !lens || stride < 1) return set_error(ACO_ERR_ARG, "null argument");
+    const InstanceData &I = c->inst[b];
+    const int m1 = c->m + 1;
+    for (int a = 0; a < m1; ++a) {
+        const int sz = sizes[a];
+        if (a == c->m && sz == 0) continue;  // empty best-so-far slot
+        if (sz < 3 || sz > I.T || sz > stride) return set_error(ACO_ERR_ARG, "tour size out of range");
+        const int32_t *t = tours + (size_t)a * stride;
+        if (t[0] != 0 || t[sz - 1] != 0) return set_error(ACO_ERR_ARG, "tours must start and end at the depot");
+        for (int q = 0; q < sz; ++q)
+            if (t[q] < 0 || t[q] >= I.n) return set_error(ACO_ERR_ARG, "tour node out of range");
+    }
+    std::vector<int32_t> buf((size_t)m1 * I.T, 0);
+    for (int a = 0; a < m1; ++a) memcpy(&buf[(size_t)a * I.T], tours + (size_t)a * stride, (size_t)sizes[a] * 4);
+    HIP_TRY(hipMemcpyAsync(I.ds.tours, buf.data(), buf.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(I.ds.sizes, sizes, (size_t)m1 * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(I.ds.lens, lens, (size_t)m1 * 4, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return ACO_OK;
 }
 
 int aco_set_ants(aco_ctx *c, const int32_t *tours, int32_t stride, const int32_t *sizes, const float *lens)
 {
-    int rc = need(c, false);
-    if (rc) return rc;
-    if (!tours || !sizes || !lens || stride < 1) return set_error(ACO_ERR_ARG, "null argument");
-    const int m1 = c->m + 1;
-    for (int a = 0; a < m1; ++a) {
-        const int sz = sizes[a];
-        if (a == c->m && sz == 0) continue;  // empty best-so-far slot
-        if (sz < 3 || sz > c->T || sz > stride) return set_error(ACO_ERR_ARG, "tour size out of range");
-        const int32_t *t = tours + (size_t)a * stride;
-        if (t[0] != 0 || t[sz - 1] != 0) return set_error(ACO_ERR_ARG, "tours must start and end at the depot");
-        for (int q = 0; q < sz; ++q)
-            if (t[q] < 0 || t[q] >= c->n) return set_error(ACO_ERR_ARG, "tour node out of range");
-    }
-    std::vector<int32_t> buf((size_t)m1 * c->T, 0);
-    for (int a = 0; a < m1; ++a)
-        memcpy(&buf[(size_t)a * c->T], tours + (size_t)a * stride, (size_t)sizes[a] * 4);
-    HIP_TRY(hipMemcpyAsync(c->ds.tours, buf.data(), buf.size() * 4, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipMemcpyAsync(c->ds.sizes, sizes, (size_t)m1 * 4, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipMemcpyAsync(c->ds.lens, lens, (size_t)m1 * 4, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    return ACO_OK;
+    return aco_set_ants_at(c, 0, tours, stride, sizes, lens);
 }
 
-int aco_get_bsf(aco_ctx *c, aco_bsf_record *rec, int32_t cap, int32_t *count)
+int aco_get_bsf_at(aco_ctx *c, int32_t b, aco_bsf_record *rec, int32_t cap, int32_t *count)
 {
-    int rc = need(c, false);
+    int rc = need_inst(c, b, false);
     if (rc) return rc;
+    const InstanceData &I = c->inst[b];
     int nb = 0;
-    HIP_TRY(hipMemcpyAsync(&nb, c->ds.ctrl + kCtrlNbsf, 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(&nb, I.ds.ctrl + kCtrlNbsf, 4, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     if (count) *count = nb;
     const int k = std::min(nb, cap);
     if (rec && k > 0) {
         static_assert(sizeof(aco_bsf_record) == sizeof(BsfRecord), "record layout");
-        HIP_TRY(hipMemcpyAsync(rec, c->ds.bsf, (size_t)k * sizeof(BsfRecord), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipMemcpyAsync(rec, I.ds.bsf, (size_t)k * sizeof(BsfRecord), hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(hipStreamSynchronize(c->stream));
     }
     return ACO_OK;
 }
 
-int aco_get_pheromone(aco_ctx *c, float *tau, float *ti)
+int aco_get_bsf(aco_ctx *c, aco_bsf_record *rec, int32_t cap, int32_t *count)
 {
-    int rc = need(c, false);
+    return aco_get_bsf_at(c, 0, rec, cap, count);
+}
+
+int aco_get_pheromone_at(aco_ctx *c, int32_t b, float *tau, float *ti)
+{
+    int rc = need_inst(c, b, false);
     if (rc) return rc;
-    if (tau) HIP_TRY(download_pitched(c, tau, c->ds.tau));
-    if (ti) HIP_TRY(download_pitched(c, ti, c->ds.tinfo));
+    const InstanceData &I = c->inst[b];
+    if (tau) HIP_TRY(hipMemcpy2DAsync(tau, (size_t)I.n * 4, I.ds.tau, (size_t)I.P * 4, (size_t)I.n * 4, I.n,
+                                      hipMemcpyDeviceToHost, c->stream));
+    if (ti) HIP_TRY(hipMemcpy2DAsync(ti, (size_t)I.n * 4, I.ds.tinfo, (size_t)I.P * 4, (size_t)I.n * 4, I.n,
+                                     hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return ACO_OK;
 }
 
-int aco_set_pheromone(aco_ctx *c, const float *tau, const float *ti)
+int aco_get_pheromone(aco_ctx *c, float *tau, float *ti) { return aco_get_pheromone_at(c, 0, tau, ti); }
+
+int aco_set_pheromone_at(aco_ctx *c, int32_t b, const float *tau, const float *ti)
 {
-    int rc = need(c, false);
+    int rc = need_inst(c, b, false);
     if (rc) return rc;
     if (!tau || !ti) return set_error(ACO_ERR_ARG, "null argument");
-    HIP_TRY(upload_pitched(c, c->ds.tau, tau));
-    HIP_TRY(upload_pitched(c, c->ds.tinfo, ti));
+    const InstanceData &I = c->inst[b];
+    HIP_TRY(hipMemcpy2DAsync(I.ds.tau, (size_t)I.P * 4, tau, (size_t)I.n * 4, (size_t)I.n * 4, I.n,
+                             hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpy2DAsync(I.ds.tinfo, (size_t)I.P * 4, ti, (size_t)I.n * 4, (size_t)I.n * 4, I.n,
+                             hipMemcpyHostToDevice, c->stream));
     if ((rc = run_gather(c))) return rc;
     HIP_TRY(hipStreamSynchronize(c->stream));
     return ACO_OK;
 }
+
+int aco_set_pheromone(aco_ctx *c, const float *tau, const float *ti) { return aco_set_pheromone_at(c, 0, tau, ti); }
 
 int aco_get_rng_state(aco_ctx *c, int32_t *state)
 {
     int rc = need(c, false);
     if (rc) return rc;
     if (!state) return set_error(ACO_ERR_ARG, "null output");
-    HIP_TRY(hipMemcpyAsync(state, c->ds.pm, (size_t)c->m * kLanes * 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(state, c->inst[0].ds.pm, (size_t)c->m * kLanes * 4, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return ACO_OK;
 }
@@ -643,16 +801,19 @@ int aco_get_counters(aco_ctx *c, aco_counters *out)
     int rc = need(c, false);
     if (rc) return rc;
     if (!out) return set_error(ACO_ERR_ARG, "null output");
-    long long t[kTotWords];
-    int ctrl[kCtrlWords];
-    HIP_TRY(hipMemcpyAsync(t, c->ds.totals, sizeof(t), hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipMemcpyAsync(ctrl, c->ds.ctrl, sizeof(ctrl), hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    out->ls_evals = t[kTotEvals];
-    out->ls_moves = t[kTotMoves];
-    out->fallbacks = t[kTotFallbacks];
-    out->iterations = t[kTotIters];
-    out->bsf_overflow = ctrl[kCtrlOverflow];
+    memset(out, 0, sizeof(*out));
+    for (size_t b = 0; b < c->inst.size(); ++b) {
+        long long t[kTotWords];
+        int ctrl[kCtrlWords];
+        HIP_TRY(hipMemcpyAsync(t, c->inst[b].ds.totals, sizeof(t), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipMemcpyAsync(ctrl, c->inst[b].ds.ctrl, sizeof(ctrl), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        out->ls_evals += t[kTotEvals];
+        out->ls_moves += t[kTotMoves];
+        out->fallbacks += t[kTotFallbacks];
+        if (b == 0) out->iterations = t[kTotIters];
+        out->bsf_overflow += ctrl[kCtrlOverflow];
+    }
     return ACO_OK;
 }
 

@@ -39,15 +39,20 @@ struct BsfRecord {
 };
 
 struct DevProblem {
-    int n, P, k, nn_ls, m, T, ras, Q;
+    int n, P, k, nn_ls, nnP, m, T, ras, Q;
     float D, svc, alpha, rho;
+    float tau0_nn;            // 1/(n * L_nn) for the nearest-neighbour tau0 rule
     int has_d;                // max_dist is finite
+    unsigned long long seed;  // Philox key of this instance
     const float *dist;        // n*P
     const float *d0;          // n, distance to the depot (column 0)
     const int *dem;           // n
     const float *heur;        // n*P
-    const int *nnk;           // n*k
-    const int *nnls;          // n*nn_ls
+    const int *nnk;           // n*k candidate ids
+    const float *nnkd;        // n*k: d(i, nnk[i][c]), read with the ids
+    const int *nnls;          // n*nnP: 2-opt neighbour ids, rows padded to nnP (multiple of 4)
+    const float *nnd;         // n*nnP: d(i, nnls[i][h]), so a row scan reads (id, distance) pairs
+    const unsigned short *nnpos;  // n*n: position of j in i's nn row (0xFFFF if absent)
     const int *rank;          // n, priority order of the I-Roulette reduction
     const int *rank64;        // 64, same for best_solution_phase_0 groups
     const int *rank64inv;     // 64
@@ -71,17 +76,30 @@ struct DevState {
     unsigned long long *clock0;
 };
 
-// ---- host launchers (aco_kernels.hip) ----
-hipError_t launch_construct(const DevProblem &p, const DevState &s, int selection, int rng,
-                            uint64_t seed, hipStream_t st);
-hipError_t launch_local_search(const DevProblem &p, const DevState &s, int flags, hipStream_t st);
-hipError_t launch_statistics(const DevProblem &p, const DevState &s, float tick_to_s,
-                             hipStream_t st);
-hipError_t launch_pheromone_update(const DevProblem &p, const DevState &s, hipStream_t st);
-hipError_t launch_gather(const DevProblem &p, const DevState &s, hipStream_t st);
-hipError_t launch_pheromone_init(const DevProblem &p, const DevState &s, float tau0, int from_bsf,
-                                 hipStream_t st);
-hipError_t launch_stamp_clock(const DevState &s, hipStream_t st);
+// A launch covers every instance of a context: per-instance problem and
+// state live in device tables, construction and local-search blocks map
+// to (instance, ant) through ant_inst / ant_base, the per-instance kernels
+// use blockIdx.y as the instance.
+struct DevBatch {
+    const DevProblem *probs;  // B
+    const DevState *states;   // B
+    const int *ant_inst;      // ants: instance of global ant g
+    const int *ant_base;      // B: first global ant of instance b
+    int B, ants;
+    int max_n, max_P, max_nk;
+    int has_d;                // some instance has a route-length limit
+};
+
+enum { kTau0Arg = 0, kTau0Bsf = 1, kTau0Nn = 2 };
+
+// ---- host launchers (aco_kernels.hip, aco_local_search.hip) ----
+hipError_t launch_construct(const DevBatch &b, int selection, int rng, hipStream_t st);
+hipError_t launch_local_search(const DevBatch &b, int flags, hipStream_t st);
+hipError_t launch_statistics(const DevBatch &b, float tick_to_s, hipStream_t st);
+hipError_t launch_pheromone_update(const DevBatch &b, hipStream_t st);
+hipError_t launch_gather(const DevBatch &b, hipStream_t st);
+hipError_t launch_pheromone_init(const DevBatch &b, int mode, float tau0, hipStream_t st);
+hipError_t launch_stamp_clock(const DevBatch &b, hipStream_t st);
 size_t construct_lds_bytes(int n, int selection);
 size_t local_search_lds_bytes(int n);
 

@@ -156,11 +156,20 @@ __device__ __forceinline__ void tour_close(int *tour, int &tbuf, int step, int l
 // wavefront per ant, lane l draws for nodes l, l+64, ... from its own
 // Park-Miller stream; the reduction's tie order is reproduced by taking the
 // max of (prob, 4095-rank, node) keys.
+// Block -> (instance, ant) of a batched launch.
+#define ANT_PROLOGUE                                        \
+    const int g_ = blockIdx.x;                              \
+    const int inst_ = bt.ant_inst[g_];                      \
+    const DevProblem p = bt.probs[inst_];                   \
+    const DevState s = bt.states[inst_];                    \
+    const int a = g_ - bt.ant_base[inst_];
+
 template <bool HAS_D>
-__global__ __launch_bounds__(64) void k_construct_iroulette(DevProblem p, DevState s)
+__global__ __launch_bounds__(64) void k_construct_iroulette(DevBatch bt)
 {
     extern __shared__ __align__(16) unsigned char smem[];
-    const int n = p.n, lane = threadIdx.x, a = blockIdx.x;
+    ANT_PROLOGUE
+    const int n = p.n, lane = threadIdx.x;
     AntLds L = ant_lds(smem, n, true);
     for (int i = lane; i < n; i += kLanes) {
         L.dem[i] = p.dem[i];
@@ -234,10 +243,12 @@ __global__ __launch_bounds__(64) void k_construct_iroulette(DevProblem p, DevSta
 // last lane with mass when rounding leaves none).  No mass -> full-row
 // greedy pick, and the depot when nothing is feasible.
 template <bool HAS_D, int RNG>
-__global__ __launch_bounds__(64) void k_construct_knn(DevProblem p, DevState s, u64 seed)
+__global__ __launch_bounds__(64) void k_construct_knn(DevBatch bt)
 {
     extern __shared__ __align__(16) unsigned char smem[];
-    const int n = p.n, k = p.k, lane = threadIdx.x, a = blockIdx.x;
+    ANT_PROLOGUE
+    const int n = p.n, k = p.k, lane = threadIdx.x;
+    const u64 seed = p.seed;
     AntLds L = ant_lds(smem, n, false);
     for (int i = lane; i < n; i += kLanes) {
         L.dem[i] = p.dem[i];
@@ -252,18 +263,19 @@ __global__ __launch_bounds__(64) void k_construct_knn(DevProblem p, DevState s, 
     float rdist = 0.0f, tdist = 0.0f;
     while (vcnt < n) {
         step++;
-        const float *drow = p.dist + (size_t)cur * p.P;
         int j = 0;
-        float pj = 0.0f;
+        float pj = 0.0f, dcj = 0.0f;
         if (lane < k) {
+            // candidate id, its total_info and d(cur, id): one contiguous read each
             j = p.nnk[cur * k + lane];
             const float tj = s.tiknn[cur * k + lane];
-            const float dcj = HAS_D ? drow[j] : 0.0f;
+            dcj = p.nnkd[cur * k + lane];
             pj = feasible<HAS_D>(p, L, j, dcj, load, rdist) ? tj : 0.0f;
         }
         const float x = wave_scan_f32(pj, lane);
         const float total = __shfl(x, k - 1);
         int w;
+        float dcw;
         if (total > 0.0f) {
             const float u = RNG == kRngPhilox ? philox_u01(seed, iter, (u32)a, (u32)step)
                                               : pm_ran01(z);
@@ -277,12 +289,14 @@ __global__ __launch_bounds__(64) void k_construct_knn(DevProblem p, DevState s, 
                 wc = 63 - __clzll((long long)m2);
             }
             w = __shfl(j, wc);
+            dcw = __shfl(dcj, wc);
         } else {
+            const float *drow = p.dist + (size_t)cur * p.P;
             w = greedy_pick<HAS_D>(p, L, s.tinfo + (size_t)cur * p.P, drow, load, rdist, lane);
             fb++;
+            dcw = drow[uniform(w)];
         }
         w = uniform(w);
-        const float dcw = drow[w];
         if (w == 0) {
             load = 0;
             rdist = 0.0f;
@@ -310,20 +324,19 @@ __global__ __launch_bounds__(64) void k_construct_knn(DevProblem p, DevState s, 
     }
 }
 
-hipError_t launch_construct(const DevProblem &p, const DevState &s, int selection, int rng,
-                            uint64_t seed, hipStream_t st)
+hipError_t launch_construct(const DevBatch &b, int selection, int rng, hipStream_t st)
 {
-    const size_t lds = construct_lds_bytes(p.n, selection);
-    const dim3 grid(p.m), block(kLanes);
+    const size_t lds = construct_lds_bytes(b.max_n, selection);
+    const dim3 grid(b.ants), block(kLanes);
     if (selection == kSelIRoulette) {
-        if (p.has_d) hipLaunchKernelGGL(k_construct_iroulette<true>, grid, block, lds, st, p, s);
-        else hipLaunchKernelGGL(k_construct_iroulette<false>, grid, block, lds, st, p, s);
+        if (b.has_d) hipLaunchKernelGGL(k_construct_iroulette<true>, grid, block, lds, st, b);
+        else hipLaunchKernelGGL(k_construct_iroulette<false>, grid, block, lds, st, b);
     } else if (rng == kRngPhilox) {
-        if (p.has_d) hipLaunchKernelGGL((k_construct_knn<true, kRngPhilox>), grid, block, lds, st, p, s, (u64)seed);
-        else hipLaunchKernelGGL((k_construct_knn<false, kRngPhilox>), grid, block, lds, st, p, s, (u64)seed);
+        if (b.has_d) hipLaunchKernelGGL((k_construct_knn<true, kRngPhilox>), grid, block, lds, st, b);
+        else hipLaunchKernelGGL((k_construct_knn<false, kRngPhilox>), grid, block, lds, st, b);
     } else {
-        if (p.has_d) hipLaunchKernelGGL((k_construct_knn<true, kRngParkMiller>), grid, block, lds, st, p, s, (u64)seed);
-        else hipLaunchKernelGGL((k_construct_knn<false, kRngParkMiller>), grid, block, lds, st, p, s, (u64)seed);
+        if (b.has_d) hipLaunchKernelGGL((k_construct_knn<true, kRngParkMiller>), grid, block, lds, st, b);
+        else hipLaunchKernelGGL((k_construct_knn<false, kRngParkMiller>), grid, block, lds, st, b);
     }
     return hipGetLastError();
 }
@@ -368,9 +381,10 @@ __device__ long long block_sum_i64(long long v, long long *red)
 // (kernel.cl:546-586) stamped with the device clock; then get_elites
 // (kernel.cl:717-741), the deposit amounts (kernel.cl:767-768) and the
 // elite edge tables the pheromone pass reads; finally the iteration count.
-__global__ __launch_bounds__(kStatThreads) void k_statistics(DevProblem p, DevState s,
-                                                            float tick_to_s)
+__global__ __launch_bounds__(kStatThreads) void k_statistics(DevBatch bt, float tick_to_s)
 {
+    const DevProblem p = bt.probs[blockIdx.x];
+    const DevState s = bt.states[blockIdx.x];
     __shared__ u64 red[kStatWaves + 1];
     __shared__ long long red64[kStatWaves];
     __shared__ int sh_elite[kMaxRas];
@@ -472,10 +486,9 @@ __global__ __launch_bounds__(kStatThreads) void k_statistics(DevProblem p, DevSt
     }
 }
 
-hipError_t launch_statistics(const DevProblem &p, const DevState &s, float tick_to_s,
-                             hipStream_t st)
+hipError_t launch_statistics(const DevBatch &b, float tick_to_s, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_statistics, dim3(1), dim3(kStatThreads), 0, st, p, s, tick_to_s);
+    hipLaunchKernelGGL(k_statistics, dim3(b.B), dim3(kStatThreads), 0, st, b, tick_to_s);
     return hipGetLastError();
 }
 
@@ -488,8 +501,11 @@ hipError_t launch_statistics(const DevProblem &p, const DevState &s, float tick_
 //   t = (1-rho)*tau;  for g in order: if edge (j,h) is in elite g: t += d_g
 //   tau = t;  total_info = t^alpha * heur
 // The elite-order sum is deterministic (the reference's CAS atomics are not).
-__global__ __launch_bounds__(256) void k_pheromone_update(DevProblem p, DevState s, float evap)
+__global__ __launch_bounds__(256) void k_pheromone_update(DevBatch bt)
 {
+    const DevProblem p = bt.probs[blockIdx.y];
+    const DevState s = bt.states[blockIdx.y];
+    const float evap = 1.0f - p.rho;
     const int lane = threadIdx.x & 63;
     const int chunks = (p.P / 4 + 63) / 64;
     const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -538,37 +554,40 @@ __global__ __launch_bounds__(256) void k_pheromone_update(DevProblem p, DevState
     *reinterpret_cast<float4 *>(s.tinfo + off) = make_float4(ti[0], ti[1], ti[2], ti[3]);
 }
 
-hipError_t launch_pheromone_update(const DevProblem &p, const DevState &s, hipStream_t st)
+hipError_t launch_pheromone_update(const DevBatch &b, hipStream_t st)
 {
-    const int chunks = (p.P / 4 + 63) / 64;
-    const int waves = p.n * chunks;
-    hipLaunchKernelGGL(k_pheromone_update, dim3((waves + 3) / 4), dim3(256), 0, st, p, s,
-                       1.0f - p.rho);
+    const int chunks = (b.max_P / 4 + 63) / 64;
+    const int waves = b.max_n * chunks;
+    hipLaunchKernelGGL(k_pheromone_update, dim3((waves + 3) / 4, b.B), dim3(256), 0, st, b);
     return hipGetLastError();
 }
 
 // total_info of each candidate row, contiguous (n x k)
-__global__ __launch_bounds__(256) void k_gather_knn(DevProblem p, DevState s)
+__global__ __launch_bounds__(256) void k_gather_knn(DevBatch bt)
 {
+    const DevProblem p = bt.probs[blockIdx.y];
+    const DevState s = bt.states[blockIdx.y];
     const int idx = blockIdx.x * 256 + threadIdx.x;
     if (idx >= p.n * p.k) return;
     const int i = idx / p.k;
     s.tiknn[idx] = s.tinfo[(size_t)i * p.P + p.nnk[idx]];
 }
 
-hipError_t launch_gather(const DevProblem &p, const DevState &s, hipStream_t st)
+hipError_t launch_gather(const DevBatch &b, hipStream_t st)
 {
-    if (p.k <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_gather_knn, dim3((p.n * p.k + 255) / 256), dim3(256), 0, st, p, s);
+    if (b.max_nk <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gather_knn, dim3((b.max_nk + 255) / 256, b.B), dim3(256), 0, st, b);
     return hipGetLastError();
 }
 
-// pheromone_init (kernel.cl:672-679); from_bsf takes tau0 = 1/(rho * L_bsf)
-// on the device (g_aco.cpp:89) so initialisation needs no host round trip.
-__global__ __launch_bounds__(256) void k_pheromone_init(DevProblem p, DevState s, float tau0,
-                                                        int from_bsf)
+// pheromone_init (kernel.cl:672-679).  mode kTau0Bsf takes tau0 =
+// 1/(rho * L_bsf) on the device (g_aco.cpp:89) so initialisation needs no
+// host round trip; kTau0Nn the instance's 1/(n * L_nn).
+__global__ __launch_bounds__(256) void k_pheromone_init(DevBatch bt, int mode, float tau0)
 {
-    const float t0 = from_bsf ? 1.0f / (p.rho * s.lens[p.m]) : tau0;
+    const DevProblem p = bt.probs[blockIdx.y];
+    const DevState s = bt.states[blockIdx.y];
+    const float t0 = mode == kTau0Bsf ? 1.0f / (p.rho * s.lens[p.m]) : mode == kTau0Nn ? p.tau0_nn : tau0;
     const float ta = p.alpha == 1.0f ? t0 : powf(t0, p.alpha);
     const size_t total = (size_t)p.n * p.P / 4;
     for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < total; q += (size_t)gridDim.x * 256) {
@@ -578,24 +597,23 @@ __global__ __launch_bounds__(256) void k_pheromone_init(DevProblem p, DevState s
     }
 }
 
-hipError_t launch_pheromone_init(const DevProblem &p, const DevState &s, float tau0, int from_bsf,
-                                 hipStream_t st)
+hipError_t launch_pheromone_init(const DevBatch &b, int mode, float tau0, hipStream_t st)
 {
-    const size_t total = (size_t)p.n * p.P / 4;
+    const size_t total = (size_t)b.max_n * b.max_P / 4;
     int blocks = (int)((total + 255) / 256);
-    if (blocks > 4096) blocks = 4096;
-    hipLaunchKernelGGL(k_pheromone_init, dim3(blocks), dim3(256), 0, st, p, s, tau0, from_bsf);
+    if (blocks > 1024) blocks = 1024;
+    hipLaunchKernelGGL(k_pheromone_init, dim3(blocks, b.B), dim3(256), 0, st, b, mode, tau0);
     return hipGetLastError();
 }
 
-__global__ void k_stamp_clock(DevState s)
+__global__ void k_stamp_clock(DevBatch bt)
 {
-    if (threadIdx.x == 0) *s.clock0 = wall_clock64();
+    if (threadIdx.x == 0) *bt.states[0].clock0 = wall_clock64();
 }
 
-hipError_t launch_stamp_clock(const DevState &s, hipStream_t st)
+hipError_t launch_stamp_clock(const DevBatch &b, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_stamp_clock, dim3(1), dim3(64), 0, st, s);
+    hipLaunchKernelGGL(k_stamp_clock, dim3(1), dim3(64), 0, st, b);
     return hipGetLastError();
 }
 

@@ -103,13 +103,21 @@ class Instance:
 
 
 class GpuACO:
-    """g_ACO (cvrp_ocl/gpu/g_aco.h:16-37) on libcvrpaco.so."""
+    """g_ACO (cvrp_ocl/gpu/g_aco.h:16-37) on libcvrpaco.so.
 
-    def __init__(self, inst: Instance, *, alpha=1.0, beta=2.0, rho=0.1, n_ants=0, ras_ranks=6,
+    ``inst`` is one Instance, or a list of Instances solved together in one
+    context (aco_create_batch: every launch covers all of them, instance b
+    seeded with seed + b).  Accessors take the instance index ``b``."""
+
+    def __init__(self, inst, *, alpha=1.0, beta=2.0, rho=0.1, n_ants=0, ras_ranks=6,
                  k=0, nn_ls=0, selection=_lib.SEL_IROULETTE, rng=_lib.RNG_PARK_MILLER,
                  ls_flags=0, tau0_rule=_lib.TAU0_FIRST_COLONY, seed=1, use_graph=True,
                  bsf_capacity=4096, prior=None, nn_list=None):
-        self.inst = inst
+        insts = list(inst) if isinstance(inst, (list, tuple)) else [inst]
+        priors = prior if isinstance(prior, (list, tuple)) else [prior] * len(insts)
+        nnls = nn_list if isinstance(nn_list, (list, tuple)) else [nn_list] * len(insts)
+        self.insts = insts
+        self.inst = insts[0]
         p = _lib.AcoParams()
         lib.aco_default_params(C.byref(p))
         p.alpha, p.beta, p.rho = alpha, beta, rho
@@ -117,19 +125,25 @@ class GpuACO:
         p.selection, p.rng, p.ls_flags, p.tau0_rule = selection, rng, ls_flags, tau0_rule
         p.seed, p.use_graph, p.bsf_capacity = seed, int(bool(use_graph)), bsf_capacity
         self.params = p
-        self._dist = np.ascontiguousarray(inst.dist, np.float32)
-        self._dem = np.ascontiguousarray(inst.demand, np.int32)
-        self._prior = None if prior is None else np.ascontiguousarray(prior, np.float32)
-        self._nn = None if nn_list is None else np.ascontiguousarray(nn_list, np.int32)
-        prob = _lib.AcoProblem(inst.n, _ptr(self._dist), _ptr(self._dem), inst.capacity,
-                               inst.max_dist, inst.service_time, _ptr(self._nn),
-                               0 if self._nn is None else self._nn.shape[1], _ptr(self._prior))
+        self._keep = []
+        probs = (_lib.AcoProblem * len(insts))()
+        for b, it in enumerate(insts):
+            d = np.ascontiguousarray(it.dist, np.float32)
+            dem = np.ascontiguousarray(it.demand, np.int32)
+            pr = None if priors[b] is None else np.ascontiguousarray(priors[b], np.float32)
+            nn = None if nnls[b] is None else np.ascontiguousarray(nnls[b], np.int32)
+            self._keep += [d, dem, pr, nn]
+            probs[b] = _lib.AcoProblem(it.n, _ptr(d), _ptr(dem), it.capacity, it.max_dist,
+                                       it.service_time, _ptr(nn), 0 if nn is None else nn.shape[1],
+                                       _ptr(pr))
         h = C.c_void_p()
-        check(lib.aco_create(C.byref(prob), C.byref(p), C.byref(h)), "aco_create")
+        check(lib.aco_create_batch(probs, len(insts), C.byref(p), C.byref(h)), "aco_create")
+        self._keep = None  # inputs are copied at creation
         self.h = h
-        self.n = inst.n
-        self.m = n_ants if n_ants > 0 else inst.n
-        self.T = 2 * inst.n
+        self.B = len(insts)
+        self.n = insts[0].n
+        self.m = n_ants if n_ants > 0 else insts[0].n
+        self.T = 2 * insts[0].n
 
     # -- lifetime ----------------------------------------------------------
     def close(self):
@@ -156,10 +170,10 @@ class GpuACO:
     def run(self, iterations: int):
         check(lib.aco_run(self.h, iterations), "aco_run")
 
-    def exit_aco(self):
+    def exit_aco(self, b: int = 0):
         """g_aco.cpp:95-117: best-so-far ant and its record trajectory."""
-        tour, length = self.best()
-        return tour, length, self.bsf_records()
+        tour, length = self.best(b)
+        return tour, length, self.bsf_records(b)
 
     def construct_solutions(self):
         check(lib.aco_construct(self.h), "aco_construct")
@@ -186,44 +200,52 @@ class GpuACO:
         check(lib.aco_get_iteration(self.h, C.byref(it)), "aco_get_iteration")
         return it.value
 
-    def best(self):
-        t = np.empty(self.T, np.int32)
+    def _T(self, b):
+        return 2 * self.insts[b].n
+
+    def best(self, b: int = 0):
+        T = self._T(b)
+        t = np.empty(T, np.int32)
         sz = C.c_int32()
         L = C.c_float()
-        check(lib.aco_get_best(self.h, _ptr(t), self.T, C.byref(sz), C.byref(L)), "aco_get_best")
+        check(lib.aco_get_best_at(self.h, b, _ptr(t), T, C.byref(sz), C.byref(L)), "aco_get_best")
         return t[: sz.value].copy(), L.value
 
-    def ants(self):
-        tours = np.zeros((self.m + 1, self.T), np.int32)
+    def ants(self, b: int = 0):
+        T = self._T(b)
+        tours = np.zeros((self.m + 1, T), np.int32)
         sizes = np.zeros(self.m + 1, np.int32)
         lens = np.zeros(self.m + 1, np.float32)
-        check(lib.aco_get_ants(self.h, _ptr(tours), self.T, _ptr(sizes), _ptr(lens)), "aco_get_ants")
+        check(lib.aco_get_ants_at(self.h, b, _ptr(tours), T, _ptr(sizes), _ptr(lens)),
+              "aco_get_ants")
         return tours, sizes, lens
 
-    def set_ants(self, tours, sizes, lens):
+    def set_ants(self, tours, sizes, lens, b: int = 0):
         tours = np.ascontiguousarray(tours, np.int32)
         sizes = np.ascontiguousarray(sizes, np.int32)
         lens = np.ascontiguousarray(lens, np.float32)
-        check(lib.aco_set_ants(self.h, _ptr(tours), tours.shape[1], _ptr(sizes), _ptr(lens)),
+        check(lib.aco_set_ants_at(self.h, b, _ptr(tours), tours.shape[1], _ptr(sizes), _ptr(lens)),
               "aco_set_ants")
 
-    def bsf_records(self):
+    def bsf_records(self, b: int = 0):
         cnt = C.c_int32()
-        check(lib.aco_get_bsf(self.h, None, 0, C.byref(cnt)), "aco_get_bsf")
+        check(lib.aco_get_bsf_at(self.h, b, None, 0, C.byref(cnt)), "aco_get_bsf")
         rec = (_lib.AcoBsfRecord * max(cnt.value, 1))()
-        check(lib.aco_get_bsf(self.h, C.cast(rec, C.c_void_p), cnt.value, C.byref(cnt)), "aco_get_bsf")
+        check(lib.aco_get_bsf_at(self.h, b, C.cast(rec, C.c_void_p), cnt.value, C.byref(cnt)),
+              "aco_get_bsf")
         return [(r.iter, r.length, r.time) for r in rec[: cnt.value]]
 
-    def pheromone(self):
-        tau = np.empty((self.n, self.n), np.float32)
-        ti = np.empty((self.n, self.n), np.float32)
-        check(lib.aco_get_pheromone(self.h, _ptr(tau), _ptr(ti)), "aco_get_pheromone")
+    def pheromone(self, b: int = 0):
+        n = self.insts[b].n
+        tau = np.empty((n, n), np.float32)
+        ti = np.empty((n, n), np.float32)
+        check(lib.aco_get_pheromone_at(self.h, b, _ptr(tau), _ptr(ti)), "aco_get_pheromone")
         return tau, ti
 
-    def set_pheromone(self, tau, ti):
+    def set_pheromone(self, tau, ti, b: int = 0):
         tau = np.ascontiguousarray(tau, np.float32)
         ti = np.ascontiguousarray(ti, np.float32)
-        check(lib.aco_set_pheromone(self.h, _ptr(tau), _ptr(ti)), "aco_set_pheromone")
+        check(lib.aco_set_pheromone_at(self.h, b, _ptr(tau), _ptr(ti)), "aco_set_pheromone")
 
     def rng_state(self):
         out = np.empty((self.m, 64), np.int32)

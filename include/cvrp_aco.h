@@ -104,6 +104,27 @@ int aco_create(const aco_problem *problem, const aco_params *params, aco_ctx **c
 /* g_ACO::~g_ACO (g_aco.cpp:45-61) */
 int aco_destroy(aco_ctx *ctx);
 
+/* Batched colonies (no reference counterpart; the reference solves one
+ * instance per process): count instances of uneven size in one context,
+ * each with params->n_ants ants and its own pheromone, best-so-far and
+ * records; every launch covers all of them.  Instance b uses the seed
+ * params->seed + b.  The *_at accessors take the instance index; the plain
+ * accessors refer to instance 0. */
+int aco_create_batch(const aco_problem *problems, int32_t count, const aco_params *params,
+                     aco_ctx **ctx);
+int32_t aco_batch_size(aco_ctx *ctx);
+int aco_get_best_at(aco_ctx *ctx, int32_t instance, int32_t *tour, int32_t capacity,
+                    int32_t *size, float *length);
+int aco_get_ants_at(aco_ctx *ctx, int32_t instance, int32_t *tours, int32_t stride,
+                    int32_t *sizes, float *lengths);
+int aco_set_ants_at(aco_ctx *ctx, int32_t instance, const int32_t *tours, int32_t stride,
+                    const int32_t *sizes, const float *lengths);
+int aco_get_bsf_at(aco_ctx *ctx, int32_t instance, aco_bsf_record *records, int32_t capacity,
+                   int32_t *count);
+int aco_get_pheromone_at(aco_ctx *ctx, int32_t instance, float *tau, float *total_info);
+int aco_set_pheromone_at(aco_ctx *ctx, int32_t instance, const float *tau,
+                         const float *total_info);
+
 /* g_ACO::init_aco (g_aco.cpp:63-93) */
 int aco_init(aco_ctx *ctx);
 /* g_ACO::run_aco_iteration (g_aco.cpp:122-134): construct -> local search
