@@ -45,7 +45,22 @@ def parse():
                     help="CPU baseline sample budget (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cores)")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run 10 device-to-device copies of MIB MiB (calibrates the "
+                         "FETCH_SIZE / WRITE_SIZE counters against known bytes) and exit")
     return ap.parse_args()
+
+
+def copy_probe(mib: int) -> None:
+    import torch
+    nbytes = mib << 20
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(10):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": nbytes, "copy_probe_bytes_written": nbytes,
+                      "launches": 10}))
 
 
 def make_solver(ca, inst, seed):
@@ -75,6 +90,9 @@ def main():
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design is "
               "described in DESIGN.md", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     import cvrp_aco_ocl_amd as ca
     from cvrp_aco_ocl_amd import instances
 
@@ -147,11 +165,9 @@ def main():
         achieved = ph_bpl / (avg["pheromone"] * 1e-3) / 1e9
         traffic_alg = ph_bpl
     else:
-        # local search: HBM bytes = tours in/out + distance rows staged per
-        # scanned position (3 rows for swap, 1 for relocate) -- the
-        # evaluation itself runs from LDS
-        steps = size_sum / args.profile_steps
-        traffic_alg = int(8 * steps + 4 * 4 * inst.n * steps)
+        # local search: 16 B per evaluated move (the four distance / edge
+        # words a 2-opt, swap or relocate delta reads)
+        traffic_alg = int(16 * ls_evals / max(args.profile_steps, 1))
         achieved = traffic_alg / (avg[dominant] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 2),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),

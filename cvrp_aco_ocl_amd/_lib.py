@@ -10,7 +10,9 @@ import ctypes as C
 import os
 
 PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG, "libcvrpaco.so")
+# CVRP_ACO_LIB selects another build of the same library (e.g. an
+# instrumented one); the default is the in-tree build.
+LIB_PATH = os.environ.get("CVRP_ACO_LIB") or os.path.join(PKG, "libcvrpaco.so")
 
 ACO_OK = 0
 ERRORS = {-1: "ACO_ERR_ARG", -2: "ACO_ERR_HIP", -3: "ACO_ERR_STATE", -4: "ACO_ERR_INFEASIBLE",
@@ -45,6 +47,19 @@ class AcoParams(C.Structure):
         ("tau0_rule", C.c_int32), ("seed", C.c_uint64), ("bsf_capacity", C.c_int32),
         ("use_graph", C.c_int32),
     ]
+
+
+class AcoPriorTrainParams(C.Structure):
+    _fields_ = [
+        ("epochs", C.c_int32), ("batch", C.c_int32), ("lr", C.c_float), ("lr_final", C.c_float),
+        ("pos_weight", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float),
+        ("shuffle_seed", C.c_uint64),
+    ]
+
+
+PRIOR_FEATURES = 12
+PRIOR_HIDDEN = 64
+PRIOR_PARAMS = 64 * 12 + 64 + 64 * 64 + 64 + 64 + 1
 
 
 class AcoBsfRecord(C.Structure):
@@ -100,6 +115,21 @@ _SIGNATURES = {
                                   C.POINTER(C.c_int32)]),
     "aco_check_solution": (C.c_int, [C.c_int32, _P, _P, C.c_int32, C.c_float, C.c_float, _P,
                                      C.c_int32]),
+    # learned prior (cvrp_prior.h)
+    "aco_prior_default_train_params": (None, [C.POINTER(AcoPriorTrainParams)]),
+    "aco_prior_create": (C.c_int, [C.c_uint64, C.POINTER(C.c_void_p)]),
+    "aco_prior_destroy": (C.c_int, [_P]),
+    "aco_prior_get_params": (C.c_int, [_P, _P, C.c_int32]),
+    "aco_prior_set_params": (C.c_int, [_P, _P, C.c_int32]),
+    "aco_prior_features": (C.c_int, [C.c_int32, _P, _P, _P, C.c_int32, _P, _P, C.c_int32,
+                                     C.c_int32, _P]),
+    "aco_prior_forward": (C.c_int, [_P, _P, C.c_int64, _P]),
+    "aco_prior_gradient": (C.c_int, [_P, _P, _P, C.c_int64, C.c_float, _P,
+                                     C.POINTER(C.c_float)]),
+    "aco_prior_adam_step": (C.c_int, [_P, _P, C.c_float, C.c_float, C.c_float, C.c_float]),
+    "aco_prior_train": (C.c_int, [_P, _P, _P, C.c_int64, C.POINTER(AcoPriorTrainParams), _P]),
+    "aco_prior_build": (C.c_int, [_P, C.c_int32, _P, _P, _P, C.c_int32, _P, _P, C.c_int32,
+                                  C.c_int32, C.c_float, C.c_float, _P]),
 }
 
 EXPORTED = tuple(_SIGNATURES)

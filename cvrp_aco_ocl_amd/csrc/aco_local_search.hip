@@ -32,13 +32,14 @@ constexpr int kLsThreads = kLsWaves * kLanes;
 constexpr int kRowSlots = 16;   // distance-row ring, slot = tour position mod ring size
 constexpr int kPrefetch = 4;    // row elements per thread held for the next round
 constexpr int kPrefetchMaxN = kLsThreads * kPrefetch;
-// LDS words of L.res: [0,4) and [4,8) per-wave scan results of even and odd
-// rounds, 8 route count.
-constexpr int kResR = 8;
+// LDS words of L.res: [0,4) and [4,8) per-wave scan results (accepted j)
+// of even and odd rounds, 8 route count, [12,16) and [16,20) the moves each
+// wave's scan evaluated up to its result.
+constexpr int kResR = 8, kResEv = 12;
 
 // Ring size: 16 rows when the next round's rows are prefetched (n <= 1024,
 // 6 current + 4 next positions), else 8 (6 current positions).
-__host__ __device__ constexpr int ls_ring_slots(int n) { return n <= kPrefetchMaxN ? 16 : 8; }
+__host__ __device__ constexpr int ls_ring_slots(int n) { return n <= kPrefetchMaxN ? kRowSlots : 8; }
 
 struct LsLds {
     int *tour, *tmp, *rl, *dem, *res;
@@ -157,6 +158,28 @@ __device__ __forceinline__ int ls_routes(const DevProblem &p, LsLds &L, int S, i
 
 // ------------------------------------------------------------- 2-opt ----
 
+// A route is searched by a group of kOptGroup lanes of one wave: the
+// candidate scan is split over the lanes and reduced; moves are applied
+// with the reversals split over the lanes.
+constexpr int kOptGroup = 8;
+
+__device__ __forceinline__ u64 group_min_u64(u64 v)
+{
+#pragma unroll
+    for (int o = 1; o < kOptGroup; o <<= 1) {
+        const unsigned lo = __shfl_xor((unsigned)v, o), hi = __shfl_xor((unsigned)(v >> 32), o);
+        const u64 w = ((u64)hi << 32) | lo;
+        v = w < v ? w : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ void group_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
 // One scan for n1 (SUCC: partner edges (n1,s1),(n2,s2); otherwise
 // (p1,n1),(p2,n2); nb1 is s1 or p1).  The reference walks n1's nn row,
 // skips nodes of other routes and stops at the first in-route node with
@@ -166,42 +189,78 @@ __device__ __forceinline__ int ls_routes(const DevProblem &p, LsLds &L, int S, i
 // improving one earliest in the row.  Here the route's own positions are
 // scanned (independent loads, no walk over other routes' neighbours) and
 // the improving candidate with the smallest row position nnpos[n1][n2] is
-// kept.  Returns true with h1..h4 set on an improving move.
+// kept (row positions are distinct, so the group minimum is unique).
+// Returns true with h1..h4 set on an improving move (same in all lanes of
+// the group).
+// Candidate q of a scan: in the row (rp < nn_ls), inside the radius, and
+// for the predecessor variant not adjacent; x is s2 or p2.
+template <bool SUCC>
+__device__ __forceinline__ bool two_opt_candidate(const DevProblem &p, const LsLds &L, int rend,
+                                                  int q, int n1, int nb1, float radius,
+                                                  const unsigned short *rowpos, int &rp, int &n2,
+                                                  int &x, float &dn)
+{
+    n2 = L.tour[q];
+    rp = rowpos[n2];
+    dn = Dd(p, n1, n2);
+    if (rp >= p.nn_ls || !(radius - dn > kEps)) return false;
+    if (SUCC) {
+        x = q == rend ? 0 : L.tour[q + 1];
+        return true;
+    }
+    x = L.tour[q - 1];
+    return !(x == n1 || nb1 == n2);
+}
+
 template <bool SUCC>
 __device__ __forceinline__ bool two_opt_scan(const DevProblem &p, const LsLds &L, int rbeg,
                                              int rend, int pos1, int n1, int nb1, float radius,
-                                             long long &evals, int &h1, int &h2, int &h3, int &h4)
+                                             int gl, long long &evals, int &h1, int &h2, int &h3,
+                                             int &h4)
 {
     const unsigned short *rowpos = p.nnpos + (size_t)n1 * p.n;
-    int best = 0x7fffffff, b2 = 0, bx = 0;
-#pragma unroll 4
-    for (int q = rbeg + 1; q <= rend; ++q) {
+    u64 best = ~0ull;
+    int cnt = 0;
+    for (int q = rbeg + 1 + gl; q <= rend; q += kOptGroup) {
         if (q == pos1) continue;
-        const int n2 = L.tour[q];
-        const int rp = rowpos[n2];
-        const float dn = Dd(p, n1, n2);
-        if (rp >= p.nn_ls || !(radius - dn > kEps)) continue;
-        if (SUCC) {
-            const int s2 = q == rend ? 0 : L.tour[q + 1];
-            evals++;
-            const float gain = -radius + dn + Dd(p, nb1, s2) - L.dtw[q];
-            if (gain < -kEps && rp < best) { best = rp; b2 = n2; bx = s2; }
-        } else {
-            const int p2 = L.tour[q - 1];
-            if (p2 == n1 || nb1 == n2) continue;
-            evals++;
-            const float gain = -radius + dn + Dd(p, nb1, p2) - L.dtw[q - 1];
-            if (gain < -kEps && rp < best) { best = rp; b2 = n2; bx = p2; }
+        int rp, n2, x;
+        float dn;
+        if (!two_opt_candidate<SUCC>(p, L, rend, q, n1, nb1, radius, rowpos, rp, n2, x, dn)) continue;
+        ++cnt;
+        const float gain = SUCC ? -radius + dn + Dd(p, nb1, x) - L.dtw[q]
+                                : -radius + dn + Dd(p, nb1, x) - L.dtw[q - 1];
+        if (gain < -kEps) {
+            const u64 key = ((u64)rp << 32) | ((u64)n2 << 16) | (u64)x;
+            best = key < best ? key : best;
         }
     }
-    if (best == 0x7fffffff) return false;
+    best = group_min_u64(best);
+    if (best == ~0ull) {
+        evals += cnt;
+        return false;
+    }
+    // the reference's walk stops at the accepted candidate: it evaluated
+    // the candidates up to that row position
+    const int rps = (int)(best >> 32);
+    cnt = 0;
+    for (int q = rbeg + 1 + gl; q <= rend; q += kOptGroup) {
+        if (q == pos1) continue;
+        int rp, n2, x;
+        float dn;
+        if (two_opt_candidate<SUCC>(p, L, rend, q, n1, nb1, radius, rowpos, rp, n2, x, dn) && rp <= rps)
+            ++cnt;
+    }
+    evals += cnt;
+    const int b2 = (int)((best >> 16) & 0xFFFF), bx = (int)(best & 0xFFFF);
     if (SUCC) { h1 = n1; h2 = nb1; h3 = b2; h4 = bx; }
     else { h1 = nb1; h2 = n1; h3 = bx; h4 = b2; }
     return true;
 }
 
+// kernel.cl:126-277 for route r at positions [rbeg, rend] (rbeg holds the
+// depot), run by the kOptGroup lanes gl = 0.. of one group.
 __device__ __forceinline__ void two_opt_route(const DevProblem &p, LsLds &L, int r, int rbeg,
-                                              int rend, long long &evals, int &moves_total)
+                                              int rend, int gl, long long &evals, int &moves_total)
 {
     bool dlb0 = r > 0;
     const int num = rend - rbeg + 1;
@@ -216,18 +275,20 @@ __device__ __forceinline__ void two_opt_route(const DevProblem &p, LsLds &L, int
             int h1 = 0, h2 = 0, h3 = 0, h4 = 0;
             const int s1 = pos1 == rend ? 0 : L.tour[pos1 + 1];
             // d(n1, s1) is the edge leaving position pos1
-            bool found = two_opt_scan<true>(p, L, rbeg, rend, pos1, n1, s1, L.dtw[pos1], evals,
+            bool found = two_opt_scan<true>(p, L, rbeg, rend, pos1, n1, s1, L.dtw[pos1], gl, evals,
                                             h1, h2, h3, h4);
             if (!found) {
                 const int p1 = pos1 == rbeg ? L.tour[rend] : L.tour[pos1 - 1];
                 // d(p1, n1): the edge entering pos1, or the closing edge
                 // (tour[rend], depot) when n1 is the depot
                 const float rad = pos1 == rbeg ? L.dtw[rend] : L.dtw[pos1 - 1];
-                found = two_opt_scan<false>(p, L, rbeg, rend, pos1, n1, p1, rad, evals, h1, h2, h3, h4);
+                found = two_opt_scan<false>(p, L, rbeg, rend, pos1, n1, p1, rad, gl, evals, h1, h2,
+                                            h3, h4);
             }
             if (!found) {
                 if (n1 == 0) dlb0 = true;
-                else L.dlb[n1] = 1;
+                else if (gl == 0) L.dlb[n1] = 1;
+                group_sync();
                 continue;
             }
             imp = true;
@@ -235,7 +296,7 @@ __device__ __forceinline__ void two_opt_route(const DevProblem &p, LsLds &L, int
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 if (hs[q] == 0) dlb0 = false;
-                else L.dlb[hs[q]] = 0;
+                else if (gl == 0) L.dlb[hs[q]] = 0;
             }
             int ph1 = h1 == 0 ? rbeg : L.pos[h1];
             int ph3 = h3 == 0 ? rbeg : L.pos[h3];
@@ -244,56 +305,51 @@ __device__ __forceinline__ void two_opt_route(const DevProblem &p, LsLds &L, int
                 t = h2; h2 = h4; h4 = t;
                 t = ph1; ph1 = ph3; ph3 = t;
             }
-            int i = ph1, j = ph3;
-            L.dtw[i] = Dd(p, h1, h3);
-            L.dtw[j] = Dd(p, h2, h4);
-            i++; j--;
-            while (i < j) {
-                const float t = L.dtw[i];
-                L.dtw[i] = L.dtw[j];
-                L.dtw[j] = t;
-                i++; j--;
+            const int tb = h2 == 0 ? rbeg : L.pos[h2];
+            group_sync();
+            // edge weights: the two new edges, the segment between reversed
+            if (gl == 0) {
+                L.dtw[ph1] = Dd(p, h1, h3);
+                L.dtw[ph3] = Dd(p, h2, h4);
             }
-            i = h2 == 0 ? rbeg : L.pos[h2];
-            j = ph3;
-            while (i < j) {
+            const int ed = (ph3 - ph1 - 1) / 2;
+            for (int t = gl; t < ed; t += kOptGroup) {
+                const int i = ph1 + 1 + t, j = ph3 - 1 - t;
+                const float a = L.dtw[i];
+                L.dtw[i] = L.dtw[j];
+                L.dtw[j] = a;
+            }
+            // tour positions [tb, ph3] reversed
+            const int tp = (ph3 - tb + 1) / 2;
+            for (int t = gl; t < tp; t += kOptGroup) {
+                const int i = tb + t, j = ph3 - t;
                 const int a = L.tour[i], b = L.tour[j];
                 L.tour[i] = b;
                 L.tour[j] = a;
                 L.pos[a] = (short)j;
                 L.pos[b] = (short)i;
-                i++; j--;
             }
-            if (++moves >= kMaxMovesPerRoute) { moves_total += moves; return; }
+            group_sync();
+            if (++moves >= kMaxMovesPerRoute) break;
         }
+        if (moves >= kMaxMovesPerRoute) break;
     }
-    moves_total += moves;
+    if (gl == 0) moves_total += moves;
 }
 
 // ------------------------------------------------- distance row cache ----
 
-// Rows d(tour[q], .) live in slot q mod R of an LDS ring of R rows.  Every thread keeps
-// the same copy of the slot tags (the node whose row a slot holds) in
-// registers, so the fetch plan needs no serial thread and no LDS round
-// trip.  The ring holds the current round's positions and the next round's
-// without collision (at most 10 consecutive positions).
-struct RowRing {
-    int tag[kRowSlots];
+// Rows d(tour[q], .) live in slot q mod R of an LDS ring of R rows (R = 16
+// or 8).  The window of positions [base, base + R) maps one to one onto the
+// slots; bit o of a validity mask says that the slot of position base + o
+// holds the row of the node now at that position.  Moving the window shifts
+// the mask, a move clears the bits of the positions it changed.  All of it
+// is uniform (every thread holds the same mask), so checks are a few scalar
+// instructions and need no LDS round trip.
+struct RowWindow {
+    int base;
+    unsigned mask;
 };
-
-__device__ __forceinline__ int ring_get(const RowRing &r, int slot)
-{
-    int v = -1;
-#pragma unroll
-    for (int s = 0; s < kRowSlots; ++s) v = s == slot ? r.tag[s] : v;
-    return v;
-}
-
-__device__ __forceinline__ void ring_set(RowRing &r, int slot, int node)
-{
-#pragma unroll
-    for (int s = 0; s < kRowSlots; ++s) r.tag[s] = s == slot ? node : r.tag[s];
-}
 
 __device__ __forceinline__ int ring_slot(const DevProblem &p, int pos)
 {
@@ -305,146 +361,192 @@ __device__ __forceinline__ const float *row_at(const DevProblem &p, const LsLds 
     return L.rows + ring_slot(p, pos) * p.n;
 }
 
-// Make the rows of positions [first, first+cnt) present (cnt <= 6).  All
-// threads load the missing ones; a barrier follows when any was missing
-// (the decision is uniform: tags and the tour are the same for all threads).
-__device__ __forceinline__ void rows_ensure(const DevProblem &p, LsLds &L, RowRing &ring, int first,
-                                            int cnt, int tid)
+// Start the window at position base (never moves backwards within a scan).
+__device__ __forceinline__ void window_move(RowWindow &w, int base)
+{
+    const int d = base - w.base;
+    w.mask = d >= 32 ? 0u : (w.mask >> d);
+    w.base = base;
+}
+
+// Positions [lo, hi] changed: their rows are stale.
+__device__ __forceinline__ void window_clear(RowWindow &w, int lo, int hi)
+{
+    const int a = lo - w.base > 0 ? lo - w.base : 0;
+    const int b = hi - w.base < 31 ? hi - w.base : 31;
+    if (a > b) return;
+    const unsigned span = (b - a + 1) >= 32 ? 0xFFFFFFFFu : ((1u << (b - a + 1)) - 1u);
+    w.mask &= ~(span << a);
+}
+
+// Make the rows of window offsets [0, cnt) present (cnt <= 6): all threads
+// load the missing ones; a barrier follows when any was missing.
+__device__ __forceinline__ void rows_ensure(const DevProblem &p, LsLds &L, RowWindow &w, int cnt,
+                                            int tid)
 {
     const int n = p.n;
     bool loaded = false;
 #pragma unroll
     for (int q = 0; q < kLsWaves + 2; ++q) {
         if (q >= cnt) break;
-        const int pos = first + q;
+        if ((w.mask >> q) & 1u) continue;
+        const int pos = w.base + q;
         const int v = L.tour[pos];
-        const int slot = ring_slot(p, pos);
-        if (ring_get(ring, slot) != v) {
-            float *row = L.rows + slot * n;
-            const float *src = p.dist + (size_t)v * p.P;
-            for (int c = tid; c < n; c += kLsThreads) row[c] = src[c];
-            ring_set(ring, slot, v);
-            loaded = true;
-        }
+        float *row = L.rows + ring_slot(p, pos) * n;
+        const float *src = p.dist + (size_t)v * p.P;
+        for (int c = tid; c < n; c += kLsThreads) row[c] = src[c];
+        loaded = true;
     }
+    w.mask |= (1u << cnt) - 1u;
     if (loaded) __syncthreads();
 }
 
-// Next round's rows (positions [first, first+cnt), cnt <= 4) are loaded
+// Next round's rows (window offsets [off, off+4), positions < S) are loaded
 // into registers before a scan and stored into the ring after it, so the
-// loads overlap the scan.  Only for n <= 1024 (4 elements per thread).
+// loads overlap the scan.  Only with the 16-row ring (n <= 1024, 4 elements
+// per thread); the 10 positions of this round and the next never share a
+// slot.
 struct RowPrefetch {
     float v[kLsWaves][kPrefetch];
-    int node[kLsWaves];
+    unsigned want;
 };
 
-__device__ __forceinline__ void prefetch_issue(const DevProblem &p, const LsLds &L, const RowRing &ring,
-                                               RowPrefetch &pf, int first, int cnt, int S, int tid)
+__device__ __forceinline__ void prefetch_issue(const DevProblem &p, const LsLds &L, const RowWindow &w,
+                                               RowPrefetch &pf, int off, int S, int tid)
 {
+    pf.want = 0;
+    if (p.n > kPrefetchMaxN) return;
 #pragma unroll
     for (int q = 0; q < kLsWaves; ++q) {
-        pf.node[q] = -1;
-        const int pos = first + q;
-        if (q < cnt && pos < S && p.n <= kPrefetchMaxN) {
-            const int v = L.tour[pos];
-            if (ring_get(ring, ring_slot(p, pos)) != v) {
-                pf.node[q] = v;
-                const float *src = p.dist + (size_t)v * p.P;
+        const int pos = w.base + off + q;
+        if (pos >= S || ((w.mask >> (off + q)) & 1u)) continue;
+        pf.want |= 1u << q;
+        const float *src = p.dist + (size_t)L.tour[pos] * p.P;
 #pragma unroll
-                for (int e = 0; e < kPrefetch; ++e) {
-                    const int c = tid + e * kLsThreads;
-                    pf.v[q][e] = c < p.n ? src[c] : 0.0f;
-                }
-            }
+        for (int e = 0; e < kPrefetch; ++e) {
+            const int c = tid + e * kLsThreads;
+            pf.v[q][e] = c < p.n ? src[c] : 0.0f;
         }
     }
 }
 
-__device__ __forceinline__ void prefetch_store(const DevProblem &p, LsLds &L, RowRing &ring,
-                                               const RowPrefetch &pf, int first, int tid)
+__device__ __forceinline__ void prefetch_store(const DevProblem &p, LsLds &L, RowWindow &w,
+                                               const RowPrefetch &pf, int off, int tid)
 {
 #pragma unroll
     for (int q = 0; q < kLsWaves; ++q) {
-        if (pf.node[q] < 0) continue;
-        const int slot = ring_slot(p, first + q);
-        float *row = L.rows + slot * p.n;
+        if (!((pf.want >> q) & 1u)) continue;
+        float *row = L.rows + ring_slot(p, w.base + off + q) * p.n;
 #pragma unroll
         for (int e = 0; e < kPrefetch; ++e) {
             const int c = tid + e * kLsThreads;
             if (c < p.n) row[c] = pf.v[q][e];
         }
-        ring_set(ring, slot, pf.node[q]);
     }
+    w.mask |= pf.want << off;
 }
 
 // -------------------------------------------------------------- swap ----
 
-// Position i against the current tour; returns the first improving j or -1.
+// Candidate chunks evaluated per loop trip of a scan (independent loads in
+// flight; the first accepted j in scan order is still the one returned).
+constexpr int kScanUnroll = 4;
+
+// Mask of lanes 0..f.
+__device__ __forceinline__ u64 lanes_upto(int f) { return (2ull << f) - 1ull; }
+
+// Swap of position i with position j (localSearch.cpp:301-410): sets ok
+// when the move improves and is feasible, val when j is a customer
+// position (an evaluated move).
+__device__ __forceinline__ void swap_eval(const DevProblem &p, const LsLds &L, int S, int i, int j,
+                                          int r1, int dn1, float a1, float b1, const float *rowA,
+                                          const float *rowB, const float *rowC, bool &ok, bool &val)
+{
+    const int jj = j < S - 1 ? j : S - 1;
+    const int n2 = L.tour[jj], pn2 = L.tour[jj - 1], sn2 = L.tour[jj < S - 1 ? jj + 1 : jj];
+    const float dm = L.dtw[jj - 1], d0 = L.dtw[jj];
+    const float rA = rowA[n2], rC = rowC[n2], rBp = rowB[pn2], rBs = rowB[sn2];
+    val = j < S && n2 != 0;
+    float gain;
+    if (j == i + 1)
+        gain = -(a1 + d0) + (rA + rBs);
+    else
+        gain = -(a1 + b1 + dm + d0) + (rA + rC + rBp + rBs);
+    ok = val && gain < -kEps;
+    if (ok) {
+        const int r2 = L.ridx[jj];
+        if (r1 != r2) {
+            const int dn2 = L.dem[n2];
+            const int load1 = L.rl[r1] - dn1 + dn2, load2 = L.rl[r2] - dn2 + dn1;
+            const float dist1 = (L.rd[r1] - (a1 + b1)) + (rA + rC);
+            const float dist2 = (L.rd[r2] - (dm + d0)) + (rBp + rBs);
+            ok = !(load1 > p.Q || load2 > p.Q || dist1 > p.D || dist2 > p.D);
+        }
+    }
+}
+
+// Position i against the current tour; returns the first accepted j or -1
+// and in evc the moves the sequential scan evaluates up to it.
 __device__ __forceinline__ int swap_scan(const DevProblem &p, const LsLds &L, int S, int i,
                                          const float *rowA, const float *rowB, const float *rowC,
-                                         int lane, long long &ev)
+                                         int lane, int &evc)
 {
     const int n1 = L.tour[i];
     const int sn1 = L.tour[i + 1], r1 = L.ridx[i];
     const float a1 = rowA[n1], b1 = rowB[sn1];
     const int dn1 = L.dem[n1];
-    for (int jb = i + 1; jb < S; jb += kLanes) {
-        const int j = jb + lane;
-        bool ok = false;
-        if (j < S) {
-            const int n2 = L.tour[j];
-            if (n2 != 0) {
-                ev++;
-                const int pn2 = L.tour[j - 1], sn2 = L.tour[j + 1], r2 = L.ridx[j];
-                float gain;
-                if (j == i + 1)
-                    gain = -(a1 + L.dtw[j]) + (rowA[n2] + rowB[sn2]);
-                else
-                    gain = -(a1 + b1 + L.dtw[j - 1] + L.dtw[j]) +
-                           (rowA[n2] + rowC[n2] + rowB[pn2] + rowB[sn2]);
-                if (gain < -kEps) {
-                    if (r1 != r2) {
-                        const int dn2 = L.dem[n2];
-                        const int load1 = L.rl[r1] - dn1 + dn2, load2 = L.rl[r2] - dn2 + dn1;
-                        const float dist1 = (L.rd[r1] - (a1 + b1)) + (rowA[n2] + rowC[n2]);
-                        const float dist2 = (L.rd[r2] - (L.dtw[j - 1] + L.dtw[j])) +
-                                            (rowB[pn2] + rowB[sn2]);
-                        ok = !(load1 > p.Q || load2 > p.Q || dist1 > p.D || dist2 > p.D);
-                    } else {
-                        ok = true;
-                    }
-                }
+    int cnt = 0;
+    for (int jb = i + 1; jb < S; jb += kScanUnroll * kLanes) {
+        bool ok[kScanUnroll], val[kScanUnroll];
+#pragma unroll
+        for (int u = 0; u < kScanUnroll; ++u)
+            swap_eval(p, L, S, i, jb + u * kLanes + lane, r1, dn1, a1, b1, rowA, rowB, rowC, ok[u],
+                      val[u]);
+#pragma unroll
+        for (int u = 0; u < kScanUnroll; ++u) {
+            const u64 mk = __ballot(ok[u]), mv = __ballot(val[u]);
+            if (mk) {
+                const int f = __ffsll(mk) - 1;
+                evc = cnt + __popcll(mv & lanes_upto(f));
+                return jb + u * kLanes + f;
             }
+            cnt += __popcll(mv);
         }
-        const u64 mk = __ballot(ok);
-        if (mk) return jb + __ffsll(mk) - 1;
     }
+    evc = cnt;
     return -1;
 }
 
-__device__ __forceinline__ void ls_swap(const DevProblem &p, LsLds &L, RowRing &ring, int S, int tid,
-                                        long long &ev, int &moves)
+__device__ __forceinline__ void ls_swap(const DevProblem &p, LsLds &L, int S, int tid, long long &ev,
+                                        int &moves)
 {
     const int lane = tid & 63, w = tid >> 6;
     const int cap = kMaxMovesFactor * p.n;
     int done = 0, i0 = 1, parity = 0;
+    RowWindow win = {0, 0u};
     while (i0 < S) {
         // rows of positions i0-1 .. i0+W (pred, self, succ of each scanned i)
-        rows_ensure(p, L, ring, i0 - 1, min(kLsWaves + 2, S - i0 + 1), tid);
+        window_move(win, i0 - 1);
+        rows_ensure(p, L, win, min(kLsWaves + 2, S - i0 + 1), tid);
         RowPrefetch pf;
-        prefetch_issue(p, L, ring, pf, i0 + kLsWaves + 1, kLsWaves, S, tid);
+        prefetch_issue(p, L, win, pf, kLsWaves + 2, S, tid);
         const int i = i0 + w;
-        int j = -1;
+        int j = -1, evc = 0;
         if (i < S && L.tour[i] != 0)
-            j = swap_scan(p, L, S, i, row_at(p, L, i - 1), row_at(p, L, i), row_at(p, L, i + 1), lane, ev);
-        int *res = L.res + 4 * parity;
+            j = swap_scan(p, L, S, i, row_at(p, L, i - 1), row_at(p, L, i), row_at(p, L, i + 1), lane, evc);
+        int *res = L.res + 4 * parity, *rese = L.res + kResEv + 4 * parity;
         parity ^= 1;
-        if (lane == 0) res[w] = j;
-        prefetch_store(p, L, ring, pf, i0 + kLsWaves + 1, tid);
+        if (lane == 0) { res[w] = j; rese[w] = evc; }
+        prefetch_store(p, L, win, pf, kLsWaves + 2, tid);
         __syncthreads();
-        int wstar = -1;
-        for (int q = 0; q < kLsWaves; ++q)
+        // the sequential scan evaluates the waves' positions in order and
+        // stops at the first accepted move
+        int wstar = -1, e = 0;
+        for (int q = 0; q < kLsWaves; ++q) {
+            e += rese[q];
             if (res[q] >= 0) { wstar = q; break; }
+        }
+        if (tid == 0) ev += e;
         if (wstar < 0) {
             // no barrier: the next round writes the other result buffer, and
             // rows it loads belong to positions no wave still reads
@@ -488,6 +590,8 @@ __device__ __forceinline__ void ls_swap(const DevProblem &p, LsLds &L, RowRing &
             L.dtw[js] = Dd(p, L.tour[js], L.tour[js + 1]);
         }
         __syncthreads();
+        window_clear(win, is, is);
+        window_clear(win, js, js);
         i0 = is;  // re-examine the position that moved (the reference's i--)
         if (++done >= cap) break;
     }
@@ -496,66 +600,91 @@ __device__ __forceinline__ void ls_swap(const DevProblem &p, LsLds &L, RowRing &
 
 // ---------------------------------------------------------- relocate ----
 
+// Relocate of position i next to position j (neighbourSearch.cpp:280-293,
+// move.cpp:32-51): ok when improving and feasible, val when evaluated.
+__device__ __forceinline__ void relocate_eval(const DevProblem &p, const LsLds &L, int S, int i,
+                                              int j, int r1, int dn1, float a1, float b1, float c1,
+                                              const float *rowB, bool &ok, bool &val)
+{
+    const int jj = j < S - 1 ? j : S - 2;
+    const int n2 = L.tour[jj], pn2 = L.tour[jj - 1], sn2 = L.tour[jj + 1];
+    const float dm = L.dtw[jj - 1], d0 = L.dtw[jj];
+    const bool before = i > jj;
+    const float rm = before ? dm : d0;
+    const float add = rowB[n2] + rowB[before ? pn2 : sn2];
+    val = j < S - 1 && j != i && n2 != 0;
+    const float gain = -(a1 + b1 + rm) + (add + c1);
+    ok = val && gain < -kEps;
+    if (ok) {
+        const int r2 = L.ridx[jj];
+        if (r1 != r2) {
+            const float dist2 = ((L.rd[r2] + p.svc) - rm) + add;
+            const float dist1 = ((L.rd[r1] - p.svc) - (a1 + b1)) + c1;
+            ok = !(L.rl[r2] + dn1 > p.Q) && !(dist2 > p.D) && !(dist1 > p.D);
+        }
+    }
+}
+
 __device__ __forceinline__ int relocate_scan(const DevProblem &p, const LsLds &L, int S, int i,
                                              const float *rowP, const float *rowB, int lane,
-                                             long long &ev)
+                                             int &evc)
 {
     const int n1 = L.tour[i];
     const int r1 = L.ridx[i], sn1 = L.tour[i + 1];
     const float a1 = L.dtw[i - 1], b1 = L.dtw[i], c1 = rowP[sn1];
     const int dn1 = L.dem[n1];
-    for (int jb = 1; jb < S - 1; jb += kLanes) {
-        const int j = jb + lane;
-        bool ok = false;
-        if (j < S - 1 && j != i) {
-            const int n2 = L.tour[j];
-            if (n2 != 0) {
-                ev++;
-                const int pn2 = L.tour[j - 1], sn2 = L.tour[j + 1], r2 = L.ridx[j];
-                float rm, add;
-                if (i > j) { rm = L.dtw[j - 1]; add = rowB[n2] + rowB[pn2]; }
-                else { rm = L.dtw[j]; add = rowB[n2] + rowB[sn2]; }
-                const float gain = -(a1 + b1 + rm) + (add + c1);
-                if (gain < -kEps) {
-                    if (r1 != r2) {
-                        const float dist2 = ((L.rd[r2] + p.svc) - rm) + add;
-                        const float dist1 = ((L.rd[r1] - p.svc) - (a1 + b1)) + c1;
-                        ok = !(L.rl[r2] + dn1 > p.Q) && !(dist2 > p.D) && !(dist1 > p.D);
-                    } else {
-                        ok = true;
-                    }
-                }
+    int cnt = 0;
+    for (int jb = 1; jb < S - 1; jb += kScanUnroll * kLanes) {
+        bool ok[kScanUnroll], val[kScanUnroll];
+#pragma unroll
+        for (int u = 0; u < kScanUnroll; ++u)
+            relocate_eval(p, L, S, i, jb + u * kLanes + lane, r1, dn1, a1, b1, c1, rowB, ok[u],
+                          val[u]);
+#pragma unroll
+        for (int u = 0; u < kScanUnroll; ++u) {
+            const u64 mk = __ballot(ok[u]), mv = __ballot(val[u]);
+            if (mk) {
+                const int f = __ffsll(mk) - 1;
+                evc = cnt + __popcll(mv & lanes_upto(f));
+                return jb + u * kLanes + f;
             }
+            cnt += __popcll(mv);
         }
-        const u64 mk = __ballot(ok);
-        if (mk) return jb + __ffsll(mk) - 1;
     }
+    evc = cnt;
     return -1;
 }
 
-__device__ __forceinline__ void ls_relocate(const DevProblem &p, LsLds &L, RowRing &ring, int S,
-                                            int tid, long long &ev, int &moves)
+__device__ __forceinline__ void ls_relocate(const DevProblem &p, LsLds &L, int S, int tid,
+                                            long long &ev, int &moves)
 {
     const int lane = tid & 63, w = tid >> 6;
     const int cap = kMaxMovesFactor * p.n;
     int done = 0, i0 = 1, parity = 0;
+    RowWindow win = {0, 0u};
     while (i0 < S - 1 && done < cap) {
         // rows of positions i0-1 .. i0+W-1: d(pred, .) gives c1, d(n1, .) the gains
-        rows_ensure(p, L, ring, i0 - 1, min(kLsWaves + 1, S - i0), tid);
+        window_move(win, i0 - 1);
+        rows_ensure(p, L, win, min(kLsWaves + 1, S - i0), tid);
         RowPrefetch pf;
-        prefetch_issue(p, L, ring, pf, i0 + kLsWaves, kLsWaves, S - 1, tid);
+        prefetch_issue(p, L, win, pf, kLsWaves + 1, S - 1, tid);
         const int i = i0 + w;
-        int j = -1;
+        int j = -1, evc = 0;
         if (i < S - 1 && L.tour[i] != 0 && L.rc[L.ridx[i]] >= 2)
-            j = relocate_scan(p, L, S, i, row_at(p, L, i - 1), row_at(p, L, i), lane, ev);
-        int *res = L.res + 4 * parity;
+            j = relocate_scan(p, L, S, i, row_at(p, L, i - 1), row_at(p, L, i), lane, evc);
+        int *res = L.res + 4 * parity, *rese = L.res + kResEv + 4 * parity;
         parity ^= 1;
-        if (lane == 0) res[w] = j;
-        prefetch_store(p, L, ring, pf, i0 + kLsWaves, tid);
+        if (lane == 0) { res[w] = j; rese[w] = evc; }
+        prefetch_store(p, L, win, pf, kLsWaves + 1, tid);
         __syncthreads();
-        int wstar = -1;
-        for (int q = 0; q < kLsWaves; ++q)
+        // the sequential scan evaluates the waves' positions in order and
+        // stops at the first accepted move
+        int wstar = -1, e = 0;
+        for (int q = 0; q < kLsWaves; ++q) {
+            e += rese[q];
             if (res[q] >= 0) { wstar = q; break; }
+        }
+        if (tid == 0) ev += e;
         if (wstar < 0) {
             // no barrier: the next round writes the other result buffer, and
             // rows it loads belong to positions no wave still reads
@@ -598,6 +727,7 @@ __device__ __forceinline__ void ls_relocate(const DevProblem &p, LsLds &L, RowRi
         __syncthreads();
         ls_edges(p, L, lo - 1, hi, tid);
         ls_routes(p, L, S, tid, false);  // refreshes ridx (and pos/zpos)
+        window_clear(win, lo, hi);
         i0 = is;
         ++done;
     }
@@ -622,29 +752,27 @@ __global__ __launch_bounds__(kLsThreads) void k_local_search(DevBatch bt, int fl
         L.dem[v] = p.dem[v];
     }
     __syncthreads();
-    RowRing ring;
-#pragma unroll
-    for (int q = 0; q < kRowSlots; ++q) ring.tag[q] = -1;
     long long ev = 0;
     int moves = 0;
     if (flags & kLs2Opt) {
         ls_edges(p, L, 0, S - 2, tid);
         const int R = ls_routes(p, L, S, tid, false);
-        for (int r = tid; r < R; r += kLsThreads)
-            two_opt_route(p, L, r, L.zpos[r], L.zpos[r + 1] - 1, ev, moves);
+        const int grp = tid / kOptGroup, gl = tid % kOptGroup;
+        for (int r = grp; r < R; r += kLsThreads / kOptGroup)
+            two_opt_route(p, L, r, L.zpos[r], L.zpos[r + 1] - 1, gl, ev, moves);
         __syncthreads();
     }
     if (flags & kLsSwap) {
         ls_edges(p, L, 0, S - 2, tid);
         __syncthreads();
         ls_routes(p, L, S, tid, true);
-        ls_swap(p, L, ring, S, tid, ev, moves);
+        ls_swap(p, L, S, tid, ev, moves);
     }
     if (flags & kLsRelocate) {
         ls_edges(p, L, 0, S - 2, tid);
         __syncthreads();
         ls_routes(p, L, S, tid, true);
-        ls_relocate(p, L, ring, S, tid, ev, moves);
+        ls_relocate(p, L, S, tid, ev, moves);
     }
     ls_edges(p, L, 0, S - 2, tid);
     __syncthreads();

@@ -17,6 +17,14 @@ run sees the same instances:
 * ``batch``         ``count`` instances with n-1 ~ U{50..500}, uniform or
                     4-12 Gaussian clusters, demand U{1..50}, routes
                     U{5..40}, Q = max(ceil(sum(d)/routes), max(d)).
+* ``training``      instance ``index`` of the learned prior's training set:
+                    n-1 ~ U{100..500}; half Gaussian clusters (4-12
+                    centres, sigma U[6,40]), a quarter uniform, a quarter
+                    half/half; depot U{0..1000}^2; demand U{1..100}; routes
+                    U{10..40}.
+* ``evaluation``    the prior's evaluation set: ``clustered`` instances
+                    (500 customers, 8 clusters, sigma 6, 30 routes) with
+                    seeds 100000 + i, disjoint from every other generator's.
 
 Distances are VRPLIB nint integers unless a caller asks otherwise.
 """
@@ -100,6 +108,38 @@ def batch(seed: int = 0, count: int = 256, dist_mode=_lib.DIST_NINT) -> list[Ins
         out.append(Instance.from_coords(x, y, dem, Q, dist_mode, name=f"batch{b}-{geom}",
                                         meta={"routes": routes, "geometry": geom}))
     return out
+
+
+def training(index: int, seed: int = 0, dist_mode=_lib.DIST_NINT) -> Instance:
+    r = _rng((seed << 32) + 7919 * index + 17)
+    customers = int(r.integers(100, 501))
+    u = r.random()
+    if u < 0.5:
+        nc = int(r.integers(4, 13))
+        pts = _clusters(r, customers, nc, float(r.uniform(6.0, 40.0)))
+        geom = f"clusters{nc}"
+    elif u < 0.75:
+        pts = r.integers(0, 1001, size=(customers, 2)).astype(np.float64)
+        geom = "uniform"
+    else:
+        half = customers // 2
+        nc = int(r.integers(4, 13))
+        pts = np.concatenate([r.integers(0, 1001, size=(half, 2)).astype(np.float64),
+                              _clusters(r, customers - half, nc, float(r.uniform(6.0, 40.0)))])
+        geom = f"mixed{nc}"
+    depot = r.integers(0, 1001, size=2).astype(np.float64)
+    x = np.concatenate([[depot[0]], pts[:, 0]])
+    y = np.concatenate([[depot[1]], pts[:, 1]])
+    dem = np.concatenate([[0], r.integers(1, 101, size=customers)]).astype(np.int32)
+    routes = int(r.integers(10, 41))
+    Q = max(int(math.ceil(dem.sum() / routes)), int(dem.max()))
+    return Instance.from_coords(x, y, dem, Q, dist_mode, name=f"train{index}-{geom}",
+                                meta={"routes": routes, "geometry": geom})
+
+
+def evaluation(count: int = 64, first: int = 0) -> list[Instance]:
+    return [clustered(100000 + i, customers=500, nclust=8, sigma=6.0, routes=30)
+            for i in range(first, first + count)]
 
 
 def read_solution(path: str):

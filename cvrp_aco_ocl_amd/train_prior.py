@@ -1,0 +1,154 @@
+"""Train the learned edge prior and compare it with the plain colony.
+
+    python -m cvrp_aco_ocl_amd.train_prior train --instances 4000
+    python -m cvrp_aco_ocl_amd.train_prior eval --instances 64
+
+``train``: generated instances (instances.training, n-1 in [100,500]) are
+solved by the plain colony in batches on the GPU; the edges of each best
+tour label that instance's k=16 candidate edges; the network is trained on
+the GPU by the library's own kernels (aco_prior_train) and its weights are
+written to data/prior_weights.f32 with a summary in data/prior_train.json.
+
+``eval``: the evaluation instances (instances.evaluation) are solved
+batched by the plain colony for ``--iterations`` (its final best is the
+reference cost C*) and by the colony with the prior (the prior's build time
+counts towards its clock); it reports the wall-clock time to reach
+1.02 C* and the gap to C* at equal wall-clock budgets.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import instances as I
+from . import prior as PR
+
+DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cmd_train(a):
+    t0 = time.perf_counter()
+    feats, labels, costs = [], [], []
+    for lo in range(0, a.instances, a.batch):
+        chunk = [I.training(i, seed=a.seed) for i in range(lo, min(a.instances, lo + a.batch))]
+        best = PR.solve_plain(chunk, a.iterations, n_ants=a.ants, k=a.k, seed=a.seed + 1 + lo)
+        for inst, (tour, length) in zip(chunk, best):
+            nn = inst.nn_lists(a.k)
+            feats.append(PR.features(inst, nn, a.k))
+            labels.append(PR.edge_labels(nn, a.k, tour))
+            costs.append(length)
+        log(f"labelled {lo + len(chunk)}/{a.instances} instances "
+            f"({time.perf_counter() - t0:.1f}s)")
+    t_label = time.perf_counter() - t0
+    f = np.concatenate(feats)
+    y = np.concatenate(labels)
+    del feats, labels
+    log(f"training set: {len(f)} edges, {int(y.sum())} positive ({y.mean():.4f})")
+    t1 = time.perf_counter()
+    net = PR.PriorNet(seed=a.seed + 7)
+    losses = net.train(f, y, epochs=a.epochs, batch=a.train_batch, lr=a.lr, lr_final=a.lr_final,
+                       pos_weight=a.pos_weight, shuffle_seed=a.seed + 11)
+    t_train = time.perf_counter() - t1
+    z = net.forward(f)
+    pos, neg = np.sort(z[y > 0.5]), np.sort(z[y < 0.5])
+    auc = float(np.searchsorted(neg, pos).sum() / (len(pos) * len(neg)))
+    # rank-only baseline: candidate rank as the score
+    rank = f[:, 1]
+    rank_auc = float(np.mean([np.mean(rank[y < 0.5] > v) + 0.5 * np.mean(rank[y < 0.5] == v)
+                              for v in rank[y > 0.5][:20000]]))
+    log(f"trained {a.epochs} epochs in {t_train:.2f}s: loss {losses[0]:.4f} -> {losses[-1]:.4f}, "
+        f"AUC {auc:.4f} (candidate rank alone {rank_auc:.4f})")
+    os.makedirs(DATA, exist_ok=True)
+    net.save(a.out)
+    summary = {
+        "instances": a.instances, "colony_iterations": a.iterations, "ants": a.ants, "k": a.k,
+        "edges": int(len(f)), "positives": int(y.sum()), "epochs": a.epochs,
+        "batch": a.train_batch, "lr": a.lr, "lr_final": a.lr_final, "pos_weight": a.pos_weight,
+        "epoch_loss": [round(float(v), 6) for v in losses], "train_auc": round(auc, 5),
+        "rank_only_auc": round(rank_auc, 5), "label_seconds": round(t_label, 2),
+        "train_seconds": round(t_train, 3), "seed": a.seed,
+        "mean_best_cost": float(np.mean(costs)),
+    }
+    with open(os.path.join(DATA, "prior_train.json"), "w") as fh:
+        json.dump(summary, fh, indent=1)
+    print(json.dumps(summary))
+
+
+def compare(insts, iterations, net, ants=64, k=16, seed=1, floor=0.05,
+            budgets=(0.25, 0.5, 1.0, 2.0)):
+    """Plain colony for ``iterations`` (C* = its final best), then the prior
+    colony until every instance is within 2% of C* or ``iterations``."""
+    plain = PR.run_colony(insts, iterations, None, n_ants=ants, k=k, seed=seed)
+    cstar = [r[-1][1] for r in plain["records"]]
+    targets = [1.02 * c for c in cstar]
+    pri = PR.run_colony(insts, iterations, net, n_ants=ants, k=k, seed=seed, floor=floor,
+                        targets=targets)
+    t_plain = [PR.time_to_target(r, t) for r, t in zip(plain["records"], targets)]
+    t_prior = [PR.time_to_target(r, t, pri["build_s"]) for r, t in zip(pri["records"], targets)]
+    gaps = {}
+    for bud in budgets:
+        gp = [PR.best_at(r, bud) / c - 1 for r, c in zip(plain["records"], cstar)]
+        gq = [PR.best_at(r, bud, pri["build_s"]) / c - 1 for r, c in zip(pri["records"], cstar)]
+        gaps[str(bud)] = {"plain": float(np.mean(gp)), "prior": float(np.mean(gq))}
+    fin = lambda v: [x for x in v if np.isfinite(x)]
+    out = {
+        "instances": len(insts), "iterations": iterations, "ants": ants, "k": k,
+        "prior_build_s": round(pri["build_s"], 4), "prior_iterations": pri["iterations"],
+        "plain_time_to_2pct_s": {"mean": float(np.mean(t_plain)), "median": float(np.median(t_plain))},
+        "prior_time_to_2pct_s": {"mean": float(np.mean(t_prior)) if len(fin(t_prior)) == len(t_prior) else None,
+                                 "median": float(np.median(t_prior)),
+                                 "reached": len(fin(t_prior))},
+        "gap_at_budget": gaps,
+        "mean_cstar": float(np.mean(cstar)),
+    }
+    out["ratio_mean"] = (out["prior_time_to_2pct_s"]["mean"] / out["plain_time_to_2pct_s"]["mean"]
+                         if out["prior_time_to_2pct_s"]["mean"] is not None else None)
+    out["ratio_median"] = out["prior_time_to_2pct_s"]["median"] / out["plain_time_to_2pct_s"]["median"]
+    return out
+
+
+def cmd_eval(a):
+    net = PR.PriorNet.load(a.weights)
+    insts = I.evaluation(a.instances)
+    res = compare(insts, a.iterations, net, ants=a.ants, k=a.k, floor=a.floor)
+    print(json.dumps(res))
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("train")
+    t.add_argument("--instances", type=int, default=4000)
+    t.add_argument("--iterations", type=int, default=200)
+    t.add_argument("--ants", type=int, default=64)
+    t.add_argument("--k", type=int, default=16)
+    t.add_argument("--batch", type=int, default=128, help="instances per colony batch")
+    t.add_argument("--epochs", type=int, default=6)
+    t.add_argument("--train-batch", type=int, default=16384)
+    t.add_argument("--lr", type=float, default=3e-3)
+    t.add_argument("--lr-final", type=float, default=2e-4)
+    t.add_argument("--pos-weight", type=float, default=3.0)
+    t.add_argument("--seed", type=int, default=0)
+    t.add_argument("--out", default=PR.WEIGHTS_PATH)
+    e = sub.add_parser("eval")
+    e.add_argument("--instances", type=int, default=64)
+    e.add_argument("--iterations", type=int, default=2000)
+    e.add_argument("--ants", type=int, default=64)
+    e.add_argument("--k", type=int, default=16)
+    e.add_argument("--floor", type=float, default=0.05)
+    e.add_argument("--weights", default=PR.WEIGHTS_PATH)
+    a = ap.parse_args(argv)
+    {"train": cmd_train, "eval": cmd_eval}[a.cmd](a)
+
+
+if __name__ == "__main__":
+    main()

@@ -17,13 +17,15 @@ from cvrp_aco_ocl_amd import _lib, instances as I
 from helpers import VRPLIB
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADER = os.path.join(ROOT, "include", "cvrp_aco.h")
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("cvrp_aco.h", "cvrp_prior.h")]
 
 
 def _declared():
-    txt = open(HEADER).read()
-    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"\b(aco_[a-z0-9_]+)\s*\(", txt)))
+    names = set()
+    for h in HEADERS:
+        txt = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        names.update(re.findall(r"\b(aco_[a-z0-9_]+)\s*\(", txt))
+    return sorted(names)
 
 
 def test_every_declared_symbol_is_exported():

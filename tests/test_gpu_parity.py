@@ -121,16 +121,22 @@ def test_local_search_matches_oracle(name, flags):
         g.set_ants(packed, sizes, lens)
         g.local_search()
         tours, gsizes, glens = g.ants()
+        g.update_statistics()  # folds the per-ant counters into the totals
         cnt = g.counters()
     changed = 0
+    evals = 0
     for a in range(m):
-        t, L, _ = P.local_search(otours[a], flags)
+        t, L, ev = P.local_search(otours[a], flags)
+        evals += ev
         assert gsizes[a] == len(t)
         np.testing.assert_array_equal(tours[a, : len(t)], t, err_msg=f"ant {a}")
         assert glens[a] == np.float32(L), (a, glens[a], L)
         assert P.check(t)
         changed += not np.array_equal(t, otours[a])
     assert changed > 0
+    # the moves the sequential search evaluates (the 4-wave speculation and
+    # the lane-parallel 2-opt scans count only those)
+    assert cnt["ls_evals"] == evals, (cnt["ls_evals"], evals)
 
 
 def test_statistics_and_pheromone_update_match_oracle():
