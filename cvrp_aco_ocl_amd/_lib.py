@@ -62,6 +62,14 @@ PRIOR_HIDDEN = 64
 PRIOR_PARAMS = 64 * 12 + 64 + 64 * 64 + 64 + 64 + 1
 
 
+class AcoLaunchShape(C.Structure):
+    _fields_ = [("ls_waves", C.c_int32), ("ls_ring", C.c_int32), ("ls_prefetch", C.c_int32)]
+
+
+class AcoTuneResult(C.Structure):
+    _fields_ = [("shape", AcoLaunchShape), ("ms", C.c_float)]
+
+
 class AcoBsfRecord(C.Structure):
     _fields_ = [("iter", C.c_int32), ("length", C.c_float), ("time", C.c_float)]
 
@@ -106,6 +114,9 @@ _SIGNATURES = {
     "aco_get_counters": (C.c_int, [_P, C.POINTER(AcoCounters)]),
     "aco_profile_enable": (C.c_int, [_P, C.c_int32]),
     "aco_profile_get": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+    "aco_get_launch_shape": (C.c_int, [_P, C.POINTER(AcoLaunchShape)]),
+    "aco_set_launch_shape": (C.c_int, [_P, C.POINTER(AcoLaunchShape)]),
+    "aco_autotune": (C.c_int, [_P, C.c_int32, _P, C.c_int32, C.POINTER(C.c_int32)]),
     "aco_last_error": (C.c_char_p, []),
     "aco_compute_distances": (C.c_int, [C.c_int32, _P, _P, C.c_int32, _P]),
     "aco_compute_nn_lists": (C.c_int, [C.c_int32, _P, C.c_int32, C.c_int32, _P]),

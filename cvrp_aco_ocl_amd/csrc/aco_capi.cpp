@@ -81,6 +81,9 @@ struct aco_ctx {
     float tick_to_s = 0.0f;
     bool profiling = false;
     std::map<std::string, ProfSlot> prof;
+    LsShape ls{4, 16, 1};
+    size_t lds_limit = 0;
+    int max_n = 0;
 };
 
 #define HIP_TRY(expr)                                                                      \
@@ -173,7 +176,8 @@ static int run_construct(aco_ctx *c)
 static int run_local_search(aco_ctx *c)
 {
     if (!c->prm.ls_flags) return ACO_OK;
-    HIP_TRY(timed(c, "local_search", [&] { return launch_local_search(c->bt, c->prm.ls_flags, c->stream); }));
+    HIP_TRY(timed(c, "local_search",
+                  [&] { return launch_local_search(c->bt, c->prm.ls_flags, c->ls, c->stream); }));
     return ACO_OK;
 }
 
@@ -361,8 +365,14 @@ int aco_create_batch(const aco_problem *pbs, int32_t count, const aco_params *pr
     CHK(hipGetDevice(&c->device));
     hipDeviceProp_t prop;
     CHK(hipGetDeviceProperties(&prop, c->device));
+    // default local-search shape: 4 waves, 16-row ring with prefetch when it
+    // fits, else an 8-row ring
+    c->lds_limit = prop.sharedMemPerBlock;
+    c->max_n = max_n;
+    c->ls = LsShape{4, 16, 1};
+    if (local_search_lds_bytes(max_n, 16) > prop.sharedMemPerBlock) c->ls = LsShape{4, 8, 0};
     const size_t lds_need = std::max(construct_lds_bytes(max_n, prm.selection),
-                                     prm.ls_flags ? local_search_lds_bytes(max_n) : 0);
+                                     prm.ls_flags ? local_search_lds_bytes(max_n, c->ls.ring) : 0);
     if (lds_need > prop.sharedMemPerBlock)
         return fail(set_error(ACO_ERR_ARG, "instance too large for LDS: needs " + std::to_string(lds_need) +
                                                " bytes, device has " + std::to_string(prop.sharedMemPerBlock)));
@@ -822,6 +832,135 @@ int aco_profile_enable(aco_ctx *c, int32_t enable)
     int rc = need(c, false);
     if (rc) return rc;
     c->profiling = enable != 0;
+    return ACO_OK;
+}
+
+static void drop_graph(aco_ctx *c)
+{
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->gexec = nullptr;
+    c->graph = nullptr;
+}
+
+static bool shape_fits(const aco_ctx *c, const LsShape &sh)
+{
+    return ls_shape_valid(sh) && local_search_lds_bytes(c->max_n, sh.ring) <= c->lds_limit;
+}
+
+int aco_get_launch_shape(aco_ctx *c, aco_launch_shape *out)
+{
+    int rc = need(c, false);
+    if (rc) return rc;
+    if (!out) return set_error(ACO_ERR_ARG, "null shape");
+    out->ls_waves = c->ls.waves;
+    out->ls_ring = c->ls.ring;
+    out->ls_prefetch = c->ls.prefetch;
+    return ACO_OK;
+}
+
+int aco_set_launch_shape(aco_ctx *c, const aco_launch_shape *in)
+{
+    int rc = need(c, false);
+    if (rc) return rc;
+    if (!in) return set_error(ACO_ERR_ARG, "null shape");
+    const LsShape sh{in->ls_waves, in->ls_ring, in->ls_prefetch ? 1 : 0};
+    if (!ls_shape_valid(sh))
+        return set_error(ACO_ERR_ARG, "invalid shape: waves 4|8, ring 8|16|32 >= waves+2, "
+                                      "prefetch needs ring >= 2*waves+2");
+    if (!shape_fits(c, sh))
+        return set_error(ACO_ERR_ARG, "shape needs " + std::to_string(local_search_lds_bytes(c->max_n, sh.ring)) +
+                                          " bytes of LDS, device has " + std::to_string(c->lds_limit));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->ls = sh;
+    drop_graph(c);
+    return ACO_OK;
+}
+
+int aco_autotune(aco_ctx *c, int32_t reps, aco_tune_result *results, int32_t capacity, int32_t *count)
+{
+    int rc = need(c, true);
+    if (rc) return rc;
+    if (reps < 1 || reps > 1000) return set_error(ACO_ERR_ARG, "reps must be in [1, 1000]");
+    static const LsShape kCandidates[] = {{4, 16, 1}, {4, 16, 0}, {4, 8, 0}, {8, 32, 1}, {8, 16, 0}};
+    constexpr int kCount = (int)(sizeof(kCandidates) / sizeof(kCandidates[0]));
+    if (count) *count = kCount;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (!c->prm.ls_flags) {
+        for (int q = 0; q < kCount && results && q < capacity; ++q)
+            results[q] = aco_tune_result{{kCandidates[q].waves, kCandidates[q].ring, kCandidates[q].prefetch}, -1.0f};
+        return ACO_OK;
+    }
+    // the mutable per-instance state lies between ds.tours and the end of
+    // ds.totals in the instance arena
+    struct Span { unsigned char *lo; size_t bytes; };
+    std::vector<Span> spans;
+    size_t total = 0;
+    for (auto &I : c->inst) {
+        unsigned char *lo = reinterpret_cast<unsigned char *>(I.ds.tours);
+        unsigned char *hi = reinterpret_cast<unsigned char *>(I.ds.totals + kTotWords);
+        spans.push_back({lo, (size_t)(hi - lo)});
+        total += (size_t)(hi - lo);
+    }
+    void *orig = nullptr, *built = nullptr;
+    HIP_TRY(hipMalloc(&orig, total));
+    if (hipMalloc(&built, total) != hipSuccess) {
+        (void)hipFree(orig);
+        return set_error(ACO_ERR_HIP, "aco_autotune: snapshot allocation failed");
+    }
+    auto copy_all = [&](void *buf, bool save) -> hipError_t {
+        size_t off = 0;
+        for (auto &sp : spans) {
+            unsigned char *b = static_cast<unsigned char *>(buf) + off;
+            hipError_t e = save ? hipMemcpyAsync(b, sp.lo, sp.bytes, hipMemcpyDeviceToDevice, c->stream)
+                                : hipMemcpyAsync(sp.lo, b, sp.bytes, hipMemcpyDeviceToDevice, c->stream);
+            if (e != hipSuccess) return e;
+            off += sp.bytes;
+        }
+        return hipSuccess;
+    };
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int best = -1;
+    float best_ms = 0.0f;
+    std::vector<float> times(kCount, -1.0f);
+    hipError_t err = hipEventCreate(&e0);
+    if (err == hipSuccess) err = hipEventCreate(&e1);
+    if (err == hipSuccess) err = copy_all(orig, true);
+    if (err == hipSuccess) err = launch_construct(c->bt, c->prm.selection, c->prm.rng, c->stream);
+    if (err == hipSuccess) err = copy_all(built, true);
+    for (int q = 0; q < kCount && err == hipSuccess; ++q) {
+        if (!shape_fits(c, kCandidates[q])) continue;
+        float sum = 0.0f;
+        for (int r = 0; r < reps && err == hipSuccess; ++r) {
+            err = copy_all(built, false);
+            if (err == hipSuccess) err = hipEventRecord(e0, c->stream);
+            if (err == hipSuccess) err = launch_local_search(c->bt, c->prm.ls_flags, kCandidates[q], c->stream);
+            if (err == hipSuccess) err = hipEventRecord(e1, c->stream);
+            if (err == hipSuccess) err = hipEventSynchronize(e1);
+            float ms = 0.0f;
+            if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+            sum += ms;
+        }
+        if (err != hipSuccess) break;
+        times[q] = sum / (float)reps;
+        if (best < 0 || times[q] < best_ms) {
+            best = q;
+            best_ms = times[q];
+        }
+    }
+    if (err == hipSuccess) err = copy_all(orig, false);
+    if (err == hipSuccess) err = hipStreamSynchronize(c->stream);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(orig);
+    (void)hipFree(built);
+    if (err != hipSuccess) return set_error(ACO_ERR_HIP, std::string("aco_autotune: ") + hipGetErrorString(err));
+    if (best >= 0) {
+        c->ls = kCandidates[best];
+        drop_graph(c);
+    }
+    for (int q = 0; q < kCount && results && q < capacity; ++q)
+        results[q] = aco_tune_result{{kCandidates[q].waves, kCandidates[q].ring, kCandidates[q].prefetch}, times[q]};
     return ACO_OK;
 }
 

@@ -92,15 +92,23 @@ struct DevBatch {
 
 enum { kTau0Arg = 0, kTau0Bsf = 1, kTau0Nn = 2 };
 
+// Local-search launch shape (see aco_launch_shape).
+struct LsShape {
+    int waves;
+    int ring;
+    int prefetch;
+};
+
 // ---- host launchers (aco_kernels.hip, aco_local_search.hip) ----
 hipError_t launch_construct(const DevBatch &b, int selection, int rng, hipStream_t st);
-hipError_t launch_local_search(const DevBatch &b, int flags, hipStream_t st);
+hipError_t launch_local_search(const DevBatch &b, int flags, const LsShape &sh, hipStream_t st);
 hipError_t launch_statistics(const DevBatch &b, float tick_to_s, hipStream_t st);
 hipError_t launch_pheromone_update(const DevBatch &b, hipStream_t st);
 hipError_t launch_gather(const DevBatch &b, hipStream_t st);
 hipError_t launch_pheromone_init(const DevBatch &b, int mode, float tau0, hipStream_t st);
 hipError_t launch_stamp_clock(const DevBatch &b, hipStream_t st);
 size_t construct_lds_bytes(int n, int selection);
-size_t local_search_lds_bytes(int n);
+size_t local_search_lds_bytes(int n, int ring);
+bool ls_shape_valid(const LsShape &sh);
 
 }  // namespace cvrp

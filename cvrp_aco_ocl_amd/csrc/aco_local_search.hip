@@ -1,45 +1,44 @@
-// Local search kernel (gfx950): one 256-thread block (4 wavefronts) per ant,
-// the ant's tour, per-route tables and a cache of distance rows in LDS.
+// Local search kernel (gfx950): one block of W wavefronts (W = 4 or 8) per
+// ant; the ant's tour, per-route tables and a ring of distance rows in LDS.
 //
 //   2-opt     the reference's GPU 2-opt (kernel.cl:126-319): per route,
 //             sequential scan order, first improvement over the nn list
 //             with radius pruning, don't-look bits.  Routes are independent
 //             (the depot's don't-look bit starts clear on route 0 and set on
-//             later routes), so thread r runs route r, r+256, ...  The nn
-//             row is read as aligned (id, distance) batches of four and the
-//             radius comes from the LDS edge array, so a candidate costs one
-//             global load only when it is evaluated.
+//             later routes); a group of 8 lanes searches one route.
 //   swap      LocalSearch::swap (localSearch.cpp:301-410): for each position
 //             i in order, the first j > i whose swap gains more than
 //             EPSILON and keeps loads and route lengths feasible.
 //   relocate  the same scan for insertion moves (neighbourSearch.cpp:280-293,
 //             move.cpp:32-51).
 //
-// swap and relocate run speculatively on four consecutive positions: wave w
-// scans position i0+w against the current tour (64 j per ballot).  The
-// smallest position with a move is exactly the sequential scan's next move,
-// because no earlier position moved; results of later positions are
-// discarded.  Distance rows d(v,.) of the nodes around i0 are cached in LDS
-// by node id and reused across rounds.
+// swap and relocate run speculatively on W consecutive positions: wave w
+// scans position i0+w against the current tour (4 x 64 j per loop trip).
+// The smallest position with a move is exactly the sequential scan's next
+// move, because no earlier position moved; results of later positions are
+// discarded.  Distance rows of the positions around i0 sit in an LDS ring
+// and the next round's rows are fetched while the current round scans.
+//
+// Launch shape (LsShape): W, the ring size and prefetch on/off; every shape
+// gives the same tours and the same evaluated-move counts.
 #include "aco_device.h"
 
 namespace cvrp {
 
 using u64 = unsigned long long;
 
-constexpr int kLsWaves = 4;
-constexpr int kLsThreads = kLsWaves * kLanes;
-constexpr int kRowSlots = 16;   // distance-row ring, slot = tour position mod ring size
 constexpr int kPrefetch = 4;    // row elements per thread held for the next round
-constexpr int kPrefetchMaxN = kLsThreads * kPrefetch;
-// LDS words of L.res: [0,4) and [4,8) per-wave scan results (accepted j)
-// of even and odd rounds, 8 route count, [12,16) and [16,20) the moves each
-// wave's scan evaluated up to its result.
-constexpr int kResR = 8, kResEv = 12;
+// LDS words of L.res for W waves: [0,W) and [W,2W) per-wave scan results
+// (accepted j) of even and odd rounds, [2W,4W) the moves each wave's scan
+// evaluated up to its result (same parity split), 4W the route count.
+constexpr int kResWords = 64;
 
-// Ring size: 16 rows when the next round's rows are prefetched (n <= 1024,
-// 6 current + 4 next positions), else 8 (6 current positions).
-__host__ __device__ constexpr int ls_ring_slots(int n) { return n <= kPrefetchMaxN ? kRowSlots : 8; }
+template <int W> struct LsW {
+    static constexpr int threads = W * kLanes;
+    static constexpr int res_ev = 2 * W;
+    static constexpr int res_r = 4 * W;
+    static constexpr int prefetch_max_n = threads * kPrefetch;
+};
 
 struct LsLds {
     int *tour, *tmp, *rl, *dem, *res;
@@ -54,7 +53,7 @@ struct LsOffsets {
     size_t tour, tmp, dtw, rl, rd, dem, rows, res, pos, rof, ridx, zpos, rc, dlb, total;
 };
 
-__host__ __device__ inline LsOffsets ls_offsets(int n)
+__host__ __device__ inline LsOffsets ls_offsets(int n, int ring)
 {
     const size_t T = 2 * (size_t)n, N = (size_t)n;
     LsOffsets o;
@@ -65,8 +64,8 @@ __host__ __device__ inline LsOffsets ls_offsets(int n)
     o.rl = at;   at = ls_align16(at + 4 * N);
     o.rd = at;   at = ls_align16(at + 4 * N);
     o.dem = at;  at = ls_align16(at + 4 * N);
-    o.rows = at; at = ls_align16(at + 4 * N * ls_ring_slots(n));
-    o.res = at;  at = ls_align16(at + 4 * 32);
+    o.rows = at; at = ls_align16(at + 4 * N * (size_t)ring);
+    o.res = at;  at = ls_align16(at + 4 * kResWords);
     o.pos = at;  at = ls_align16(at + 2 * N);
     o.rof = at;  at = ls_align16(at + 2 * N);
     o.ridx = at; at = ls_align16(at + 2 * T);
@@ -77,9 +76,9 @@ __host__ __device__ inline LsOffsets ls_offsets(int n)
     return o;
 }
 
-__device__ __forceinline__ LsLds ls_bind(int n, unsigned char *b)
+__device__ __forceinline__ LsLds ls_bind(int n, int ring, unsigned char *b)
 {
-    const LsOffsets o = ls_offsets(n);
+    const LsOffsets o = ls_offsets(n, ring);
     LsLds L;
     L.tour = (int *)(b + o.tour); L.tmp = (int *)(b + o.tmp); L.dtw = (float *)(b + o.dtw);
     L.rl = (int *)(b + o.rl); L.rd = (float *)(b + o.rd); L.dem = (int *)(b + o.dem);
@@ -89,14 +88,7 @@ __device__ __forceinline__ LsLds ls_bind(int n, unsigned char *b)
     return L;
 }
 
-// Bytes for any instance with at most n nodes: the ring shrinks above
-// kPrefetchMaxN, so the largest layout may belong to a smaller instance.
-size_t local_search_lds_bytes(int n)
-{
-    const size_t big = ls_offsets(n).total;
-    const size_t mid = ls_offsets(n < kPrefetchMaxN ? n : kPrefetchMaxN).total;
-    return big > mid ? big : mid;
-}
+size_t local_search_lds_bytes(int n, int ring) { return ls_offsets(n, ring).total; }
 
 __device__ __forceinline__ float Dd(const DevProblem &p, int a, int b)
 {
@@ -104,15 +96,17 @@ __device__ __forceinline__ float Dd(const DevProblem &p, int a, int b)
 }
 
 // dtw[q] = d(tour[q], tour[q+1]) for q in [lo, hi]
+template <int W>
 __device__ __forceinline__ void ls_edges(const DevProblem &p, LsLds &L, int lo, int hi, int tid)
 {
-    for (int q = lo + tid; q <= hi; q += kLsThreads) L.dtw[q] = Dd(p, L.tour[q], L.tour[q + 1]);
+    for (int q = lo + tid; q <= hi; q += LsW<W>::threads) L.dtw[q] = Dd(p, L.tour[q], L.tour[q + 1]);
 }
 
 // Wave 0: depot positions (zpos), customer positions and route ids (pos,
 // rof) and route index per position (ridx, counted as localSearch.cpp:319-337
 // does).  Then, with tables, the per-route load / distance+service /
 // customer count, one thread per route.  Returns R.
+template <int W>
 __device__ __forceinline__ int ls_routes(const DevProblem &p, LsLds &L, int S, int tid, bool tables)
 {
     if (tid < kLanes) {
@@ -134,12 +128,12 @@ __device__ __forceinline__ int ls_routes(const DevProblem &p, LsLds &L, int S, i
             }
             zc += __popcll(mz);
         }
-        if (lane == 0) L.res[kResR] = zc - 1;
+        if (lane == 0) L.res[LsW<W>::res_r] = zc - 1;
     }
     __syncthreads();
-    const int R = L.res[kResR];
+    const int R = L.res[LsW<W>::res_r];
     if (tables) {
-        for (int r = tid; r < R; r += kLsThreads) {
+        for (int r = tid; r < R; r += LsW<W>::threads) {
             const int beg = L.zpos[r], end = L.zpos[r + 1];
             int load = 0;
             float dist = 0.0f;
@@ -339,8 +333,9 @@ __device__ __forceinline__ void two_opt_route(const DevProblem &p, LsLds &L, int
 
 // ------------------------------------------------- distance row cache ----
 
-// Rows d(tour[q], .) live in slot q mod R of an LDS ring of R rows (R = 16
-// or 8).  The window of positions [base, base + R) maps one to one onto the
+// Rows d(tour[q], .) live in slot q mod R of an LDS ring of R rows (R = 8,
+// 16 or 32, a launch parameter; a round needs W+2 positions, W+2 more with
+// prefetch).  The window of positions [base, base + R) maps one to one onto the
 // slots; bit o of a validity mask says that the slot of position base + o
 // holds the row of the node now at that position.  Moving the window shifts
 // the mask, a move clears the bits of the positions it changed.  All of it
@@ -351,15 +346,13 @@ struct RowWindow {
     unsigned mask;
 };
 
-__device__ __forceinline__ int ring_slot(const DevProblem &p, int pos)
-{
-    return pos & (ls_ring_slots(p.n) - 1);
-}
-
-__device__ __forceinline__ const float *row_at(const DevProblem &p, const LsLds &L, int pos)
-{
-    return L.rows + ring_slot(p, pos) * p.n;
-}
+struct RowRing {
+    float *rows;
+    int mask;   // ring size - 1
+    int n;
+    __device__ __forceinline__ int slot(int pos) const { return pos & mask; }
+    __device__ __forceinline__ float *at(int pos) const { return rows + slot(pos) * n; }
+};
 
 // Start the window at position base (never moves backwards within a scan).
 __device__ __forceinline__ void window_move(RowWindow &w, int base)
@@ -379,67 +372,72 @@ __device__ __forceinline__ void window_clear(RowWindow &w, int lo, int hi)
     w.mask &= ~(span << a);
 }
 
-// Make the rows of window offsets [0, cnt) present (cnt <= 6): all threads
-// load the missing ones; a barrier follows when any was missing.
-__device__ __forceinline__ void rows_ensure(const DevProblem &p, LsLds &L, RowWindow &w, int cnt,
-                                            int tid)
+// Make the rows of window offsets [0, cnt) present (cnt <= W+2): all
+// threads load the missing ones; a barrier follows when any was missing.
+template <int W>
+__device__ __forceinline__ void rows_ensure(const DevProblem &p, const LsLds &L, const RowRing &rr,
+                                            RowWindow &w, int cnt, int tid)
 {
     const int n = p.n;
     bool loaded = false;
 #pragma unroll
-    for (int q = 0; q < kLsWaves + 2; ++q) {
+    for (int q = 0; q < W + 2; ++q) {
         if (q >= cnt) break;
         if ((w.mask >> q) & 1u) continue;
         const int pos = w.base + q;
         const int v = L.tour[pos];
-        float *row = L.rows + ring_slot(p, pos) * n;
+        float *row = rr.at(pos);
         const float *src = p.dist + (size_t)v * p.P;
-        for (int c = tid; c < n; c += kLsThreads) row[c] = src[c];
+        for (int c = tid; c < n; c += LsW<W>::threads) row[c] = src[c];
         loaded = true;
     }
     w.mask |= (1u << cnt) - 1u;
     if (loaded) __syncthreads();
 }
 
-// Next round's rows (window offsets [off, off+4), positions < S) are loaded
+// Next round's rows (window offsets [off, off+W), positions < S) are loaded
 // into registers before a scan and stored into the ring after it, so the
-// loads overlap the scan.  Only with the 16-row ring (n <= 1024, 4 elements
-// per thread); the 10 positions of this round and the next never share a
-// slot.
-struct RowPrefetch {
-    float v[kLsWaves][kPrefetch];
+// loads overlap the scan.  Enabled per launch when the ring holds both
+// rounds' positions (2W+2 <= R) and per instance when n <= 256 W (4
+// elements per thread): the positions of this round and the next never
+// share a slot.
+template <int W> struct RowPrefetch {
+    float v[W][kPrefetch];
     unsigned want;
 };
 
-__device__ __forceinline__ void prefetch_issue(const DevProblem &p, const LsLds &L, const RowWindow &w,
-                                               RowPrefetch &pf, int off, int S, int tid)
+template <int W>
+__device__ __forceinline__ void prefetch_issue(const DevProblem &p, const LsLds &L, bool on,
+                                               const RowWindow &w, RowPrefetch<W> &pf, int off,
+                                               int S, int tid)
 {
     pf.want = 0;
-    if (p.n > kPrefetchMaxN) return;
+    if (!on) return;
 #pragma unroll
-    for (int q = 0; q < kLsWaves; ++q) {
+    for (int q = 0; q < W; ++q) {
         const int pos = w.base + off + q;
         if (pos >= S || ((w.mask >> (off + q)) & 1u)) continue;
         pf.want |= 1u << q;
         const float *src = p.dist + (size_t)L.tour[pos] * p.P;
 #pragma unroll
         for (int e = 0; e < kPrefetch; ++e) {
-            const int c = tid + e * kLsThreads;
+            const int c = tid + e * LsW<W>::threads;
             pf.v[q][e] = c < p.n ? src[c] : 0.0f;
         }
     }
 }
 
-__device__ __forceinline__ void prefetch_store(const DevProblem &p, LsLds &L, RowWindow &w,
-                                               const RowPrefetch &pf, int off, int tid)
+template <int W>
+__device__ __forceinline__ void prefetch_store(const DevProblem &p, const RowRing &rr, RowWindow &w,
+                                               const RowPrefetch<W> &pf, int off, int tid)
 {
 #pragma unroll
-    for (int q = 0; q < kLsWaves; ++q) {
+    for (int q = 0; q < W; ++q) {
         if (!((pf.want >> q) & 1u)) continue;
-        float *row = L.rows + ring_slot(p, w.base + off + q) * p.n;
+        float *row = rr.at(w.base + off + q);
 #pragma unroll
         for (int e = 0; e < kPrefetch; ++e) {
-            const int c = tid + e * kLsThreads;
+            const int c = tid + e * LsW<W>::threads;
             if (c < p.n) row[c] = pf.v[q][e];
         }
     }
@@ -517,8 +515,9 @@ __device__ __forceinline__ int swap_scan(const DevProblem &p, const LsLds &L, in
     return -1;
 }
 
-__device__ __forceinline__ void ls_swap(const DevProblem &p, LsLds &L, int S, int tid, long long &ev,
-                                        int &moves)
+template <int W>
+__device__ __forceinline__ void ls_swap(const DevProblem &p, LsLds &L, const RowRing &rr, bool pf_on,
+                                        int S, int tid, long long &ev, int &moves)
 {
     const int lane = tid & 63, w = tid >> 6;
     const int cap = kMaxMovesFactor * p.n;
@@ -527,22 +526,22 @@ __device__ __forceinline__ void ls_swap(const DevProblem &p, LsLds &L, int S, in
     while (i0 < S) {
         // rows of positions i0-1 .. i0+W (pred, self, succ of each scanned i)
         window_move(win, i0 - 1);
-        rows_ensure(p, L, win, min(kLsWaves + 2, S - i0 + 1), tid);
-        RowPrefetch pf;
-        prefetch_issue(p, L, win, pf, kLsWaves + 2, S, tid);
+        rows_ensure<W>(p, L, rr, win, min(W + 2, S - i0 + 1), tid);
+        RowPrefetch<W> pf;
+        prefetch_issue<W>(p, L, pf_on, win, pf, W + 2, S, tid);
         const int i = i0 + w;
         int j = -1, evc = 0;
         if (i < S && L.tour[i] != 0)
-            j = swap_scan(p, L, S, i, row_at(p, L, i - 1), row_at(p, L, i), row_at(p, L, i + 1), lane, evc);
-        int *res = L.res + 4 * parity, *rese = L.res + kResEv + 4 * parity;
+            j = swap_scan(p, L, S, i, rr.at(i - 1), rr.at(i), rr.at(i + 1), lane, evc);
+        int *res = L.res + W * parity, *rese = L.res + LsW<W>::res_ev + W * parity;
         parity ^= 1;
         if (lane == 0) { res[w] = j; rese[w] = evc; }
-        prefetch_store(p, L, win, pf, kLsWaves + 2, tid);
+        prefetch_store<W>(p, rr, win, pf, W + 2, tid);
         __syncthreads();
         // the sequential scan evaluates the waves' positions in order and
         // stops at the first accepted move
         int wstar = -1, e = 0;
-        for (int q = 0; q < kLsWaves; ++q) {
+        for (int q = 0; q < W; ++q) {
             e += rese[q];
             if (res[q] >= 0) { wstar = q; break; }
         }
@@ -550,12 +549,12 @@ __device__ __forceinline__ void ls_swap(const DevProblem &p, LsLds &L, int S, in
         if (wstar < 0) {
             // no barrier: the next round writes the other result buffer, and
             // rows it loads belong to positions no wave still reads
-            i0 += kLsWaves;
+            i0 += W;
             continue;
         }
         const int is = i0 + wstar, js = res[wstar];
         if (tid == 0) {
-            const float *rowA = row_at(p, L, is - 1), *rowB = row_at(p, L, is), *rowC = row_at(p, L, is + 1);
+            const float *rowA = rr.at(is - 1), *rowB = rr.at(is), *rowC = rr.at(is + 1);
             const int n1 = L.tour[is], sn1 = L.tour[is + 1], r1 = L.ridx[is];
             const int n2 = L.tour[js], pn2 = L.tour[js - 1], sn2 = L.tour[js + 1], r2 = L.ridx[js];
             const float a1 = rowA[n1], b1 = rowB[sn1];
@@ -655,8 +654,9 @@ __device__ __forceinline__ int relocate_scan(const DevProblem &p, const LsLds &L
     return -1;
 }
 
-__device__ __forceinline__ void ls_relocate(const DevProblem &p, LsLds &L, int S, int tid,
-                                            long long &ev, int &moves)
+template <int W>
+__device__ __forceinline__ void ls_relocate(const DevProblem &p, LsLds &L, const RowRing &rr,
+                                            bool pf_on, int S, int tid, long long &ev, int &moves)
 {
     const int lane = tid & 63, w = tid >> 6;
     const int cap = kMaxMovesFactor * p.n;
@@ -665,22 +665,22 @@ __device__ __forceinline__ void ls_relocate(const DevProblem &p, LsLds &L, int S
     while (i0 < S - 1 && done < cap) {
         // rows of positions i0-1 .. i0+W-1: d(pred, .) gives c1, d(n1, .) the gains
         window_move(win, i0 - 1);
-        rows_ensure(p, L, win, min(kLsWaves + 1, S - i0), tid);
-        RowPrefetch pf;
-        prefetch_issue(p, L, win, pf, kLsWaves + 1, S - 1, tid);
+        rows_ensure<W>(p, L, rr, win, min(W + 1, S - i0), tid);
+        RowPrefetch<W> pf;
+        prefetch_issue<W>(p, L, pf_on, win, pf, W + 1, S - 1, tid);
         const int i = i0 + w;
         int j = -1, evc = 0;
         if (i < S - 1 && L.tour[i] != 0 && L.rc[L.ridx[i]] >= 2)
-            j = relocate_scan(p, L, S, i, row_at(p, L, i - 1), row_at(p, L, i), lane, evc);
-        int *res = L.res + 4 * parity, *rese = L.res + kResEv + 4 * parity;
+            j = relocate_scan(p, L, S, i, rr.at(i - 1), rr.at(i), lane, evc);
+        int *res = L.res + W * parity, *rese = L.res + LsW<W>::res_ev + W * parity;
         parity ^= 1;
         if (lane == 0) { res[w] = j; rese[w] = evc; }
-        prefetch_store(p, L, win, pf, kLsWaves + 1, tid);
+        prefetch_store<W>(p, rr, win, pf, W + 1, tid);
         __syncthreads();
         // the sequential scan evaluates the waves' positions in order and
         // stops at the first accepted move
         int wstar = -1, e = 0;
-        for (int q = 0; q < kLsWaves; ++q) {
+        for (int q = 0; q < W; ++q) {
             e += rese[q];
             if (res[q] >= 0) { wstar = q; break; }
         }
@@ -688,14 +688,14 @@ __device__ __forceinline__ void ls_relocate(const DevProblem &p, LsLds &L, int S
         if (wstar < 0) {
             // no barrier: the next round writes the other result buffer, and
             // rows it loads belong to positions no wave still reads
-            i0 += kLsWaves;
+            i0 += W;
             continue;
         }
         const int is = i0 + wstar, js = res[wstar];
         const int n1 = L.tour[is];
         const int lo = is < js ? is : js, hi = is < js ? js : is;
         if (tid == 0) {
-            const float *rowP = row_at(p, L, is - 1), *rowB = row_at(p, L, is);
+            const float *rowP = rr.at(is - 1), *rowB = rr.at(is);
             const int r1 = L.ridx[is], sn1 = L.tour[is + 1];
             const int n2 = L.tour[js], pn2 = L.tour[js - 1], sn2 = L.tour[js + 1], r2 = L.ridx[js];
             const float a1 = L.dtw[is - 1], b1 = L.dtw[is], c1 = rowP[sn1];
@@ -716,17 +716,17 @@ __device__ __forceinline__ void ls_relocate(const DevProblem &p, LsLds &L, int S
             }
         }
         // shift: copy [lo, hi] aside, then write it back moved by one
-        for (int q = lo + tid; q <= hi; q += kLsThreads) L.tmp[q] = L.tour[q];
+        for (int q = lo + tid; q <= hi; q += LsW<W>::threads) L.tmp[q] = L.tour[q];
         __syncthreads();
-        for (int q = lo + tid; q <= hi; q += kLsThreads) {
+        for (int q = lo + tid; q <= hi; q += LsW<W>::threads) {
             int v;
             if (is < js) v = q == js ? n1 : L.tmp[q + 1];
             else v = q == js ? n1 : L.tmp[q - 1];
             L.tour[q] = v;
         }
         __syncthreads();
-        ls_edges(p, L, lo - 1, hi, tid);
-        ls_routes(p, L, S, tid, false);  // refreshes ridx (and pos/zpos)
+        ls_edges<W>(p, L, lo - 1, hi, tid);
+        ls_routes<W>(p, L, S, tid, false);  // refreshes ridx (and pos/zpos)
         window_clear(win, lo, hi);
         i0 = is;
         ++done;
@@ -736,18 +736,22 @@ __device__ __forceinline__ void ls_relocate(const DevProblem &p, LsLds &L, int S
 
 // ------------------------------------------------------------ kernel ----
 
-__global__ __launch_bounds__(kLsThreads) void k_local_search(DevBatch bt, int flags)
+template <int W>
+__global__ __launch_bounds__(W * 64) void k_local_search(DevBatch bt, int flags, int ring, int prefetch)
 {
+    constexpr int NT = LsW<W>::threads;
     extern __shared__ __align__(16) unsigned char smem[];
     const int inst = bt.ant_inst[blockIdx.x];
     const DevProblem p = bt.probs[inst];
     const DevState s = bt.states[inst];
     const int a = blockIdx.x - bt.ant_base[inst], tid = threadIdx.x;
-    LsLds L = ls_bind(p.n, smem);
+    LsLds L = ls_bind(p.n, ring, smem);
+    const RowRing rr = {L.rows, ring - 1, p.n};
+    const bool pf_on = prefetch && p.n <= LsW<W>::prefetch_max_n;
     const int S = s.sizes[a];
     int *gt = s.tours + (size_t)a * p.T;
-    for (int q = tid; q < S; q += kLsThreads) L.tour[q] = gt[q];
-    for (int v = tid; v < p.n; v += kLsThreads) {
+    for (int q = tid; q < S; q += NT) L.tour[q] = gt[q];
+    for (int v = tid; v < p.n; v += NT) {
         L.dlb[v] = 0;
         L.dem[v] = p.dem[v];
     }
@@ -755,36 +759,36 @@ __global__ __launch_bounds__(kLsThreads) void k_local_search(DevBatch bt, int fl
     long long ev = 0;
     int moves = 0;
     if (flags & kLs2Opt) {
-        ls_edges(p, L, 0, S - 2, tid);
-        const int R = ls_routes(p, L, S, tid, false);
+        ls_edges<W>(p, L, 0, S - 2, tid);
+        const int R = ls_routes<W>(p, L, S, tid, false);
         const int grp = tid / kOptGroup, gl = tid % kOptGroup;
-        for (int r = grp; r < R; r += kLsThreads / kOptGroup)
+        for (int r = grp; r < R; r += NT / kOptGroup)
             two_opt_route(p, L, r, L.zpos[r], L.zpos[r + 1] - 1, gl, ev, moves);
         __syncthreads();
     }
     if (flags & kLsSwap) {
-        ls_edges(p, L, 0, S - 2, tid);
+        ls_edges<W>(p, L, 0, S - 2, tid);
         __syncthreads();
-        ls_routes(p, L, S, tid, true);
-        ls_swap(p, L, S, tid, ev, moves);
+        ls_routes<W>(p, L, S, tid, true);
+        ls_swap<W>(p, L, rr, pf_on, S, tid, ev, moves);
     }
     if (flags & kLsRelocate) {
-        ls_edges(p, L, 0, S - 2, tid);
+        ls_edges<W>(p, L, 0, S - 2, tid);
         __syncthreads();
-        ls_routes(p, L, S, tid, true);
-        ls_relocate(p, L, S, tid, ev, moves);
+        ls_routes<W>(p, L, S, tid, true);
+        ls_relocate<W>(p, L, rr, pf_on, S, tid, ev, moves);
     }
-    ls_edges(p, L, 0, S - 2, tid);
+    ls_edges<W>(p, L, 0, S - 2, tid);
     __syncthreads();
-    for (int q = tid; q < S; q += kLsThreads) gt[q] = L.tour[q];
+    for (int q = tid; q < S; q += NT) gt[q] = L.tour[q];
     // tour length in position order (vrpHelper.cpp:244-253)
     if (tid == 0) {
         float len = 0.0f;
         for (int q = 0; q < S - 1; ++q) len += L.dtw[q];
         s.lens[a] = len;
     }
-    __shared__ long long red_ev[kLsWaves];
-    __shared__ int red_mv[kLsWaves];
+    __shared__ long long red_ev[W];
+    __shared__ int red_mv[W];
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
         ev += __shfl_xor(ev, o);
@@ -798,17 +802,32 @@ __global__ __launch_bounds__(kLsThreads) void k_local_search(DevBatch bt, int fl
     if (tid == 0) {
         long long e = 0;
         int mv = 0;
-        for (int q = 0; q < kLsWaves; ++q) { e += red_ev[q]; mv += red_mv[q]; }
+        for (int q = 0; q < W; ++q) { e += red_ev[q]; mv += red_mv[q]; }
         s.ant_evals[a] = e;
         s.ant_moves[a] = mv;
     }
 }
 
-hipError_t launch_local_search(const DevBatch &b, int flags, hipStream_t st)
+bool ls_shape_valid(const LsShape &sh)
+{
+    if (sh.waves != 4 && sh.waves != 8) return false;
+    if (sh.ring != 8 && sh.ring != 16 && sh.ring != 32) return false;
+    if (sh.ring < sh.waves + 2) return false;
+    if (sh.prefetch && sh.ring < 2 * sh.waves + 2) return false;
+    return true;
+}
+
+hipError_t launch_local_search(const DevBatch &b, int flags, const LsShape &sh, hipStream_t st)
 {
     if (!flags) return hipSuccess;
-    hipLaunchKernelGGL(k_local_search, dim3(b.ants), dim3(kLsThreads), local_search_lds_bytes(b.max_n),
-                       st, b, flags);
+    if (!ls_shape_valid(sh)) return hipErrorInvalidValue;
+    const size_t lds = local_search_lds_bytes(b.max_n, sh.ring);
+    if (sh.waves == 8)
+        hipLaunchKernelGGL(k_local_search<8>, dim3(b.ants), dim3(8 * kLanes), lds, st, b, flags, sh.ring,
+                           sh.prefetch);
+    else
+        hipLaunchKernelGGL(k_local_search<4>, dim3(b.ants), dim3(4 * kLanes), lds, st, b, flags, sh.ring,
+                           sh.prefetch);
     return hipGetLastError();
 }
 

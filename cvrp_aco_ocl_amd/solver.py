@@ -258,6 +258,27 @@ class GpuACO:
         return {"ls_evals": c.ls_evals, "ls_moves": c.ls_moves, "fallbacks": c.fallbacks,
                 "iterations": c.iterations, "bsf_overflow": c.bsf_overflow}
 
+    # -- launch shapes and the autotuner ---------------------------------
+    def launch_shape(self) -> tuple:
+        sh = _lib.AcoLaunchShape()
+        check(lib.aco_get_launch_shape(self.h, C.byref(sh)), "aco_get_launch_shape")
+        return (sh.ls_waves, sh.ls_ring, sh.ls_prefetch)
+
+    def set_launch_shape(self, waves: int, ring: int, prefetch: int):
+        sh = _lib.AcoLaunchShape(waves, ring, prefetch)
+        check(lib.aco_set_launch_shape(self.h, C.byref(sh)), "aco_set_launch_shape")
+
+    def autotune(self, reps: int = 3):
+        """Times every candidate local-search shape on the current state and
+        keeps the fastest (state and results unchanged).  Returns
+        [((waves, ring, prefetch), ms or -1)]."""
+        res = (_lib.AcoTuneResult * 16)()
+        cnt = C.c_int32()
+        check(lib.aco_autotune(self.h, reps, C.cast(res, C.c_void_p), 16, C.byref(cnt)),
+              "aco_autotune")
+        return [((r.shape.ls_waves, r.shape.ls_ring, r.shape.ls_prefetch), r.ms)
+                for r in res[: cnt.value]]
+
     def profile_enable(self, on: bool = True):
         check(lib.aco_profile_enable(self.h, int(on)), "aco_profile_enable")
 

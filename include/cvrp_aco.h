@@ -167,6 +167,33 @@ int aco_profile_enable(aco_ctx *ctx, int32_t enable);
  * returns total milliseconds and launch count since enabling */
 int aco_profile_get(aco_ctx *ctx, const char *name, double *total_ms, int64_t *launches);
 
+/* ---- launch shapes and the autotuner (no counterpart in the reference,
+ * whose launch geometry is fixed in g_aco.cpp:141-177,279-318) ----
+ * Local-search kernel shape: waves per ant (4 or 8), rows in the LDS
+ * distance-row ring (8, 16 or 32; >= waves + 2) and prefetch of the next
+ * round's rows (needs ring >= 2 waves + 2; applies to instances with
+ * n <= 256 waves).  Every valid shape gives identical results. */
+typedef struct aco_launch_shape {
+    int32_t ls_waves;
+    int32_t ls_ring;
+    int32_t ls_prefetch;
+} aco_launch_shape;
+
+typedef struct aco_tune_result {
+    aco_launch_shape shape;
+    float ms;            /* mean local-search launch time, -1 if it does not fit */
+} aco_tune_result;
+
+int aco_get_launch_shape(aco_ctx *ctx, aco_launch_shape *shape);
+/* validates the shape and its LDS size; the iteration graph is re-captured */
+int aco_set_launch_shape(aco_ctx *ctx, const aco_launch_shape *shape);
+/* After aco_init: constructs once from the current state, times every
+ * candidate shape `reps` times on those same tours, keeps the fastest and
+ * restores the state, so the colony's results do not change.  results
+ * (optional, capacity entries) receives every candidate's time. */
+int aco_autotune(aco_ctx *ctx, int32_t reps, aco_tune_result *results, int32_t capacity,
+                 int32_t *count);
+
 const char *aco_last_error(void);
 
 /* ---- host-side instance preparation (no device needed) ---- */

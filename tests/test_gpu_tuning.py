@@ -1,0 +1,88 @@
+"""Launch shapes and the autotuner: every local-search shape gives the same
+tours, lengths and evaluated-move counts, and autotuning leaves the
+colony's trajectory unchanged."""
+import numpy as np
+import pytest
+
+import cvrp_aco_ocl_amd as ca
+from cvrp_aco_ocl_amd import instances as I
+
+from helpers import log
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(4, 16, 1), (4, 16, 0), (4, 8, 0), (8, 32, 1), (8, 16, 0)]
+
+
+def _small():
+    return [I.uniform_100(1), I.clustered(2, customers=480), I.batch(4, count=2)[1]]
+
+
+def _large():
+    # n = 1101: past the 4-wave prefetch limit (n <= 1024), so those ants
+    # run without prefetch while the 100-node instance prefetches
+    return [I.clustered(3, customers=1100, routes=80), I.uniform_100(2)]
+
+
+def _ls_once(insts, shape):
+    with ca.GpuACO(insts, n_ants=12, k=16, seed=7, selection=ca.SEL_KNN, rng=ca.RNG_PHILOX,
+                   ls_flags=7, use_graph=False) as g:
+        g.init_aco()
+        if shape is not None:
+            g.set_launch_shape(*shape)
+        g.construct_solutions()
+        g.local_search()
+        out = [g.ants(b) for b in range(len(insts))]
+        g.update_statistics()
+        return out, g.counters()["ls_evals"]
+
+
+@pytest.mark.parametrize("which", ["small", "large"])
+def test_every_shape_gives_identical_results(which):
+    insts = _small() if which == "small" else _large()
+    ref, ref_ev = _ls_once(insts, SHAPES[0])
+    assert ref_ev > 0
+    for sh in SHAPES[1:]:
+        if which == "large" and sh == (8, 32, 1):
+            # 32 rows of 1101 floats do not fit in 160 KB of LDS
+            with pytest.raises(ca.AcoError):
+                _ls_once(insts, sh)
+            continue
+        log(f"{which} shape {sh}")
+        got, ev = _ls_once(insts, sh)
+        assert ev == ref_ev, (sh, ev, ref_ev)
+        for b in range(len(insts)):
+            for x, y in zip(got[b], ref[b]):
+                np.testing.assert_array_equal(x, y, err_msg=f"shape {sh} instance {b}")
+
+
+def test_invalid_shapes_rejected():
+    inst = I.uniform_100(1)
+    with ca.GpuACO(inst, n_ants=8, ls_flags=7) as g:
+        assert g.launch_shape() == (4, 16, 1)
+        for bad in [(2, 16, 0), (4, 4, 0), (4, 8, 1), (8, 8, 0), (8, 16, 1), (4, 12, 0)]:
+            with pytest.raises(ca.AcoError):
+                g.set_launch_shape(*bad)
+        assert g.launch_shape() == (4, 16, 1)
+
+
+def test_autotune_keeps_the_trajectory():
+    insts = I.batch(5, count=6)
+    runs = []
+    for tune in (False, True):
+        with ca.GpuACO(insts, n_ants=16, k=16, seed=3, selection=ca.SEL_KNN, rng=ca.RNG_PHILOX,
+                       ls_flags=7) as g:
+            g.init_aco()
+            if tune:
+                res = g.autotune(reps=2)
+                log(f"autotune: {res} -> {g.launch_shape()}")
+                assert any(ms > 0 for _, ms in res)
+                assert g.launch_shape() in [sh for sh, ms in res if ms > 0]
+            g.run(4)
+            runs.append([(g.ants(b), g.pheromone(b), g.bsf_records(b)) for b in range(len(insts))])
+    for b in range(len(insts)):
+        (a0, p0, r0), (a1, p1, r1) = runs[0][b], runs[1][b]
+        for x, y in zip(a0, a1):
+            np.testing.assert_array_equal(x, y)
+        np.testing.assert_array_equal(p0[0], p1[0])
+        assert [r[:2] for r in r0] == [r[:2] for r in r1]
