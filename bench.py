@@ -69,6 +69,22 @@ def make_solver(ca, inst, seed):
                      use_graph=True)
 
 
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of ``kernel`` on this workload from the committed
+    FETCH_SIZE / WRITE_SIZE profile (calibrated against a plain copy), or
+    None when there is no measurement for it."""
+    path = os.path.join(ROOT, "profiles", "traffic_clustered481.md")
+    key = f"bench: {kernel}_traffic_bytes="
+    try:
+        with open(path) as fh:
+            for line in fh:
+                if line.startswith(key):
+                    return int(line[len(key):].strip())
+    except OSError:
+        pass
+    return None
+
+
 def construct_bytes(inst, sizes, m, k, fallbacks):
     """Algorithmic bytes of one construction launch: per step k candidate
     ids + k total_info + the chosen distance + the tour write (8k+8 B), per
@@ -171,7 +187,8 @@ def main():
         achieved = traffic_alg / (avg[dominant] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 2),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                "traffic": None, "algorithmic_bytes_per_launch": traffic_alg}
+                "traffic": measured_traffic(dominant) if args.seed == 0 else None,
+                "algorithmic_bytes_per_launch": traffic_alg}
 
     # ---- best-so-far trajectory: time to 2% of the run's final best ----
     quality = None

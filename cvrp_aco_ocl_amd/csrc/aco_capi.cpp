@@ -992,3 +992,28 @@ int aco_profile_get(aco_ctx *c, const char *name, double *total_ms, int64_t *lau
 }
 
 }  // extern "C"
+
+int cvrp::ctx_prior_view(aco_ctx *c, std::vector<CtxInstanceView> &out, void **stream)
+{
+    int rc = need(c, false);
+    if (rc) return rc;
+    if (c->inited) return set_error(ACO_ERR_STATE, "the prior must be applied before aco_init");
+    out.clear();
+    for (auto &I : c->inst) {
+        CtxInstanceView v;
+        v.n = I.n;
+        v.P = I.P;
+        v.Q = I.Q;
+        v.dist = I.dp.dist;
+        v.dem = I.dp.dem;
+        v.nnk = c->prm.k > 0 ? I.dp.nnk : nullptr;
+        v.k_ctx = c->prm.k;
+        v.nnls = I.dp.nnls;
+        v.nnP = I.dp.nnP;
+        v.nn_ls = I.dp.nn_ls;
+        v.heur = const_cast<float *>(I.dp.heur);
+        out.push_back(v);
+    }
+    *stream = (void *)c->stream;
+    return ACO_OK;
+}

@@ -34,4 +34,18 @@ float nn_tour_length(int n, const float *dist, const int *dem, int Q, float D, f
 void tree_rank(int N, int *rank);
 int read_vrplib(const char *path, VrpInstance &inst, std::string &err);
 
+// Device view of one instance of a context, for aco_prior_apply.
+struct CtxInstanceView {
+    int n, P, Q;
+    const float *dist;         // n x P
+    const int *dem;            // n
+    const int *nnk;            // n x k_ctx (construction candidates), or null
+    int k_ctx;
+    const int *nnls;           // n x nnP (2-opt neighbour rows)
+    int nnP, nn_ls;
+    float *heur;               // n x P
+};
+// Fails unless the context exists and aco_init has not run yet.
+int ctx_prior_view(aco_ctx *c, std::vector<CtxInstanceView> &out, void **stream);
+
 }  // namespace cvrp

@@ -364,7 +364,8 @@ __global__ void k_pack_features(const float *feat, long long E, unsigned short *
     xb[t] = f < NF ? f2bf(feat[e * NF + f]) : (unsigned short)0;
 }
 
-__global__ void k_node_stats(int n, const float *x, const float *y, const float *dist,
+// dist rows have pitch P (>= n)
+__global__ void k_node_stats(int n, int P, const float *x, const float *y, const float *dist,
                              const int *nn, int nnw, int k, float *rad, float *ang, float *kd)
 {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
@@ -372,7 +373,7 @@ __global__ void k_node_stats(int n, const float *x, const float *y, const float 
     const float dx = x[v] - x[0], dy = y[v] - y[0];
     rad[v] = sqrtf(dx * dx + dy * dy);
     ang[v] = atan2f(dy, dx) / 3.14159265358979323846f;
-    kd[v] = dist[(size_t)v * n + nn[(size_t)v * nnw + k - 1]];
+    kd[v] = dist[(size_t)v * P + nn[(size_t)v * nnw + k - 1]];
 }
 
 // One block: R = max radius (1 if 0), mkd = mean k-th candidate distance
@@ -406,7 +407,7 @@ __global__ void k_node_reduce(int n, const float *rad, const float *kd, float *o
     }
 }
 
-__global__ void k_edge_features(int n, int k, const float *dist, const int *nn, int nnw,
+__global__ void k_edge_features(int n, int P, int k, const float *dist, const int *nn, int nnw,
                                 const int *dem, float Q, const float *rad, const float *ang,
                                 const float *kd, const float *Rm, float *feat, unsigned short *xb)
 {
@@ -415,7 +416,7 @@ __global__ void k_edge_features(int n, int k, const float *dist, const int *nn, 
     const int i = (int)(t / k), r = (int)(t % k);
     const int j = nn[(size_t)i * nnw + r];
     const float R = Rm[0], mkd = Rm[1];
-    const float dij = dist[(size_t)i * n + j];
+    const float dij = dist[(size_t)i * P + j];
     float f[NF];
     f[0] = dij / mkd;
     f[1] = (float)r / (float)k;
@@ -459,6 +460,16 @@ __global__ void k_prior_scatter(int n, int k, const int *nn, int nnw, const floa
     const int j = nn[(size_t)i * nnw + r];
     const float p = 1.0f / (1.0f + expf(-logits[t]));
     out[(size_t)i * n + j] = prior_pow(p, ex);
+}
+
+// heur[i][j] (pitch P) *= prior[i][j] (pitch n), j < n
+__global__ void k_heur_mul(int n, int P, const float *prior, float *heur)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)n * n) return;
+    const int i = (int)(t / n), j = (int)(t % n);
+    const size_t e = (size_t)i * P + j;
+    heur[e] = heur[e] * prior[t];
 }
 
 // ------------------------------------------------------------------- host --
@@ -713,14 +724,14 @@ static int device_features(hipStream_t st, int32_t n, const float *x, const floa
     PCHK(hipMemcpyAsync(dd.p, dist, N * N * 4, hipMemcpyHostToDevice, st));
     PCHK(hipMemcpyAsync(demd.p, demand, N * 4, hipMemcpyHostToDevice, st));
     PCHK(hipMemcpyAsync(nn_d.p, nn, N * nnw * 4, hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_node_stats, dim3((n + 255) / 256), dim3(256), 0, st, n, xd.as<float>(),
+    hipLaunchKernelGGL(k_node_stats, dim3((n + 255) / 256), dim3(256), 0, st, n, n, xd.as<float>(),
                        yd.as<float>(), dd.as<float>(), nn_d.as<int>(), nnw, k, rad.as<float>(),
                        ang.as<float>(), kd.as<float>());
     PCHK(hipGetLastError());
     hipLaunchKernelGGL(k_node_reduce, dim3(1), dim3(256), 0, st, n, rad.as<float>(), kd.as<float>(),
                        Rm.as<float>());
     PCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_edge_features, dim3((unsigned)((E + 255) / 256)), dim3(256), 0, st, n, k,
+    hipLaunchKernelGGL(k_edge_features, dim3((unsigned)((E + 255) / 256)), dim3(256), 0, st, n, n, k,
                        dd.as<float>(), nn_d.as<int>(), nnw, demd.as<int>(), (float)capacity,
                        rad.as<float>(), ang.as<float>(), kd.as<float>(), Rm.as<float>(),
                        want_f32 ? feat_d.as<float>() : nullptr, xb.as<unsigned short>());
