@@ -141,6 +141,7 @@ _SIGNATURES = {
     "aco_prior_train": (C.c_int, [_P, _P, _P, C.c_int64, C.POINTER(AcoPriorTrainParams), _P]),
     "aco_prior_build": (C.c_int, [_P, C.c_int32, _P, _P, _P, C.c_int32, _P, _P, C.c_int32,
                                   C.c_int32, C.c_float, C.c_float, _P]),
+    "aco_prior_apply": (C.c_int, [_P, _P, _P, _P, C.c_int32, C.c_float, C.c_float]),
 }
 
 EXPORTED = tuple(_SIGNATURES)

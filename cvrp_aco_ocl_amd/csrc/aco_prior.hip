@@ -576,16 +576,22 @@ static int stage_features(aco_prior *net, const float *feat, long long count, De
     return ACO_OK;
 }
 
-static int launch_forward(aco_prior *net, const unsigned short *xb, long long count, float *logits)
+static int launch_forward(aco_prior *net, const unsigned short *xb, long long count, float *logits,
+                          hipStream_t st = nullptr)
 {
     const long long tiles = (count + 15) / 16;
     long long blocks = (tiles + kWaves - 1) / kWaves;
     if (blocks > 2048) blocks = 2048;
     if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(k_forward, dim3((unsigned)blocks), dim3(kThreads), 0, net->stream, xb, count,
-                       net->wb, net->pf, logits);
+    hipLaunchKernelGGL(k_forward, dim3((unsigned)blocks), dim3(kThreads), 0, st ? st : net->stream, xb,
+                       count, net->wb, net->pf, logits);
     PCHK(hipGetLastError());
     return ACO_OK;
+}
+
+static float floor_pow(float floor, float exponent)
+{
+    return exponent == 1.0f ? floor : exponent == 2.0f ? floor * floor : powf(floor, exponent);
 }
 
 // One batch: fused forward/backward, then the ordered reduce (+ Adam).
@@ -895,7 +901,7 @@ int aco_prior_build(aco_prior *net, int32_t n, const float *x, const float *y,
     PCHK(dalloc(out, (size_t)NN * 4));
     rc = launch_forward(net, xb.as<unsigned short>(), E, lg.as<float>());
     if (rc != ACO_OK) return rc;
-    const float fl = exponent == 1.0f ? floor : exponent == 2.0f ? floor * floor : powf(floor, exponent);
+    const float fl = floor_pow(floor, exponent);
     hipLaunchKernelGGL(k_prior_fill, dim3((unsigned)((NN + 255) / 256)), dim3(256), 0, net->stream, NN, fl,
                        out.as<float>());
     PCHK(hipGetLastError());
@@ -904,6 +910,73 @@ int aco_prior_build(aco_prior *net, int32_t n, const float *x, const float *y,
     PCHK(hipGetLastError());
     PCHK(hipMemcpyAsync(prior, out.p, (size_t)NN * 4, hipMemcpyDeviceToHost, net->stream));
     PCHK(hipStreamSynchronize(net->stream));
+    return ACO_OK;
+}
+
+int aco_prior_apply(aco_ctx *ctx, aco_prior *net, const float *const *x, const float *const *y,
+                    int32_t k, float exponent, float floor)
+{
+    if (!ctx || !net || !x || !y) return set_error(ACO_ERR_ARG, "null argument");
+    if (!(exponent >= 0.0f) || !std::isfinite(exponent)) return set_error(ACO_ERR_ARG, "exponent must be >= 0");
+    if (!(floor > 0.0f && floor <= 1.0f)) return set_error(ACO_ERR_ARG, "floor must be in (0, 1]");
+    std::vector<CtxInstanceView> views;
+    void *sv = nullptr;
+    int rc = ctx_prior_view(ctx, views, &sv);
+    if (rc != ACO_OK) return rc;
+    hipStream_t st = (hipStream_t)sv;
+    int max_n = 0;
+    for (size_t b = 0; b < views.size(); ++b) {
+        const CtxInstanceView &v = views[b];
+        if (!x[b] || !y[b]) return set_error(ACO_ERR_ARG, "null coordinates for instance " + std::to_string(b));
+        if (k < 1 || k > v.n - 1) return set_error(ACO_ERR_ARG, "need 1 <= k < n");
+        if (!((v.nnk && k <= v.k_ctx) || k <= v.nn_ls))
+            return set_error(ACO_ERR_ARG, "k exceeds the context's candidate and neighbour lists");
+        max_n = std::max(max_n, v.n);
+    }
+    const size_t N = (size_t)max_n, E = N * (size_t)k;
+    DevBuf xd, yd, rad, ang, kd, Rm, xb, lg, dense;
+    PCHK(dalloc(xd, N * 4));
+    PCHK(dalloc(yd, N * 4));
+    PCHK(dalloc(rad, N * 4));
+    PCHK(dalloc(ang, N * 4));
+    PCHK(dalloc(kd, N * 4));
+    PCHK(dalloc(Rm, 16));
+    PCHK(dalloc(xb, E * NFP * 2));
+    PCHK(dalloc(lg, E * 4));
+    PCHK(dalloc(dense, N * N * 4));
+    const float fl = floor_pow(floor, exponent);
+    for (size_t b = 0; b < views.size(); ++b) {
+        const CtxInstanceView &v = views[b];
+        const int n = v.n;
+        const bool use_k = v.nnk && k <= v.k_ctx;
+        const int *nn = use_k ? v.nnk : v.nnls;
+        const int nnw = use_k ? v.k_ctx : v.nnP;
+        const long long En = (long long)n * k, NN = (long long)n * n;
+        PCHK(hipMemcpyAsync(xd.p, x[b], (size_t)n * 4, hipMemcpyHostToDevice, st));
+        PCHK(hipMemcpyAsync(yd.p, y[b], (size_t)n * 4, hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_node_stats, dim3((n + 255) / 256), dim3(256), 0, st, n, v.P, xd.as<float>(),
+                           yd.as<float>(), v.dist, nn, nnw, k, rad.as<float>(), ang.as<float>(), kd.as<float>());
+        PCHK(hipGetLastError());
+        hipLaunchKernelGGL(k_node_reduce, dim3(1), dim3(256), 0, st, n, rad.as<float>(), kd.as<float>(),
+                           Rm.as<float>());
+        PCHK(hipGetLastError());
+        hipLaunchKernelGGL(k_edge_features, dim3((unsigned)((En + 255) / 256)), dim3(256), 0, st, n, v.P, k,
+                           v.dist, nn, nnw, v.dem, (float)v.Q, rad.as<float>(), ang.as<float>(),
+                           kd.as<float>(), Rm.as<float>(), nullptr, xb.as<unsigned short>());
+        PCHK(hipGetLastError());
+        rc = launch_forward(net, xb.as<unsigned short>(), En, lg.as<float>(), st);
+        if (rc != ACO_OK) return rc;
+        hipLaunchKernelGGL(k_prior_fill, dim3((unsigned)((NN + 255) / 256)), dim3(256), 0, st, NN, fl,
+                           dense.as<float>());
+        PCHK(hipGetLastError());
+        hipLaunchKernelGGL(k_prior_scatter, dim3((unsigned)((En + 255) / 256)), dim3(256), 0, st, n, k, nn,
+                           nnw, lg.as<float>(), exponent, dense.as<float>());
+        PCHK(hipGetLastError());
+        hipLaunchKernelGGL(k_heur_mul, dim3((unsigned)((NN + 255) / 256)), dim3(256), 0, st, n, v.P,
+                           dense.as<float>(), v.heur);
+        PCHK(hipGetLastError());
+    }
+    PCHK(hipStreamSynchronize(st));
     return ACO_OK;
 }
 

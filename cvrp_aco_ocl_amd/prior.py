@@ -146,6 +146,17 @@ class PriorNet:
         return out
 
 
+    def apply(self, g, k=16, exponent=2.0, floor=0.05):
+        """Multiply eta of every instance of the GpuACO context ``g`` (before
+        init_aco) by the prior, on the device (aco_prior_apply)."""
+        xs = [_f32(it.x) for it in g.insts]
+        ys = [_f32(it.y) for it in g.insts]
+        xp = (C.c_void_p * len(xs))(*[a.ctypes.data for a in xs])
+        yp = (C.c_void_p * len(ys))(*[a.ctypes.data for a in ys])
+        check(lib.aco_prior_apply(g.h, self.h, C.cast(xp, C.c_void_p), C.cast(yp, C.c_void_p), k,
+                                  exponent, floor), "aco_prior_apply")
+
+
 def solve_plain(insts, iterations, n_ants=64, k=16, seed=1, ls_flags=7):
     """Plain colony (roulette over k candidates, Philox, local search) on a
     batch of instances in one context; returns the best tours."""
@@ -161,23 +172,24 @@ def solve_plain(insts, iterations, n_ants=64, k=16, seed=1, ls_flags=7):
 
 
 def run_colony(insts, iterations, net=None, n_ants=64, k=16, seed=1, ls_flags=7, beta=2.0,
-               floor=0.05, targets=None, chunk=25):
+               floor=0.05, targets=None, chunk=25, prior_exponent=None):
     """Batched colony over ``insts`` (the prior from ``net`` multiplies eta
     when given).  Stops early once every instance reached its target cost
-    (``targets``).  Returns dict with per-instance best-so-far records
-    [(iteration, length, seconds since aco_init)], the prior build time
-    (seconds, host wall clock) and the iterations run."""
+    (``targets``).  The prior is applied on the device (aco_prior_apply)
+    with exponent ``prior_exponent`` (default beta: the probability
+    multiplies eta).  Returns dict with per-instance best-so-far records
+    [(iteration, length, seconds since aco_init)], the prior's application
+    time (seconds, host wall clock) and the iterations run."""
     import time
     from .solver import GpuACO
-    t0 = time.perf_counter()
-    priors = None
-    if net is not None:
-        priors = [net.build(inst, inst.nn_lists(k), k, exponent=beta, floor=floor) for inst in insts]
-    t_build = time.perf_counter() - t0
     g = GpuACO(list(insts), n_ants=n_ants, k=k, beta=beta, selection=_lib.SEL_KNN,
-               rng=_lib.RNG_PHILOX, ls_flags=ls_flags, tau0_rule=_lib.TAU0_NN_TOUR, seed=seed,
-               prior=priors)
+               rng=_lib.RNG_PHILOX, ls_flags=ls_flags, tau0_rule=_lib.TAU0_NN_TOUR, seed=seed)
     try:
+        t0 = time.perf_counter()
+        if net is not None:
+            net.apply(g, k=k, exponent=prior_exponent if prior_exponent is not None else beta,
+                      floor=floor)
+        t_build = time.perf_counter() - t0
         g.init_aco()
         done = 0
         while done < iterations:

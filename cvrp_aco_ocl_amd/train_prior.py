@@ -35,6 +35,15 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def auc(score, label):
+    """P(score of a positive > score of a negative), ties counted half."""
+    neg = np.sort(score[label < 0.5])
+    pos = score[label > 0.5]
+    lo = np.searchsorted(neg, pos, side="left")
+    hi = np.searchsorted(neg, pos, side="right")
+    return float((lo.sum() + 0.5 * (hi - lo).sum()) / (len(pos) * len(neg)))
+
+
 def cmd_train(a):
     t0 = time.perf_counter()
     feats, labels, costs = [], [], []
@@ -59,21 +68,18 @@ def cmd_train(a):
                        pos_weight=a.pos_weight, shuffle_seed=a.seed + 11)
     t_train = time.perf_counter() - t1
     z = net.forward(f)
-    pos, neg = np.sort(z[y > 0.5]), np.sort(z[y < 0.5])
-    auc = float(np.searchsorted(neg, pos).sum() / (len(pos) * len(neg)))
-    # rank-only baseline: candidate rank as the score
-    rank = f[:, 1]
-    rank_auc = float(np.mean([np.mean(rank[y < 0.5] > v) + 0.5 * np.mean(rank[y < 0.5] == v)
-                              for v in rank[y > 0.5][:20000]]))
+    train_auc = auc(z, y)
+    # baseline: the candidate rank alone (lower rank scores higher)
+    rank_auc = auc(-f[:, 1], y)
     log(f"trained {a.epochs} epochs in {t_train:.2f}s: loss {losses[0]:.4f} -> {losses[-1]:.4f}, "
-        f"AUC {auc:.4f} (candidate rank alone {rank_auc:.4f})")
+        f"AUC {train_auc:.4f} (candidate rank alone {rank_auc:.4f})")
     os.makedirs(DATA, exist_ok=True)
     net.save(a.out)
     summary = {
         "instances": a.instances, "colony_iterations": a.iterations, "ants": a.ants, "k": a.k,
         "edges": int(len(f)), "positives": int(y.sum()), "epochs": a.epochs,
         "batch": a.train_batch, "lr": a.lr, "lr_final": a.lr_final, "pos_weight": a.pos_weight,
-        "epoch_loss": [round(float(v), 6) for v in losses], "train_auc": round(auc, 5),
+        "epoch_loss": [round(float(v), 6) for v in losses], "train_auc": round(train_auc, 5),
         "rank_only_auc": round(rank_auc, 5), "label_seconds": round(t_label, 2),
         "train_seconds": round(t_train, 3), "seed": a.seed,
         "mean_best_cost": float(np.mean(costs)),
@@ -84,14 +90,14 @@ def cmd_train(a):
 
 
 def compare(insts, iterations, net, ants=64, k=16, seed=1, floor=0.05,
-            budgets=(0.25, 0.5, 1.0, 2.0)):
+            budgets=(0.25, 0.5, 1.0, 2.0), exponent=None):
     """Plain colony for ``iterations`` (C* = its final best), then the prior
     colony until every instance is within 2% of C* or ``iterations``."""
     plain = PR.run_colony(insts, iterations, None, n_ants=ants, k=k, seed=seed)
     cstar = [r[-1][1] for r in plain["records"]]
     targets = [1.02 * c for c in cstar]
     pri = PR.run_colony(insts, iterations, net, n_ants=ants, k=k, seed=seed, floor=floor,
-                        targets=targets)
+                        targets=targets, prior_exponent=exponent)
     t_plain = [PR.time_to_target(r, t) for r, t in zip(plain["records"], targets)]
     t_prior = [PR.time_to_target(r, t, pri["build_s"]) for r, t in zip(pri["records"], targets)]
     gaps = {}

@@ -15,6 +15,8 @@
 
 #include <stdint.h>
 
+#include "cvrp_aco.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -81,6 +83,15 @@ int aco_prior_build(aco_prior *net, int32_t n, const float *x, const float *y,
                     const int32_t *demand, int32_t capacity, const float *dist,
                     const int32_t *nn, int32_t nn_width, int32_t k, float exponent, float floor,
                     float *prior);
+
+/* The same multiplier applied on the device to every instance of a context
+ * created without a prior (before aco_init): features from the context's
+ * own distances, demands and candidate lists (the construction list when
+ * k <= params.k, else the 2-opt neighbour list), the MLP, then
+ * heur[i][j] *= prior[i][j].  x[b], y[b] are instance b's coordinates.
+ * Equal to creating the context with aco_problem.prior = aco_prior_build(). */
+int aco_prior_apply(aco_ctx *ctx, aco_prior *net, const float *const *x, const float *const *y,
+                    int32_t k, float exponent, float floor);
 
 #ifdef __cplusplus
 }

@@ -149,3 +149,33 @@ def test_build_prior(ca):
     i = np.repeat(np.arange(inst.n), k)
     expect[i, nn[:, :k].ravel()] = sig ** 2
     np.testing.assert_allclose(dense, expect, rtol=2e-6, atol=0)
+
+
+def test_apply_equals_dense_prior(ca):
+    """aco_prior_apply on a context equals creating it with the dense prior
+    of aco_prior_build: same total_info after init, same colony afterwards."""
+    from cvrp_aco_ocl_amd import instances
+    insts = [instances.uniform_100(7), instances.clustered(8, customers=250)]
+    p = _net_params(10)
+    k = 16
+    with ca.prior.PriorNet(params=p) as net:
+        dense = [net.build(it, it.nn_lists(k), k, exponent=2.0, floor=0.05) for it in insts]
+        runs = []
+        for mode in ("dense", "apply"):
+            kw = dict(n_ants=16, k=20, seed=5, selection=ca.SEL_KNN, rng=ca.RNG_PHILOX, ls_flags=7,
+                      tau0_rule=ca.TAU0_NN_TOUR)
+            with ca.GpuACO(insts, prior=dense if mode == "dense" else None, **kw) as g:
+                if mode == "apply":
+                    net.apply(g, k=k, exponent=2.0, floor=0.05)
+                g.init_aco()
+                ti0 = [g.pheromone(b)[1] for b in range(len(insts))]
+                g.run(3)
+                runs.append((ti0, [g.ants(b) for b in range(len(insts))]))
+    for b in range(len(insts)):
+        np.testing.assert_array_equal(runs[0][0][b], runs[1][0][b])
+        for x, y in zip(runs[0][1][b], runs[1][1][b]):
+            np.testing.assert_array_equal(x, y)
+    # the prior changed the heuristic
+    with ca.GpuACO(insts, n_ants=16, k=20, seed=5, tau0_rule=ca.TAU0_NN_TOUR) as g:
+        g.init_aco()
+        assert not np.array_equal(g.pheromone(0)[1], runs[0][0][0])
