@@ -45,6 +45,12 @@ def parse():
                     help="CPU baseline sample budget (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cores)")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--prior-instances", type=int, default=64,
+                    help="learned-prior vs plain leg: evaluation instances (0 = skip)")
+    ap.add_argument("--prior-iterations", type=int, default=2000,
+                    help="plain iterations that fix the reference cost C*")
+    ap.add_argument("--n1000-seconds", type=float, default=10.0,
+                    help="GPU vs CPU oracle on the 1001-node instance (0 = skip)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 10 device-to-device copies of MIB MiB (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes) and exit")
@@ -122,6 +128,9 @@ def main():
     # ---- timed region: K iterations, graph replay ----
     g = make_solver(ca, inst, args.seed)
     g.init_aco()
+    tune = g.autotune(reps=3)   # local-search launch shape; results are the same for every shape
+    shape = g.launch_shape()
+    print(f"# autotune: {tune} -> {shape}", file=sys.stderr, flush=True)
     g.run(args.warmup)
     g.synchronize()
     t0 = time.perf_counter()
@@ -194,6 +203,7 @@ def main():
     quality = None
     if args.full_iterations > 0:
         g = make_solver(ca, inst, args.seed)
+        g.set_launch_shape(*shape)
         t0 = time.perf_counter()
         g.init_aco()
         g.run(args.full_iterations)
@@ -235,6 +245,67 @@ def main():
         print(f"# cpu baseline: {cpu['value']} ant-sol/s on {threads} threads", file=sys.stderr,
               flush=True)
 
+    # ---- learned prior vs plain: 64 clustered n=501 instances, batched ----
+    prior_leg = None
+    if args.prior_instances > 0:
+        from cvrp_aco_ocl_amd import prior as PR, train_prior as TP
+        net = PR.PriorNet.load()
+        ev = instances.evaluation(args.prior_instances)
+        print(f"# prior leg: {len(ev)} instances, plain {args.prior_iterations} iterations",
+              file=sys.stderr, flush=True)
+        prior_leg = TP.compare(ev, args.prior_iterations, net, ants=64, k=16,
+                               exponent=TP.PRIOR_EXPONENT, budgets=(0.5, 1.0, 2.0, 5.0, 10.0))
+        prior_leg["workload"] = ("64 clustered instances (500 customers, 8 clusters, sigma 6, "
+                                 "30 routes; seeds 100000+), 64 ants each, k=16 Philox roulette, "
+                                 "2-opt+swap+relocate, one batched context; C* = plain best after "
+                                 f"{args.prior_iterations} iterations; prior time includes "
+                                 "aco_prior_apply")
+        net.close()
+        print(f"# prior leg: plain median {prior_leg['plain_time_to_2pct_s']['median']:.3f}s, "
+              f"prior median {prior_leg['prior_time_to_2pct_s']['median']:.3f}s",
+              file=sys.stderr, flush=True)
+
+    # ---- 1001-node instance: GPU vs the CPU oracle on all host cores ----
+    n1000 = None
+    if args.n1000_seconds > 0:
+        big = instances.mixed_1000(0)
+        gb = ca.GpuACO(big, n_ants=256, k=32, seed=1, selection=ca.SEL_KNN, rng=ca.RNG_PHILOX,
+                       ls_flags=7)
+        gb.init_aco()
+        tuned = gb.autotune(reps=2)
+        gb.run(3)
+        gb.synchronize()
+        t0 = time.perf_counter()
+        gb.run(20)
+        gb.synchronize()
+        g_rate = 256 * 20 / (time.perf_counter() - t0)
+        gb_shape = gb.launch_shape()
+        gb.close()
+        import oracle
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        Pb = oracle.Problem(big.dist, big.demand, big.capacity, big.max_dist, big.service_time,
+                            k=32, tie=0)
+        colb = oracle.Colony(Pb, 256, selection=oracle.SEL_KNN, rng=oracle.RNG_PHILOX, ls_flags=7,
+                             seed=1, threads=threads)
+        colb.init()
+        its, t0 = 0, time.perf_counter()
+        while True:
+            colb.iteration()
+            its += 1
+            el = time.perf_counter() - t0
+            if el >= args.n1000_seconds or its >= 50:
+                break
+        c_rate = 256 * its / el
+        n1000 = {"workload": "1001 nodes (500 uniform + 500 clustered, 5% heavy demands, ~12 "
+                             "customers/route), 256 ants, k=32, full local search",
+                 "gpu_ant_solutions_per_s": round(g_rate, 1),
+                 "cpu_ant_solutions_per_s": round(c_rate, 2), "cpu_threads": threads,
+                 "cpu_sample": f"{its} iterations in {el:.2f}s",
+                 "ratio": round(g_rate / c_rate, 2), "launch_shape": gb_shape,
+                 "autotune_ms": {str(sh): round(ms, 3) for sh, ms in tuned}}
+        print(f"# n=1001: GPU {g_rate:.0f} vs CPU {c_rate:.1f} ant-sol/s ({g_rate / c_rate:.1f}x)",
+              file=sys.stderr, flush=True)
+
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -254,11 +325,14 @@ def main():
                         f"(Philox), 2-opt+swap+relocate, alpha 1 beta 2 rho 0.1, 6 ranks",
             "n": inst.n, "ants": m, "k": k, "capacity": inst.capacity, "seed": args.seed,
             "step": "one ACO iteration (hipGraph replay)",
+            "ls_launch_shape": {"waves": shape[0], "ring": shape[1], "prefetch": shape[2]},
         },
         "roofline": roofline,
         "kernels": per_kernel,
         "ls_move_evals_per_iter": int(ls_evals / max(args.profile_steps, 1)),
         "quality": quality,
+        "learned_prior": prior_leg,
+        "n1001": n1000,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

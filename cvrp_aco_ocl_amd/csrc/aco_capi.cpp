@@ -949,6 +949,8 @@ int aco_autotune(aco_ctx *c, int32_t reps, aco_tune_result *results, int32_t cap
         }
     }
     if (err == hipSuccess) err = copy_all(orig, false);
+    // the search clock (best-so-far record times) restarts after tuning
+    if (err == hipSuccess) err = launch_stamp_clock(c->bt, c->stream);
     if (err == hipSuccess) err = hipStreamSynchronize(c->stream);
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);

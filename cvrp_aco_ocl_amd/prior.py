@@ -172,14 +172,15 @@ def solve_plain(insts, iterations, n_ants=64, k=16, seed=1, ls_flags=7):
 
 
 def run_colony(insts, iterations, net=None, n_ants=64, k=16, seed=1, ls_flags=7, beta=2.0,
-               floor=0.05, targets=None, chunk=25, prior_exponent=None):
+               floor=0.05, targets=None, chunk=25, prior_exponent=None, autotune=True):
     """Batched colony over ``insts`` (the prior from ``net`` multiplies eta
     when given).  Stops early once every instance reached its target cost
     (``targets``).  The prior is applied on the device (aco_prior_apply)
     with exponent ``prior_exponent`` (default beta: the probability
     multiplies eta).  Returns dict with per-instance best-so-far records
-    [(iteration, length, seconds since aco_init)], the prior's application
-    time (seconds, host wall clock) and the iterations run."""
+    [(iteration, length, seconds of search)], the prior's application time
+    (seconds, host wall clock), the iterations run and the search's wall
+    time."""
     import time
     from .solver import GpuACO
     g = GpuACO(list(insts), n_ants=n_ants, k=k, beta=beta, selection=_lib.SEL_KNN,
@@ -191,6 +192,10 @@ def run_colony(insts, iterations, net=None, n_ants=64, k=16, seed=1, ls_flags=7,
                       floor=floor)
         t_build = time.perf_counter() - t0
         g.init_aco()
+        if autotune:
+            g.autotune(reps=2)   # launch shape only; results and record clock unaffected
+        g.synchronize()
+        t0 = time.perf_counter()
         done = 0
         while done < iterations:
             step = min(chunk, iterations - done)
@@ -200,10 +205,12 @@ def run_colony(insts, iterations, net=None, n_ants=64, k=16, seed=1, ls_flags=7,
                 if all(g.best(b)[1] <= targets[b] for b in range(len(insts))):
                     break
         g.synchronize()
+        wall = time.perf_counter() - t0
         recs = [g.bsf_records(b) for b in range(len(insts))]
     finally:
         g.close()
-    return {"records": recs, "build_s": t_build, "iterations": done}
+    return {"records": recs, "build_s": t_build, "iterations": done, "wall_s": wall,
+            "ant_solutions": len(insts) * n_ants * done}
 
 
 def time_to_target(records, target, offset=0.0):

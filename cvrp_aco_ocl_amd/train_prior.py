@@ -119,13 +119,19 @@ def compare(insts, iterations, net, ants=64, k=16, seed=1, floor=0.05,
         gq = [PR.best_at(r, bud, pri["build_s"]) / c - 1 for r, c in zip(pri["records"], cstar)]
         gaps[str(bud)] = {"plain": float(np.mean(gp)), "prior": float(np.mean(gq))}
     fin = lambda v: [x for x in v if np.isfinite(x)]
+    reached = fin(t_prior)
     out = {
         "instances": len(insts), "iterations": iterations, "ants": ants, "k": k,
         "prior_build_s": round(pri["build_s"], 4), "prior_iterations": pri["iterations"],
         "plain_time_to_2pct_s": {"mean": float(np.mean(t_plain)), "median": float(np.median(t_plain))},
-        "prior_time_to_2pct_s": {"mean": float(np.mean(t_prior)) if len(fin(t_prior)) == len(t_prior) else None,
+        "plain_all_reached_s": float(np.max(t_plain)),
+        "prior_time_to_2pct_s": {"mean": float(np.mean(t_prior)) if len(reached) == len(t_prior) else None,
                                  "median": float(np.median(t_prior)),
-                                 "reached": len(fin(t_prior))},
+                                 "reached": len(reached)},
+        "prior_all_reached_s": float(np.max(t_prior)) if len(reached) == len(t_prior) else None,
+        "plain_wall_s": round(plain["wall_s"], 3),
+        "plain_ant_solutions_per_s": round(plain["ant_solutions"] / plain["wall_s"], 1),
+        "prior_exponent": exponent,
         "gap_at_budget": gaps,
         "mean_cstar": float(np.mean(cstar)),
     }

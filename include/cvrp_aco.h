@@ -189,8 +189,9 @@ int aco_get_launch_shape(aco_ctx *ctx, aco_launch_shape *shape);
 int aco_set_launch_shape(aco_ctx *ctx, const aco_launch_shape *shape);
 /* After aco_init: constructs once from the current state, times every
  * candidate shape `reps` times on those same tours, keeps the fastest and
- * restores the state, so the colony's results do not change.  results
- * (optional, capacity entries) receives every candidate's time. */
+ * restores the state, so the colony's results do not change; the clock of
+ * the best-so-far records restarts afterwards.  results (optional,
+ * capacity entries) receives every candidate's time. */
 int aco_autotune(aco_ctx *ctx, int32_t reps, aco_tune_result *results, int32_t capacity,
                  int32_t *count);
 
