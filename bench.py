@@ -302,7 +302,8 @@ def main():
                  "cpu_ant_solutions_per_s": round(c_rate, 2), "cpu_threads": threads,
                  "cpu_sample": f"{its} iterations in {el:.2f}s",
                  "ratio": round(g_rate / c_rate, 2), "launch_shape": gb_shape,
-                 "autotune_ms": {str(sh): round(ms, 3) for sh, ms in tuned}}
+                 "autotune_ms": {f"{'construct' if st == 0 else 'local_search'}{sh}": round(ms, 3)
+                                 for sh, st, ms in tuned}}
         print(f"# n=1001: GPU {g_rate:.0f} vs CPU {c_rate:.1f} ant-sol/s ({g_rate / c_rate:.1f}x)",
               file=sys.stderr, flush=True)
 
@@ -325,7 +326,8 @@ def main():
                         f"(Philox), 2-opt+swap+relocate, alpha 1 beta 2 rho 0.1, 6 ranks",
             "n": inst.n, "ants": m, "k": k, "capacity": inst.capacity, "seed": args.seed,
             "step": "one ACO iteration (hipGraph replay)",
-            "ls_launch_shape": {"waves": shape[0], "ring": shape[1], "prefetch": shape[2]},
+            "launch_shape": {"ls_waves": shape[0], "ls_ring": shape[1], "ls_prefetch": shape[2],
+                             "construct_ants_per_block": shape[3]},
         },
         "roofline": roofline,
         "kernels": per_kernel,

@@ -63,11 +63,12 @@ PRIOR_PARAMS = 64 * 12 + 64 + 64 * 64 + 64 + 64 + 1
 
 
 class AcoLaunchShape(C.Structure):
-    _fields_ = [("ls_waves", C.c_int32), ("ls_ring", C.c_int32), ("ls_prefetch", C.c_int32)]
+    _fields_ = [("ls_waves", C.c_int32), ("ls_ring", C.c_int32), ("ls_prefetch", C.c_int32),
+                ("construct_apb", C.c_int32)]
 
 
 class AcoTuneResult(C.Structure):
-    _fields_ = [("shape", AcoLaunchShape), ("ms", C.c_float)]
+    _fields_ = [("shape", AcoLaunchShape), ("stage", C.c_int32), ("ms", C.c_float)]
 
 
 class AcoBsfRecord(C.Structure):

@@ -169,7 +169,10 @@ static hipError_t timed(aco_ctx *c, const char *name, F &&fn)
 static int run_construct(aco_ctx *c)
 {
     HIP_TRY(timed(c, "construct",
-                  [&] { return launch_construct(c->bt, c->prm.selection, c->prm.rng, c->stream); }));
+                  [&] {
+                      return launch_construct(c->bt, c->prm.selection, c->prm.rng, c->ls.construct_apb,
+                                              c->prm.k, c->stream);
+                  }));
     return ACO_OK;
 }
 
@@ -369,8 +372,8 @@ int aco_create_batch(const aco_problem *pbs, int32_t count, const aco_params *pr
     // fits, else an 8-row ring
     c->lds_limit = prop.sharedMemPerBlock;
     c->max_n = max_n;
-    c->ls = LsShape{4, 16, 1};
-    if (local_search_lds_bytes(max_n, 16) > prop.sharedMemPerBlock) c->ls = LsShape{4, 8, 0};
+    c->ls = LsShape{4, 16, 1, 0};
+    if (local_search_lds_bytes(max_n, 16) > prop.sharedMemPerBlock) c->ls = LsShape{4, 8, 0, 0};
     const size_t lds_need = std::max(construct_lds_bytes(max_n, prm.selection),
                                      prm.ls_flags ? local_search_lds_bytes(max_n, c->ls.ring) : 0);
     if (lds_need > prop.sharedMemPerBlock)
@@ -845,7 +848,15 @@ static void drop_graph(aco_ctx *c)
 
 static bool shape_fits(const aco_ctx *c, const LsShape &sh)
 {
-    return ls_shape_valid(sh) && local_search_lds_bytes(c->max_n, sh.ring) <= c->lds_limit;
+    if (!ls_shape_valid(sh) || local_search_lds_bytes(c->max_n, sh.ring) > c->lds_limit) return false;
+    if (!construct_staged_ok(c->bt, c->prm.selection, c->prm.k, c->m, sh.construct_apb)) return false;
+    return sh.construct_apb == 0 ||
+           construct_lds_staged_bytes(c->max_n, c->prm.k, sh.construct_apb) <= c->lds_limit;
+}
+
+static aco_launch_shape to_public(const LsShape &sh)
+{
+    return aco_launch_shape{sh.waves, sh.ring, sh.prefetch, sh.construct_apb};
 }
 
 int aco_get_launch_shape(aco_ctx *c, aco_launch_shape *out)
@@ -853,9 +864,7 @@ int aco_get_launch_shape(aco_ctx *c, aco_launch_shape *out)
     int rc = need(c, false);
     if (rc) return rc;
     if (!out) return set_error(ACO_ERR_ARG, "null shape");
-    out->ls_waves = c->ls.waves;
-    out->ls_ring = c->ls.ring;
-    out->ls_prefetch = c->ls.prefetch;
+    *out = to_public(c->ls);
     return ACO_OK;
 }
 
@@ -864,13 +873,16 @@ int aco_set_launch_shape(aco_ctx *c, const aco_launch_shape *in)
     int rc = need(c, false);
     if (rc) return rc;
     if (!in) return set_error(ACO_ERR_ARG, "null shape");
-    const LsShape sh{in->ls_waves, in->ls_ring, in->ls_prefetch ? 1 : 0};
+    const LsShape sh{in->ls_waves, in->ls_ring, in->ls_prefetch ? 1 : 0, in->construct_apb};
     if (!ls_shape_valid(sh))
         return set_error(ACO_ERR_ARG, "invalid shape: waves 4|8, ring 8|16|32 >= waves+2, "
                                       "prefetch needs ring >= 2*waves+2");
+    if (!construct_staged_ok(c->bt, c->prm.selection, c->prm.k, c->m, sh.construct_apb))
+        return set_error(ACO_ERR_ARG, "construct_apb must be 0, or 1|2|4 dividing the ants per instance "
+                                      "for candidate roulette without a route-length limit");
     if (!shape_fits(c, sh))
-        return set_error(ACO_ERR_ARG, "shape needs " + std::to_string(local_search_lds_bytes(c->max_n, sh.ring)) +
-                                          " bytes of LDS, device has " + std::to_string(c->lds_limit));
+        return set_error(ACO_ERR_ARG, "shape needs more LDS than the device's " + std::to_string(c->lds_limit) +
+                                          " bytes per block");
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->ls = sh;
     drop_graph(c);
@@ -882,15 +894,12 @@ int aco_autotune(aco_ctx *c, int32_t reps, aco_tune_result *results, int32_t cap
     int rc = need(c, true);
     if (rc) return rc;
     if (reps < 1 || reps > 1000) return set_error(ACO_ERR_ARG, "reps must be in [1, 1000]");
-    static const LsShape kCandidates[] = {{4, 16, 1}, {4, 16, 0}, {4, 8, 0}, {8, 32, 1}, {8, 16, 0}};
-    constexpr int kCount = (int)(sizeof(kCandidates) / sizeof(kCandidates[0]));
-    if (count) *count = kCount;
+    static const int kApb[] = {0, 1, 2, 4};
+    static const LsShape kLs[] = {{4, 16, 1, 0}, {4, 16, 0, 0}, {4, 8, 0, 0}, {8, 32, 1, 0}, {8, 16, 0, 0}};
+    constexpr int kNc = (int)(sizeof(kApb) / sizeof(kApb[0]));
+    constexpr int kNl = (int)(sizeof(kLs) / sizeof(kLs[0]));
+    if (count) *count = kNc + kNl;
     HIP_TRY(hipStreamSynchronize(c->stream));
-    if (!c->prm.ls_flags) {
-        for (int q = 0; q < kCount && results && q < capacity; ++q)
-            results[q] = aco_tune_result{{kCandidates[q].waves, kCandidates[q].ring, kCandidates[q].prefetch}, -1.0f};
-        return ACO_OK;
-    }
     // the mutable per-instance state lies between ds.tours and the end of
     // ds.totals in the instance arena
     struct Span { unsigned char *lo; size_t bytes; };
@@ -920,33 +929,45 @@ int aco_autotune(aco_ctx *c, int32_t reps, aco_tune_result *results, int32_t cap
         return hipSuccess;
     };
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    int best = -1;
-    float best_ms = 0.0f;
-    std::vector<float> times(kCount, -1.0f);
+    std::vector<float> tc(kNc, -1.0f), tl(kNl, -1.0f);
+    int best_c = -1, best_l = -1;
     hipError_t err = hipEventCreate(&e0);
     if (err == hipSuccess) err = hipEventCreate(&e1);
     if (err == hipSuccess) err = copy_all(orig, true);
-    if (err == hipSuccess) err = launch_construct(c->bt, c->prm.selection, c->prm.rng, c->stream);
-    if (err == hipSuccess) err = copy_all(built, true);
-    for (int q = 0; q < kCount && err == hipSuccess; ++q) {
-        if (!shape_fits(c, kCandidates[q])) continue;
+    // mean time of `reps` launches, each from the state saved in `from`
+    auto time_it = [&](void *from, auto &&launch, float &out) {
         float sum = 0.0f;
         for (int r = 0; r < reps && err == hipSuccess; ++r) {
-            err = copy_all(built, false);
+            err = copy_all(from, false);
             if (err == hipSuccess) err = hipEventRecord(e0, c->stream);
-            if (err == hipSuccess) err = launch_local_search(c->bt, c->prm.ls_flags, kCandidates[q], c->stream);
+            if (err == hipSuccess) err = launch();
             if (err == hipSuccess) err = hipEventRecord(e1, c->stream);
             if (err == hipSuccess) err = hipEventSynchronize(e1);
             float ms = 0.0f;
             if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
             sum += ms;
         }
-        if (err != hipSuccess) break;
-        times[q] = sum / (float)reps;
-        if (best < 0 || times[q] < best_ms) {
-            best = q;
-            best_ms = times[q];
-        }
+        out = sum / (float)reps;
+    };
+    // stage 0: construction
+    for (int q = 0; q < kNc && err == hipSuccess; ++q) {
+        LsShape sh = c->ls;
+        sh.construct_apb = kApb[q];
+        if (!shape_fits(c, sh)) continue;
+        time_it(orig, [&] { return launch_construct(c->bt, c->prm.selection, c->prm.rng, kApb[q], c->prm.k, c->stream); },
+                tc[q]);
+        if (err == hipSuccess && (best_c < 0 || tc[q] < tc[best_c])) best_c = q;
+    }
+    // stage 1: local search on the tours of one construction
+    if (err == hipSuccess) err = copy_all(orig, false);
+    if (err == hipSuccess) err = launch_construct(c->bt, c->prm.selection, c->prm.rng, 0, c->prm.k, c->stream);
+    if (err == hipSuccess) err = copy_all(built, true);
+    for (int q = 0; q < kNl && err == hipSuccess && c->prm.ls_flags; ++q) {
+        LsShape sh = kLs[q];
+        sh.construct_apb = 0;
+        if (!shape_fits(c, sh)) continue;
+        time_it(built, [&] { return launch_local_search(c->bt, c->prm.ls_flags, kLs[q], c->stream); }, tl[q]);
+        if (err == hipSuccess && (best_l < 0 || tl[q] < tl[best_l])) best_l = q;
     }
     if (err == hipSuccess) err = copy_all(orig, false);
     // the search clock (best-so-far record times) restarts after tuning
@@ -957,12 +978,26 @@ int aco_autotune(aco_ctx *c, int32_t reps, aco_tune_result *results, int32_t cap
     (void)hipFree(orig);
     (void)hipFree(built);
     if (err != hipSuccess) return set_error(ACO_ERR_HIP, std::string("aco_autotune: ") + hipGetErrorString(err));
-    if (best >= 0) {
-        c->ls = kCandidates[best];
-        drop_graph(c);
+    LsShape chosen = c->ls;
+    if (best_l >= 0) {
+        chosen.waves = kLs[best_l].waves;
+        chosen.ring = kLs[best_l].ring;
+        chosen.prefetch = kLs[best_l].prefetch;
     }
-    for (int q = 0; q < kCount && results && q < capacity; ++q)
-        results[q] = aco_tune_result{{kCandidates[q].waves, kCandidates[q].ring, kCandidates[q].prefetch}, times[q]};
+    if (best_c >= 0) chosen.construct_apb = kApb[best_c];
+    c->ls = chosen;
+    drop_graph(c);
+    int w = 0;
+    for (int q = 0; q < kNc && results && w < capacity; ++q, ++w) {
+        LsShape sh = chosen;
+        sh.construct_apb = kApb[q];
+        results[w] = aco_tune_result{to_public(sh), 0, tc[q]};
+    }
+    for (int q = 0; q < kNl && results && w < capacity; ++q, ++w) {
+        LsShape sh = kLs[q];
+        sh.construct_apb = chosen.construct_apb;
+        results[w] = aco_tune_result{to_public(sh), 1, tl[q]};
+    }
     return ACO_OK;
 }
 

@@ -16,6 +16,56 @@
 namespace cvrp {
 
 constexpr int kLanes = 64;             // wavefront = one ant
+
+// ---- lane exchange on the DPP network (no LDS traffic) ----
+// DPP controls: quad_perm [1,0,3,2] (lane ^ 1), quad_perm [2,3,0,1]
+// (lane ^ 2), row_shl / row_shr by 4 and 8 (lane +- 4, +- 8 within a row of
+// 16 lanes).  Out-of-row reads give 0.
+enum { kDppXor1 = 0xB1, kDppXor2 = 0x4E, kDppShl4 = 0x104, kDppShr4 = 0x114, kDppShl8 = 0x108,
+       kDppShr8 = 0x118 };
+
+template <int CTRL>
+__device__ __forceinline__ unsigned long long dpp_u64(unsigned long long v)
+{
+    const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)v, CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(v >> 32), CTRL, 0xF, 0xF, true);
+    return ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
+}
+
+// value of lane (lane ^ 4) / (lane ^ 8), within the row
+__device__ __forceinline__ unsigned long long dpp_xor4_u64(unsigned long long v, int lane)
+{
+    const unsigned long long a = dpp_u64<kDppShl4>(v), b = dpp_u64<kDppShr4>(v);
+    return (lane & 4) ? b : a;
+}
+
+__device__ __forceinline__ unsigned long long dpp_xor8_u64(unsigned long long v, int lane)
+{
+    const unsigned long long a = dpp_u64<kDppShl8>(v), b = dpp_u64<kDppShr8>(v);
+    return (lane & 8) ? b : a;
+}
+
+__device__ __forceinline__ unsigned long long readlane_u64(unsigned long long v, int l)
+{
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)v, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), l);
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+// Max / min over all 64 lanes (all lanes active), uniform result.
+template <bool MAX>
+__device__ __forceinline__ unsigned long long wave_reduce_u64(unsigned long long v)
+{
+    const int lane = (int)(threadIdx.x & 63);
+    auto pick = [](unsigned long long a, unsigned long long b) { return MAX ? (a > b ? a : b) : (a < b ? a : b); };
+    v = pick(v, dpp_u64<kDppXor1>(v));
+    v = pick(v, dpp_u64<kDppXor2>(v));
+    v = pick(v, dpp_xor4_u64(v, lane));
+    v = pick(v, dpp_xor8_u64(v, lane));
+    const unsigned long long r0 = readlane_u64(v, 0), r1 = readlane_u64(v, 16);
+    const unsigned long long r2 = readlane_u64(v, 32), r3 = readlane_u64(v, 48);
+    return pick(pick(r0, r1), pick(r2, r3));
+}
 constexpr float kEps = 0.001f;         // utilities.h:19
 constexpr int kMaxNodes = 4096;        // 12-bit node ids in reduction keys
 constexpr int kMaxRas = 16;
@@ -97,10 +147,15 @@ struct LsShape {
     int waves;
     int ring;
     int prefetch;
+    int construct_apb;   // construction: ants per block with the candidate table in LDS, 0 = off
 };
 
 // ---- host launchers (aco_kernels.hip, aco_local_search.hip) ----
-hipError_t launch_construct(const DevBatch &b, int selection, int rng, hipStream_t st);
+// apb = ants per block of the LDS-staged candidate roulette (0: rows from
+// L2/HBM, one ant per block); k = candidate list width
+hipError_t launch_construct(const DevBatch &b, int selection, int rng, int apb, int k, hipStream_t st);
+size_t construct_lds_staged_bytes(int n, int k, int apb);
+bool construct_staged_ok(const DevBatch &b, int selection, int k, int ants_per_instance, int apb);
 hipError_t launch_local_search(const DevBatch &b, int flags, const LsShape &sh, hipStream_t st);
 hipError_t launch_statistics(const DevBatch &b, float tick_to_s, hipStream_t st);
 hipError_t launch_pheromone_update(const DevBatch &b, hipStream_t st);

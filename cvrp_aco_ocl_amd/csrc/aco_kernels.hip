@@ -44,37 +44,37 @@ __device__ __forceinline__ float philox_u01(u64 seed, u32 iter, u32 ant, u32 ste
 
 // ------------------------------------------------------- wave helpers ----
 
-__device__ __forceinline__ u64 wave_max_u64(u64 v)
+__device__ __forceinline__ u64 wave_max_u64(u64 v) { return wave_reduce_u64<true>(v); }
+
+__device__ __forceinline__ u64 wave_min_u64(u64 v) { return wave_reduce_u64<false>(v); }
+
+// Value of the lane selected by a DPP control (0 where the source lane is
+// outside the row or the row is masked off).
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_src(float x)
 {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        const u64 w = __shfl_xor(v, o);
-        v = w > v ? w : v;
-    }
-    return v;
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, ROW_MASK, 0xF, false));
 }
 
-__device__ __forceinline__ u64 wave_min_u64(u64 v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        const u64 w = __shfl_xor(v, o);
-        v = w < v ? w : v;
-    }
-    return v;
-}
-
-// Inclusive Hillis-Steele scan, offsets 1..32 (the association the CPU
-// restatement emulates exactly).
+// Inclusive wave scan on the DPP network (no LDS traffic): Hillis-Steele
+// inside each row of 16 lanes (row_shr 1, 2, 4, 8), then row 0's total into
+// row 1 and row 2's into row 3 (row_bcast:15), then lane 31's prefix into
+// rows 2 and 3 (row_bcast:31).  The CPU restatement adds in the same order.
+// Adding 0 to a non-negative value leaves it unchanged.
 __device__ __forceinline__ float wave_scan_f32(float x, int lane)
 {
-#pragma unroll
-    for (int d = 1; d < kLanes; d <<= 1) {
-        const float y = __shfl_up(x, d);
-        if (lane >= d) x = x + y;
-    }
+    (void)lane;
+    x = x + dpp_src<0x111, 0xF>(x);
+    x = x + dpp_src<0x112, 0xF>(x);
+    x = x + dpp_src<0x114, 0xF>(x);
+    x = x + dpp_src<0x118, 0xF>(x);
+    x = x + dpp_src<0x142, 0xA>(x);
+    x = x + dpp_src<0x143, 0xC>(x);
     return x;
 }
+
+__device__ __forceinline__ float lane_f32(float x, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l)); }
+__device__ __forceinline__ int lane_i32(int x, int l) { return __builtin_amdgcn_readlane(x, l); }
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
@@ -273,7 +273,7 @@ __global__ __launch_bounds__(64) void k_construct_knn(DevBatch bt)
             pj = feasible<HAS_D>(p, L, j, dcj, load, rdist) ? tj : 0.0f;
         }
         const float x = wave_scan_f32(pj, lane);
-        const float total = __shfl(x, k - 1);
+        const float total = lane_f32(x, k - 1);
         int w;
         float dcw;
         if (total > 0.0f) {
@@ -288,8 +288,8 @@ __global__ __launch_bounds__(64) void k_construct_knn(DevBatch bt)
                 const u64 m2 = __ballot(pj > 0.0f);
                 wc = 63 - __clzll((long long)m2);
             }
-            w = __shfl(j, wc);
-            dcw = __shfl(dcj, wc);
+            w = lane_i32(j, wc);
+            dcw = lane_f32(dcj, wc);
         } else {
             const float *drow = p.dist + (size_t)cur * p.P;
             w = greedy_pick<HAS_D>(p, L, s.tinfo + (size_t)cur * p.P, drow, load, rdist, lane);
@@ -324,8 +324,134 @@ __global__ __launch_bounds__(64) void k_construct_knn(DevBatch bt)
     }
 }
 
-hipError_t launch_construct(const DevBatch &b, int selection, int rng, hipStream_t st)
+
+// The same roulette with the instance's candidate table staged in LDS, for
+// instances without a route-length limit: A ants per block (one wave each,
+// no block-wide barrier after staging) share the n*k candidate ids (16-bit)
+// and total_info values, so a step's dependent read is an LDS read.  The
+// chosen edge's length is fetched after the pick and added one step later
+// (the same additions in the same order).
+template <int RNG, int A>
+__global__ __launch_bounds__(64 * A) void k_construct_knn_lds(DevBatch bt)
 {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int inst = bt.ant_inst[blockIdx.x * A];       // all A ants in one instance
+    const DevProblem p = bt.probs[inst];
+    const DevState s = bt.states[inst];
+    const int a = blockIdx.x * A + wv - bt.ant_base[inst];
+    const int n = p.n, k = p.k, nk = n * k;
+    float *tik = reinterpret_cast<float *>(smem);
+    int *dem = reinterpret_cast<int *>(tik + nk);
+    float *d0 = reinterpret_cast<float *>(dem + n);
+    unsigned short *nid = reinterpret_cast<unsigned short *>(d0 + n);
+    unsigned char *vis = reinterpret_cast<unsigned char *>(nid + ((nk + 7) & ~7)) + wv * ((n + 15) & ~15);
+    for (int e = threadIdx.x; e < nk; e += 64 * A) {
+        tik[e] = s.tiknn[e];
+        nid[e] = (unsigned short)p.nnk[e];
+    }
+    for (int i = threadIdx.x; i < n; i += 64 * A) {
+        dem[i] = p.dem[i];
+        d0[i] = p.d0[i];
+    }
+    for (int i = lane; i < n; i += kLanes) vis[i] = i == 0;
+    __syncthreads();
+    const AntLds L{dem, d0, nullptr, vis};
+    const u64 seed = p.seed;
+    const u32 iter = (u32)s.ctrl[kCtrlIter];
+    int z = RNG == kRngParkMiller ? s.pm[a * kLanes] : 0;
+    int *tour = s.tours + (size_t)a * p.T;
+    int cur = 0, load = 0, vcnt = 1, step = 0, fb = 0, tbuf = 0;
+    float tdist = 0.0f, pend = 0.0f;
+    while (vcnt < n) {
+        step++;
+        const float uph = RNG == kRngPhilox ? philox_u01(seed, iter, (u32)a, (u32)step) : 0.0f;
+        int j = 0;
+        float pj = 0.0f;
+        if (lane < k) {
+            j = nid[cur * k + lane];
+            const float tj = tik[cur * k + lane];
+            pj = feasible<false>(p, L, j, 0.0f, load, 0.0f) ? tj : 0.0f;
+        }
+        tdist = tdist + pend;   // previous step's edge
+        const float x = wave_scan_f32(pj, lane);
+        const float total = lane_f32(x, k - 1);
+        int w;
+        float dcw;
+        if (total > 0.0f) {
+            const float u = RNG == kRngPhilox ? uph : pm_ran01(z);
+            const float r = u * total;
+            const u64 m1 = __ballot(lane < k && x > r);
+            int wc;
+            if (m1) {
+                wc = __ffsll((unsigned long long)m1) - 1;
+            } else {
+                const u64 m2 = __ballot(pj > 0.0f);
+                wc = 63 - __clzll((long long)m2);
+            }
+            w = lane_i32(j, wc);
+            dcw = p.nnkd[cur * k + uniform(wc)];
+        } else {
+            const float *drow = p.dist + (size_t)cur * p.P;
+            w = greedy_pick<false>(p, L, s.tinfo + (size_t)cur * p.P, drow, load, 0.0f, lane);
+            fb++;
+            dcw = drow[uniform(w)];
+        }
+        w = uniform(w);
+        if (w == 0) {
+            load = 0;
+        } else {
+            load += dem[w];
+            vcnt++;
+        }
+        if (lane == 0) vis[w] = 1;
+        pend = dcw;
+        tour_put(tour, tbuf, step, w, lane);
+        cur = w;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    }
+    tdist = tdist + pend;
+    tdist = tdist + d0[cur];
+    step++;
+    tour_close(tour, tbuf, step, lane);
+    if (lane == 0) {
+        s.sizes[a] = step + 1;
+        s.lens[a] = tdist;
+        s.ant_fb[a] = fb;
+        s.ant_evals[a] = 0;
+        s.ant_moves[a] = 0;
+        if (RNG == kRngParkMiller) s.pm[a * kLanes] = z;
+    }
+}
+
+size_t construct_lds_staged_bytes(int n, int k, int apb)
+{
+    const size_t nk = (size_t)n * k;
+    return nk * 4 + (size_t)n * 8 + ((nk + 7) & ~(size_t)7) * 2 + (size_t)apb * ((n + 15) & ~15) + 16;
+}
+
+bool construct_staged_ok(const DevBatch &b, int selection, int k, int ants_per_instance, int apb)
+{
+    if (apb == 0) return true;
+    if (apb != 1 && apb != 2 && apb != 4) return false;
+    return selection == kSelKnn && !b.has_d && k > 0 && ants_per_instance % apb == 0;
+}
+
+hipError_t launch_construct(const DevBatch &b, int selection, int rng, int apb, int k, hipStream_t st)
+{
+    if (apb > 0) {
+        const size_t lds = construct_lds_staged_bytes(b.max_n, k, apb);
+        const dim3 grid(b.ants / apb), block(kLanes * apb);
+#define CKL(R, A) hipLaunchKernelGGL((k_construct_knn_lds<R, A>), grid, block, lds, st, b)
+        if (rng == kRngPhilox) {
+            if (apb == 1) CKL(kRngPhilox, 1); else if (apb == 2) CKL(kRngPhilox, 2); else CKL(kRngPhilox, 4);
+        } else {
+            if (apb == 1) CKL(kRngParkMiller, 1); else if (apb == 2) CKL(kRngParkMiller, 2); else CKL(kRngParkMiller, 4);
+        }
+#undef CKL
+        return hipGetLastError();
+    }
     const size_t lds = construct_lds_bytes(b.max_n, selection);
     const dim3 grid(b.ants), block(kLanes);
     if (selection == kSelIRoulette) {

@@ -157,15 +157,18 @@ __device__ __forceinline__ int ls_routes(const DevProblem &p, LsLds &L, int S, i
 // with the reversals split over the lanes.
 constexpr int kOptGroup = 8;
 
+// Minimum over the aligned group of 8 lanes (lane ^ 1, ^ 2, ^ 4 stay inside
+// the group, all of whose lanes are active).
 __device__ __forceinline__ u64 group_min_u64(u64 v)
 {
-#pragma unroll
-    for (int o = 1; o < kOptGroup; o <<= 1) {
-        const unsigned lo = __shfl_xor((unsigned)v, o), hi = __shfl_xor((unsigned)(v >> 32), o);
-        const u64 w = ((u64)hi << 32) | lo;
-        v = w < v ? w : v;
-    }
-    return v;
+    static_assert(kOptGroup == 8, "the DPP exchanges cover groups of 8 lanes");
+    const int lane = (int)(threadIdx.x & 63);
+    u64 w = dpp_u64<kDppXor1>(v);
+    v = w < v ? w : v;
+    w = dpp_u64<kDppXor2>(v);
+    v = w < v ? w : v;
+    w = dpp_xor4_u64(v, lane);
+    return w < v ? w : v;
 }
 
 __device__ __forceinline__ void group_sync()

@@ -172,23 +172,30 @@ int aco_profile_get(aco_ctx *ctx, const char *name, double *total_ms, int64_t *l
  * Local-search kernel shape: waves per ant (4 or 8), rows in the LDS
  * distance-row ring (8, 16 or 32; >= waves + 2) and prefetch of the next
  * round's rows (needs ring >= 2 waves + 2; applies to instances with
- * n <= 256 waves).  Every valid shape gives identical results. */
+ * n <= 256 waves).  Construction: construct_apb > 0 stages the candidate
+ * table in LDS and runs that many ants per block (1, 2 or 4; candidate
+ * roulette without a route-length limit, ants per instance divisible by
+ * it); 0 reads candidate rows from L2/HBM.  Every valid shape gives
+ * identical results. */
 typedef struct aco_launch_shape {
     int32_t ls_waves;
     int32_t ls_ring;
     int32_t ls_prefetch;
+    int32_t construct_apb;
 } aco_launch_shape;
 
 typedef struct aco_tune_result {
     aco_launch_shape shape;
-    float ms;            /* mean local-search launch time, -1 if it does not fit */
+    int32_t stage;       /* 0: construction candidate, 1: local-search candidate */
+    float ms;            /* mean launch time of that stage, -1 if it does not apply */
 } aco_tune_result;
 
 int aco_get_launch_shape(aco_ctx *ctx, aco_launch_shape *shape);
 /* validates the shape and its LDS size; the iteration graph is re-captured */
 int aco_set_launch_shape(aco_ctx *ctx, const aco_launch_shape *shape);
-/* After aco_init: constructs once from the current state, times every
- * candidate shape `reps` times on those same tours, keeps the fastest and
+/* After aco_init: times every construction candidate `reps` times from the
+ * current state, then every local-search candidate on the same constructed
+ * tours, keeps the fastest of each and
  * restores the state, so the colony's results do not change; the clock of
  * the best-so-far records restarts afterwards.  results (optional,
  * capacity entries) receives every candidate's time. */

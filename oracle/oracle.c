@@ -301,8 +301,10 @@ static int greedy_pick(const orc_problem *p, const float *tinfo, const unsigned 
  *   sends the ant to the depot (kernel.cl:461-465) unless it is already
  *   there, where the greedy pick above applies.
  * selection 1 (roulette_knn, this project's rule): the k candidate nodes of
- *   the current row, masked, inclusive Hillis-Steele scan (offsets
- *   1,2,..,32, as a 64-lane wavefront computes it), r = u * prefix[k-1],
+ *   the current row, masked, inclusive scan in the wavefront's DPP order
+ *   (Hillis-Steele with offsets 1,2,4,8 inside each row of 16 lanes, then
+ *   row 0's total into row 1 and row 2's into row 3, then lane 31's prefix
+ *   into rows 2 and 3), r = u * prefix[k-1],
  *   first candidate with prefix > r (else the last with positive mass);
  *   zero mass falls back to the greedy pick over the full row, and to the
  *   depot when no node is feasible. */
@@ -346,8 +348,18 @@ int orc_construct_ant(const orc_problem *p, const float *tinfo, const float *tik
                 }
                 x[c] = pv[c];
             }
-            for (int d = 1; d < ORC_LANES; d <<= 1)
-                for (int c = ORC_LANES - 1; c >= d; --c) x[c] = x[c] + x[c - d];
+            for (int d = 1; d < 16; d <<= 1)
+                for (int c = ORC_LANES - 1; c >= 0; --c)
+                    if ((c & 15) >= d) x[c] = x[c] + x[c - d];
+            {
+                const float t15 = x[15], t47 = x[47];
+                for (int c = 16; c < 32; ++c) x[c] = x[c] + t15;
+                for (int c = 48; c < 64; ++c) x[c] = x[c] + t47;
+            }
+            {
+                const float t31 = x[31];
+                for (int c = 32; c < 64; ++c) x[c] = x[c] + t31;
+            }
             float total = x[k - 1];
             if (total > 0.0f) {
                 float u = cfg->rng == 1 ? orc_philox_u01(cfg->seed, cfg->iteration, (uint32_t)ant,

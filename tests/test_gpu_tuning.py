@@ -1,6 +1,6 @@
-"""Launch shapes and the autotuner: every local-search shape gives the same
-tours, lengths and evaluated-move counts, and autotuning leaves the
-colony's trajectory unchanged."""
+"""Launch shapes and the autotuner: every construction and local-search
+shape gives the same tours, lengths, fallback and evaluated-move counts,
+and autotuning leaves the colony's trajectory unchanged."""
 import numpy as np
 import pytest
 
@@ -11,7 +11,7 @@ from helpers import log
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = [(4, 16, 1), (4, 16, 0), (4, 8, 0), (8, 32, 1), (8, 16, 0)]
+SHAPES = [(4, 16, 1, 0), (4, 16, 0, 1), (4, 8, 0, 2), (8, 32, 1, 4), (8, 16, 0, 0), (8, 16, 0, 2)]
 
 
 def _small():
@@ -24,8 +24,8 @@ def _large():
     return [I.clustered(3, customers=1100, routes=80), I.uniform_100(2)]
 
 
-def _ls_once(insts, shape):
-    with ca.GpuACO(insts, n_ants=12, k=16, seed=7, selection=ca.SEL_KNN, rng=ca.RNG_PHILOX,
+def _ls_once(insts, shape, k=16, rng=ca.RNG_PHILOX):
+    with ca.GpuACO(insts, n_ants=12, k=k, seed=7, selection=ca.SEL_KNN, rng=rng,
                    ls_flags=7, use_graph=False) as g:
         g.init_aco()
         if shape is not None:
@@ -34,16 +34,17 @@ def _ls_once(insts, shape):
         g.local_search()
         out = [g.ants(b) for b in range(len(insts))]
         g.update_statistics()
-        return out, g.counters()["ls_evals"]
+        c = g.counters()
+        return out, (c["ls_evals"], c["fallbacks"])
 
 
 @pytest.mark.parametrize("which", ["small", "large"])
 def test_every_shape_gives_identical_results(which):
     insts = _small() if which == "small" else _large()
     ref, ref_ev = _ls_once(insts, SHAPES[0])
-    assert ref_ev > 0
+    assert ref_ev[0] > 0
     for sh in SHAPES[1:]:
-        if which == "large" and sh == (8, 32, 1):
+        if which == "large" and sh[:3] == (8, 32, 1):
             # 32 rows of 1101 floats do not fit in 160 KB of LDS
             with pytest.raises(ca.AcoError):
                 _ls_once(insts, sh)
@@ -56,14 +57,33 @@ def test_every_shape_gives_identical_results(which):
                 np.testing.assert_array_equal(x, y, err_msg=f"shape {sh} instance {b}")
 
 
+@pytest.mark.parametrize("rng", [ca.RNG_PHILOX, ca.RNG_PARK_MILLER])
+def test_staged_construction_with_fallbacks(rng):
+    """k = 4 candidates leave many steps without mass (full-row fallback)."""
+    insts = [I.uniform_100(3), I.batch(6, count=1)[0]]
+    ref, ref_c = _ls_once(insts, (4, 16, 1, 0), k=4, rng=rng)
+    assert ref_c[1] > 0
+    for apb in (1, 2, 4):
+        got, c = _ls_once(insts, (4, 16, 1, apb), k=4, rng=rng)
+        assert c == ref_c, (apb, c, ref_c)
+        for b in range(len(insts)):
+            for x, y in zip(got[b], ref[b]):
+                np.testing.assert_array_equal(x, y, err_msg=f"apb {apb} instance {b}")
+
+
 def test_invalid_shapes_rejected():
     inst = I.uniform_100(1)
-    with ca.GpuACO(inst, n_ants=8, ls_flags=7) as g:
-        assert g.launch_shape() == (4, 16, 1)
-        for bad in [(2, 16, 0), (4, 4, 0), (4, 8, 1), (8, 8, 0), (8, 16, 1), (4, 12, 0)]:
+    with ca.GpuACO(inst, n_ants=8, ls_flags=7) as g:   # I-Roulette: no staged construction
+        assert g.launch_shape() == (4, 16, 1, 0)
+        for bad in [(2, 16, 0), (4, 4, 0), (4, 8, 1), (8, 8, 0), (8, 16, 1), (4, 12, 0),
+                    (4, 16, 1, 1), (4, 16, 1, 3)]:
             with pytest.raises(ca.AcoError):
                 g.set_launch_shape(*bad)
-        assert g.launch_shape() == (4, 16, 1)
+        assert g.launch_shape() == (4, 16, 1, 0)
+    with ca.GpuACO(inst, n_ants=6, k=16, selection=ca.SEL_KNN, ls_flags=7) as g:
+        g.set_launch_shape(4, 16, 1, 2)
+        with pytest.raises(ca.AcoError):     # 6 ants are not a multiple of 4
+            g.set_launch_shape(4, 16, 1, 4)
 
 
 def test_autotune_keeps_the_trajectory():
@@ -76,8 +96,10 @@ def test_autotune_keeps_the_trajectory():
             if tune:
                 res = g.autotune(reps=2)
                 log(f"autotune: {res} -> {g.launch_shape()}")
-                assert any(ms > 0 for _, ms in res)
-                assert g.launch_shape() in [sh for sh, ms in res if ms > 0]
+                assert any(ms > 0 for _, st, ms in res if st == 0)
+                assert any(ms > 0 for _, st, ms in res if st == 1)
+                assert g.launch_shape()[:3] in [sh[:3] for sh, st, ms in res if st == 1 and ms > 0]
+                assert g.launch_shape()[3] in [sh[3] for sh, st, ms in res if st == 0 and ms > 0]
             g.run(4)
             runs.append([(g.ants(b), g.pheromone(b), g.bsf_records(b)) for b in range(len(insts))])
     for b in range(len(insts)):
