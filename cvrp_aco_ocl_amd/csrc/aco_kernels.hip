@@ -42,6 +42,13 @@ __device__ __forceinline__ float philox_u01(u64 seed, u32 iter, u32 ant, u32 ste
     return (float)(c0 >> 8) * 0x1p-24f;
 }
 
+// Philox draws of 64 consecutive steps at once: lane l holds the draw of
+// step first + l (the same counters as one draw per step).
+__device__ __forceinline__ float philox_block(u64 seed, u32 iter, u32 ant, u32 first, int lane)
+{
+    return philox_u01(seed, iter, ant, first + (u32)lane);
+}
+
 // ------------------------------------------------------- wave helpers ----
 
 __device__ __forceinline__ u64 wave_max_u64(u64 v) { return wave_reduce_u64<true>(v); }
@@ -150,6 +157,23 @@ __device__ __forceinline__ void tour_close(int *tour, int &tbuf, int step, int l
 {
     if (lane == (step & 63)) tbuf = 0;
     if (lane <= (step & 63)) tour[(step & ~63) + lane] = tbuf;
+}
+
+// Tour length after construction: the edges d(tour[q-1], tour[q]), q = 1..
+// last, added one by one in step order -- the same additions the step loop
+// would make -- with the loads issued 64 at a time.
+__device__ __forceinline__ float tour_length_in_order(const DevProblem &p, const int *tour, int last,
+                                                      int lane)
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");   // this wave's tour stores, fresh L1
+    float t = 0.0f;
+    for (int base = 1; base <= last; base += kLanes) {
+        const int q = base + lane;
+        const float d = q <= last ? p.dist[(size_t)tour[q - 1] * p.P + tour[q]] : 0.0f;
+        const int cnt = last - base + 1 < kLanes ? last - base + 1 : kLanes;
+        for (int l = 0; l < cnt; ++l) t = t + lane_f32(d, l);
+    }
+    return t;
 }
 
 // construct_solution with I-Roulette (kernel.cl:325-373, 382-493): one
@@ -261,15 +285,18 @@ __global__ __launch_bounds__(64) void k_construct_knn(DevBatch bt)
     int *tour = s.tours + (size_t)a * p.T;
     int cur = 0, load = 0, vcnt = 1, step = 0, fb = 0, tbuf = 0;
     float rdist = 0.0f, tdist = 0.0f;
+    float ublk = 0.0f;   // Philox draws of steps step+1 .. step+64, refreshed every 64 steps
     while (vcnt < n) {
         step++;
+        if (RNG == kRngPhilox && ((step - 1) & 63) == 0) ublk = philox_block(seed, iter, (u32)a, (u32)step, lane);
         int j = 0;
         float pj = 0.0f, dcj = 0.0f;
         if (lane < k) {
-            // candidate id, its total_info and d(cur, id): one contiguous read each
+            // candidate id, its total_info and (with a length limit) d(cur, id):
+            // one contiguous read each
             j = p.nnk[cur * k + lane];
             const float tj = s.tiknn[cur * k + lane];
-            dcj = p.nnkd[cur * k + lane];
+            if (HAS_D) dcj = p.nnkd[cur * k + lane];
             pj = feasible<HAS_D>(p, L, j, dcj, load, rdist) ? tj : 0.0f;
         }
         const float x = wave_scan_f32(pj, lane);
@@ -277,8 +304,7 @@ __global__ __launch_bounds__(64) void k_construct_knn(DevBatch bt)
         int w;
         float dcw;
         if (total > 0.0f) {
-            const float u = RNG == kRngPhilox ? philox_u01(seed, iter, (u32)a, (u32)step)
-                                              : pm_ran01(z);
+            const float u = RNG == kRngPhilox ? lane_f32(ublk, (step - 1) & 63) : pm_ran01(z);
             const float r = u * total;
             const u64 m1 = __ballot(lane < k && x > r);
             int wc;
@@ -289,12 +315,12 @@ __global__ __launch_bounds__(64) void k_construct_knn(DevBatch bt)
                 wc = 63 - __clzll((long long)m2);
             }
             w = lane_i32(j, wc);
-            dcw = lane_f32(dcj, wc);
+            dcw = HAS_D ? lane_f32(dcj, wc) : 0.0f;
         } else {
             const float *drow = p.dist + (size_t)cur * p.P;
             w = greedy_pick<HAS_D>(p, L, s.tinfo + (size_t)cur * p.P, drow, load, rdist, lane);
             fb++;
-            dcw = drow[uniform(w)];
+            dcw = HAS_D ? drow[uniform(w)] : 0.0f;
         }
         w = uniform(w);
         if (w == 0) {
@@ -311,9 +337,12 @@ __global__ __launch_bounds__(64) void k_construct_knn(DevBatch bt)
         cur = w;
         __syncthreads();
     }
-    tdist = tdist + L.d0[cur];
+    if (HAS_D) tdist = tdist + L.d0[cur];
     step++;
     tour_close(tour, tbuf, step, lane);
+    // without a length limit the step loop carries no edge lengths: the same
+    // additions in the same order afterwards
+    if (!HAS_D) tdist = tour_length_in_order(p, tour, step, lane);
     if (lane == 0) {
         s.sizes[a] = step + 1;
         s.lens[a] = tdist;
@@ -362,24 +391,23 @@ __global__ __launch_bounds__(64 * A) void k_construct_knn_lds(DevBatch bt)
     int z = RNG == kRngParkMiller ? s.pm[a * kLanes] : 0;
     int *tour = s.tours + (size_t)a * p.T;
     int cur = 0, load = 0, vcnt = 1, step = 0, fb = 0, tbuf = 0;
-    float tdist = 0.0f, pend = 0.0f;
+    float ublk = 0.0f;   // Philox draws of steps step+1 .. step+64, refreshed every 64 steps
     while (vcnt < n) {
         step++;
-        const float uph = RNG == kRngPhilox ? philox_u01(seed, iter, (u32)a, (u32)step) : 0.0f;
-        int j = 0;
+        if (RNG == kRngPhilox && ((step - 1) & 63) == 0) ublk = philox_block(seed, iter, (u32)a, (u32)step, lane);
+        int j = 0, dj = 0;
         float pj = 0.0f;
         if (lane < k) {
             j = nid[cur * k + lane];
             const float tj = tik[cur * k + lane];
-            pj = feasible<false>(p, L, j, 0.0f, load, 0.0f) ? tj : 0.0f;
+            dj = dem[j];
+            pj = (!vis[j] && load + dj <= p.Q) ? tj : 0.0f;   // the mask of feasible<false>
         }
-        tdist = tdist + pend;   // previous step's edge
         const float x = wave_scan_f32(pj, lane);
         const float total = lane_f32(x, k - 1);
-        int w;
-        float dcw;
+        int w, dw;
         if (total > 0.0f) {
-            const float u = RNG == kRngPhilox ? uph : pm_ran01(z);
+            const float u = RNG == kRngPhilox ? lane_f32(ublk, (step - 1) & 63) : pm_ran01(z);
             const float r = u * total;
             const u64 m1 = __ballot(lane < k && x > r);
             int wc;
@@ -390,31 +418,31 @@ __global__ __launch_bounds__(64 * A) void k_construct_knn_lds(DevBatch bt)
                 wc = 63 - __clzll((long long)m2);
             }
             w = lane_i32(j, wc);
-            dcw = p.nnkd[cur * k + uniform(wc)];
+            dw = lane_i32(dj, wc);
         } else {
-            const float *drow = p.dist + (size_t)cur * p.P;
-            w = greedy_pick<false>(p, L, s.tinfo + (size_t)cur * p.P, drow, load, 0.0f, lane);
+            w = greedy_pick<false>(p, L, s.tinfo + (size_t)cur * p.P, nullptr, load, 0.0f, lane);
             fb++;
-            dcw = drow[uniform(w)];
+            w = uniform(w);
+            dw = dem[w];
         }
         w = uniform(w);
         if (w == 0) {
             load = 0;
         } else {
-            load += dem[w];
+            load += dw;
             vcnt++;
         }
+        // lane 0's flag is read by the other lanes next step: LDS accesses of
+        // one wave complete in order, and the store and the loads use the
+        // same array, so the compiler keeps their order
         if (lane == 0) vis[w] = 1;
-        pend = dcw;
         tour_put(tour, tbuf, step, w, lane);
         cur = w;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
     }
-    tdist = tdist + pend;
-    tdist = tdist + d0[cur];
     step++;
     tour_close(tour, tbuf, step, lane);
+    const float tdist = tour_length_in_order(p, tour, step, lane);
     if (lane == 0) {
         s.sizes[a] = step + 1;
         s.lens[a] = tdist;
