@@ -327,7 +327,7 @@ def main():
             "n": inst.n, "ants": m, "k": k, "capacity": inst.capacity, "seed": args.seed,
             "step": "one ACO iteration (hipGraph replay)",
             "launch_shape": {"ls_waves": shape[0], "ls_ring": shape[1], "ls_prefetch": shape[2],
-                             "construct_ants_per_block": shape[3]},
+                             "construct_ants_per_block": shape[3], "ls_row16": shape[4]},
         },
         "roofline": roofline,
         "kernels": per_kernel,

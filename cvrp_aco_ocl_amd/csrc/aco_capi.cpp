@@ -81,9 +81,10 @@ struct aco_ctx {
     float tick_to_s = 0.0f;
     bool profiling = false;
     std::map<std::string, ProfSlot> prof;
-    LsShape ls{4, 16, 1};
+    LsShape ls{4, 16, 1, 0, 0};
     size_t lds_limit = 0;
     int max_n = 0;
+    bool int16_rows = true;   // every distance an integer in [0, 65535]
 };
 
 #define HIP_TRY(expr)                                                                      \
@@ -124,6 +125,7 @@ static size_t carve_instance(InstanceData &I, int m, int k, int nnP, int ras, in
     float *nnd = c.take<float>((size_t)n * nnP);
     int *rank = c.take<int>(n);
     unsigned short *nnpos = c.take<unsigned short>((size_t)n * n);
+    unsigned short *dist16 = c.take<unsigned short>((size_t)n * P);
     int *r64 = c.take<int>(kLanes);
     int *r64i = c.take<int>(kLanes);
     ds.tau = c.take<float>((size_t)n * P);
@@ -147,6 +149,7 @@ static size_t carve_instance(InstanceData &I, int m, int k, int nnP, int ras, in
     dp.dist = dist; dp.heur = heur; dp.d0 = d0; dp.dem = dem; dp.nnk = nnk; dp.nnkd = nnkd;
     dp.nnls = nnls; dp.nnd = nnd; dp.rank = rank; dp.rank64 = r64; dp.rank64inv = r64i;
     dp.nnpos = nnpos;
+    dp.dist16 = dist16;
     return c.off;
 }
 
@@ -373,9 +376,9 @@ int aco_create_batch(const aco_problem *pbs, int32_t count, const aco_params *pr
     c->lds_limit = prop.sharedMemPerBlock;
     c->max_n = max_n;
     c->ls = LsShape{4, 16, 1, 0};
-    if (local_search_lds_bytes(max_n, 16) > prop.sharedMemPerBlock) c->ls = LsShape{4, 8, 0, 0};
+    if (local_search_lds_bytes(max_n, 16, 0) > prop.sharedMemPerBlock) c->ls = LsShape{4, 8, 0, 0, 0};
     const size_t lds_need = std::max(construct_lds_bytes(max_n, prm.selection),
-                                     prm.ls_flags ? local_search_lds_bytes(max_n, c->ls.ring) : 0);
+                                     prm.ls_flags ? local_search_lds_bytes(max_n, c->ls.ring, 0) : 0);
     if (lds_need > prop.sharedMemPerBlock)
         return fail(set_error(ACO_ERR_ARG, "instance too large for LDS: needs " + std::to_string(lds_need) +
                                                " bytes, device has " + std::to_string(prop.sharedMemPerBlock)));
@@ -406,6 +409,11 @@ int aco_create_batch(const aco_problem *pbs, int32_t count, const aco_params *pr
         I.P = (n + 3) & ~3;
         I.T = 2 * n;
         I.h_dist.assign(pb->dist, pb->dist + (size_t)n * n);
+        for (float d : I.h_dist)
+            if (!(d >= 0.0f && d <= 65535.0f && d == floorf(d))) {
+                c->int16_rows = false;
+                break;
+            }
         I.h_dem.assign(pb->demand, pb->demand + n);
         I.Q = pb->capacity;
         I.D = pb->max_dist;
@@ -462,6 +470,14 @@ int aco_create_batch(const aco_problem *pbs, int32_t count, const aco_params *pr
         CHK(h2d((void *)dp.nnd, nnd.data(), nnd.size() * 4));
         CHK(h2d((void *)dp.rank, rank.data(), (size_t)n * 4));
         CHK(h2d((void *)dp.nnpos, nnpos.data(), nnpos.size() * 2));
+        std::vector<unsigned short> d16((size_t)n * P, 0);
+        bool fits16 = true;
+        for (size_t e = 0; e < (size_t)n * n && fits16; ++e) {
+            const float d = pb->dist[e];
+            fits16 = d >= 0.0f && d <= 65535.0f && d == floorf(d);
+            if (fits16) d16[(e / n) * P + e % n] = (unsigned short)d;
+        }
+        if (fits16) CHK(h2d((void *)dp.dist16, d16.data(), d16.size() * 2));
         CHK(h2d((void *)dp.rank64, r64.data(), kLanes * 4));
         CHK(h2d((void *)dp.rank64inv, r64inv.data(), kLanes * 4));
         CHK(hipStreamSynchronize(c->stream));  // host vectors die at the end of this iteration
@@ -848,7 +864,8 @@ static void drop_graph(aco_ctx *c)
 
 static bool shape_fits(const aco_ctx *c, const LsShape &sh)
 {
-    if (!ls_shape_valid(sh) || local_search_lds_bytes(c->max_n, sh.ring) > c->lds_limit) return false;
+    if (!ls_shape_valid(sh) || local_search_lds_bytes(c->max_n, sh.ring, sh.row16) > c->lds_limit) return false;
+    if (sh.row16 && !c->int16_rows) return false;
     if (!construct_staged_ok(c->bt, c->prm.selection, c->prm.k, c->m, sh.construct_apb)) return false;
     return sh.construct_apb == 0 ||
            construct_lds_staged_bytes(c->max_n, c->prm.k, sh.construct_apb) <= c->lds_limit;
@@ -856,7 +873,7 @@ static bool shape_fits(const aco_ctx *c, const LsShape &sh)
 
 static aco_launch_shape to_public(const LsShape &sh)
 {
-    return aco_launch_shape{sh.waves, sh.ring, sh.prefetch, sh.construct_apb};
+    return aco_launch_shape{sh.waves, sh.ring, sh.prefetch, sh.construct_apb, sh.row16};
 }
 
 int aco_get_launch_shape(aco_ctx *c, aco_launch_shape *out)
@@ -873,7 +890,10 @@ int aco_set_launch_shape(aco_ctx *c, const aco_launch_shape *in)
     int rc = need(c, false);
     if (rc) return rc;
     if (!in) return set_error(ACO_ERR_ARG, "null shape");
-    const LsShape sh{in->ls_waves, in->ls_ring, in->ls_prefetch ? 1 : 0, in->construct_apb};
+    const LsShape sh{in->ls_waves, in->ls_ring, in->ls_prefetch ? 1 : 0, in->construct_apb,
+                     in->ls_row16 ? 1 : 0};
+    if (sh.row16 && !c->int16_rows)
+        return set_error(ACO_ERR_ARG, "16-bit rows need every distance to be an integer in [0, 65535]");
     if (!ls_shape_valid(sh))
         return set_error(ACO_ERR_ARG, "invalid shape: waves 4|8, ring 8|16|32 >= waves+2, "
                                       "prefetch needs ring >= 2*waves+2");
@@ -895,7 +915,8 @@ int aco_autotune(aco_ctx *c, int32_t reps, aco_tune_result *results, int32_t cap
     if (rc) return rc;
     if (reps < 1 || reps > 1000) return set_error(ACO_ERR_ARG, "reps must be in [1, 1000]");
     static const int kApb[] = {0, 1, 2, 4};
-    static const LsShape kLs[] = {{4, 16, 1, 0}, {4, 16, 0, 0}, {4, 8, 0, 0}, {8, 32, 1, 0}, {8, 16, 0, 0}};
+    static const LsShape kLs[] = {{4, 16, 1, 0, 0}, {4, 16, 0, 0, 0}, {4, 8, 0, 0, 0}, {8, 32, 1, 0, 0},
+                                  {8, 16, 0, 0, 0}, {4, 16, 1, 0, 1}, {8, 32, 1, 0, 1}, {8, 16, 0, 0, 1}};
     constexpr int kNc = (int)(sizeof(kApb) / sizeof(kApb[0]));
     constexpr int kNl = (int)(sizeof(kLs) / sizeof(kLs[0]));
     if (count) *count = kNc + kNl;
@@ -983,6 +1004,7 @@ int aco_autotune(aco_ctx *c, int32_t reps, aco_tune_result *results, int32_t cap
         chosen.waves = kLs[best_l].waves;
         chosen.ring = kLs[best_l].ring;
         chosen.prefetch = kLs[best_l].prefetch;
+        chosen.row16 = kLs[best_l].row16;
     }
     if (best_c >= 0) chosen.construct_apb = kApb[best_c];
     c->ls = chosen;

@@ -103,6 +103,7 @@ struct DevProblem {
     const int *nnls;          // n*nnP: 2-opt neighbour ids, rows padded to nnP (multiple of 4)
     const float *nnd;         // n*nnP: d(i, nnls[i][h]), so a row scan reads (id, distance) pairs
     const unsigned short *nnpos;  // n*n: position of j in i's nn row (0xFFFF if absent)
+    const unsigned short *dist16; // n*P: distances as 16-bit integers (valid when they all are)
     const int *rank;          // n, priority order of the I-Roulette reduction
     const int *rank64;        // 64, same for best_solution_phase_0 groups
     const int *rank64inv;     // 64
@@ -148,6 +149,7 @@ struct LsShape {
     int ring;
     int prefetch;
     int construct_apb;   // construction: ants per block with the candidate table in LDS, 0 = off
+    int row16;           // local-search rows as exact 16-bit integers (integer distances <= 65535)
 };
 
 // ---- host launchers (aco_kernels.hip, aco_local_search.hip) ----
@@ -163,7 +165,7 @@ hipError_t launch_gather(const DevBatch &b, hipStream_t st);
 hipError_t launch_pheromone_init(const DevBatch &b, int mode, float tau0, hipStream_t st);
 hipError_t launch_stamp_clock(const DevBatch &b, hipStream_t st);
 size_t construct_lds_bytes(int n, int selection);
-size_t local_search_lds_bytes(int n, int ring);
+size_t local_search_lds_bytes(int n, int ring, int row16);
 bool ls_shape_valid(const LsShape &sh);
 
 }  // namespace cvrp

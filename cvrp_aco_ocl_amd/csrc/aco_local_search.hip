@@ -27,7 +27,6 @@ namespace cvrp {
 
 using u64 = unsigned long long;
 
-constexpr int kPrefetch = 4;    // row elements per thread held for the next round
 // LDS words of L.res for W waves: [0,W) and [W,2W) per-wave scan results
 // (accepted j) of even and odd rounds, [2W,4W) the moves each wave's scan
 // evaluated up to its result (same parity split), 4W the route count.
@@ -37,7 +36,6 @@ template <int W> struct LsW {
     static constexpr int threads = W * kLanes;
     static constexpr int res_ev = 2 * W;
     static constexpr int res_r = 4 * W;
-    static constexpr int prefetch_max_n = threads * kPrefetch;
 };
 
 struct LsLds {
@@ -53,7 +51,7 @@ struct LsOffsets {
     size_t tour, tmp, dtw, rl, rd, dem, rows, res, pos, rof, ridx, zpos, rc, dlb, total;
 };
 
-__host__ __device__ inline LsOffsets ls_offsets(int n, int ring)
+__host__ __device__ inline LsOffsets ls_offsets(int n, int ring, int rowbytes)
 {
     const size_t T = 2 * (size_t)n, N = (size_t)n;
     LsOffsets o;
@@ -64,7 +62,7 @@ __host__ __device__ inline LsOffsets ls_offsets(int n, int ring)
     o.rl = at;   at = ls_align16(at + 4 * N);
     o.rd = at;   at = ls_align16(at + 4 * N);
     o.dem = at;  at = ls_align16(at + 4 * N);
-    o.rows = at; at = ls_align16(at + 4 * N * (size_t)ring);
+    o.rows = at; at = ls_align16(at + (size_t)rowbytes * (size_t)(rowbytes == 4 ? n : (n + 1) & ~1) * (size_t)ring);
     o.res = at;  at = ls_align16(at + 4 * kResWords);
     o.pos = at;  at = ls_align16(at + 2 * N);
     o.rof = at;  at = ls_align16(at + 2 * N);
@@ -76,9 +74,9 @@ __host__ __device__ inline LsOffsets ls_offsets(int n, int ring)
     return o;
 }
 
-__device__ __forceinline__ LsLds ls_bind(int n, int ring, unsigned char *b)
+__device__ __forceinline__ LsLds ls_bind(int n, int ring, int rowbytes, unsigned char *b)
 {
-    const LsOffsets o = ls_offsets(n, ring);
+    const LsOffsets o = ls_offsets(n, ring, rowbytes);
     LsLds L;
     L.tour = (int *)(b + o.tour); L.tmp = (int *)(b + o.tmp); L.dtw = (float *)(b + o.dtw);
     L.rl = (int *)(b + o.rl); L.rd = (float *)(b + o.rd); L.dem = (int *)(b + o.dem);
@@ -88,7 +86,7 @@ __device__ __forceinline__ LsLds ls_bind(int n, int ring, unsigned char *b)
     return L;
 }
 
-size_t local_search_lds_bytes(int n, int ring) { return ls_offsets(n, ring).total; }
+size_t local_search_lds_bytes(int n, int ring, int row16) { return ls_offsets(n, ring, row16 ? 2 : 4).total; }
 
 __device__ __forceinline__ float Dd(const DevProblem &p, int a, int b)
 {
@@ -349,13 +347,48 @@ struct RowWindow {
     unsigned mask;
 };
 
-struct RowRing {
-    float *rows;
-    int mask;   // ring size - 1
-    int n;
+// Rows hold fp32 distances, or (when every distance of the context is an
+// integer in [0, 65535]) the same values as exact 16-bit integers, copied
+// from the 16-bit distance matrix.
+__device__ __forceinline__ float rget(const float *r, int i) { return r[i]; }
+__device__ __forceinline__ float rget(const unsigned short *r, int i) { return (float)r[i]; }
+
+template <typename RT> struct RowRing {
+    RT *rows;
+    int mask;     // ring size - 1
+    int stride;   // elements per row slot: n rounded up to a whole number of dwords
     __device__ __forceinline__ int slot(int pos) const { return pos & mask; }
-    __device__ __forceinline__ float *at(int pos) const { return rows + slot(pos) * n; }
+    __device__ __forceinline__ RT *at(int pos) const { return rows + slot(pos) * stride; }
 };
+
+template <typename RT> __host__ __device__ constexpr int row_stride(int n)
+{
+    return sizeof(RT) == 4 ? n : (n + 1) & ~1;
+}
+
+// Source rows in global memory: fp32 distances, or their 16-bit copy.
+__device__ __forceinline__ const float *row_src(const DevProblem &p, int v, float *) { return p.dist + (size_t)v * p.P; }
+__device__ __forceinline__ const unsigned short *row_src(const DevProblem &p, int v, unsigned short *)
+{
+    return p.dist16 + (size_t)v * p.P;
+}
+
+// Copy distance row v into LDS with direct global->LDS loads (one dword per
+// lane per instruction, no registers; completes with vmcnt, so readers wait
+// for vmcnt(0) and a barrier).  Rows of both types start dword aligned and
+// the copy may run up to one padding element past n (within P and within
+// the slot stride).
+template <int W, typename RT>
+__device__ __forceinline__ void row_copy_lds(const DevProblem &p, int v, RT *dst, int tid)
+{
+    const int nd = (p.n * (int)sizeof(RT) + 3) >> 2;
+    const unsigned *src = reinterpret_cast<const unsigned *>(row_src(p, v, (RT *)nullptr));
+    unsigned *d = reinterpret_cast<unsigned *>(dst);
+    const int lane = tid & 63;
+    for (int c = tid; c < nd; c += LsW<W>::threads)
+        __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(src + c),
+                                         (__attribute__((address_space(3))) void *)(d + (c - lane)), 4, 0, 0);
+}
 
 // Start the window at position base (never moves backwards within a scan).
 __device__ __forceinline__ void window_move(RowWindow &w, int base)
@@ -377,74 +410,52 @@ __device__ __forceinline__ void window_clear(RowWindow &w, int lo, int hi)
 
 // Make the rows of window offsets [0, cnt) present (cnt <= W+2): all
 // threads load the missing ones; a barrier follows when any was missing.
-template <int W>
-__device__ __forceinline__ void rows_ensure(const DevProblem &p, const LsLds &L, const RowRing &rr,
+template <int W, typename RT>
+__device__ __forceinline__ void rows_ensure(const DevProblem &p, const LsLds &L, const RowRing<RT> &rr,
                                             RowWindow &w, int cnt, int tid)
 {
-    const int n = p.n;
     bool loaded = false;
 #pragma unroll
     for (int q = 0; q < W + 2; ++q) {
         if (q >= cnt) break;
         if ((w.mask >> q) & 1u) continue;
         const int pos = w.base + q;
-        const int v = L.tour[pos];
-        float *row = rr.at(pos);
-        const float *src = p.dist + (size_t)v * p.P;
-        for (int c = tid; c < n; c += LsW<W>::threads) row[c] = src[c];
+        row_copy_lds<W, RT>(p, L.tour[pos], rr.at(pos), tid);
         loaded = true;
     }
     w.mask |= (1u << cnt) - 1u;
-    if (loaded) __syncthreads();
+    if (loaded) {
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+    }
 }
 
-// Next round's rows (window offsets [off, off+W), positions < S) are loaded
-// into registers before a scan and stored into the ring after it, so the
-// loads overlap the scan.  Enabled per launch when the ring holds both
-// rounds' positions (2W+2 <= R) and per instance when n <= 256 W (4
-// elements per thread): the positions of this round and the next never
-// share a slot.
-template <int W> struct RowPrefetch {
-    float v[W][kPrefetch];
-    unsigned want;
-};
-
-template <int W>
-__device__ __forceinline__ void prefetch_issue(const DevProblem &p, const LsLds &L, bool on,
-                                               const RowWindow &w, RowPrefetch<W> &pf, int off,
-                                               int S, int tid)
+// Next round's rows (window offsets [off, off+W), positions < S) are copied
+// into their ring slots before the scan (direct loads, no registers) and
+// waited for before the round's barrier.  Enabled per launch when the ring
+// holds both rounds' positions (2W+2 <= R): the positions of this round and
+// the next never share a slot.
+template <int W, typename RT>
+__device__ __forceinline__ unsigned prefetch_issue(const DevProblem &p, const LsLds &L, const RowRing<RT> &rr,
+                                                   bool on, const RowWindow &w, int off, int S, int tid)
 {
-    pf.want = 0;
-    if (!on) return;
+    unsigned want = 0;
+    if (!on) return 0;
 #pragma unroll
     for (int q = 0; q < W; ++q) {
         const int pos = w.base + off + q;
         if (pos >= S || ((w.mask >> (off + q)) & 1u)) continue;
-        pf.want |= 1u << q;
-        const float *src = p.dist + (size_t)L.tour[pos] * p.P;
-#pragma unroll
-        for (int e = 0; e < kPrefetch; ++e) {
-            const int c = tid + e * LsW<W>::threads;
-            pf.v[q][e] = c < p.n ? src[c] : 0.0f;
-        }
+        want |= 1u << q;
+        row_copy_lds<W, RT>(p, L.tour[pos], rr.at(pos), tid);
     }
+    return want;
 }
 
-template <int W>
-__device__ __forceinline__ void prefetch_store(const DevProblem &p, const RowRing &rr, RowWindow &w,
-                                               const RowPrefetch<W> &pf, int off, int tid)
+// Before the round's barrier: the prefetched rows have landed.
+__device__ __forceinline__ void prefetch_wait(RowWindow &w, unsigned want, int off)
 {
-#pragma unroll
-    for (int q = 0; q < W; ++q) {
-        if (!((pf.want >> q) & 1u)) continue;
-        float *row = rr.at(w.base + off + q);
-#pragma unroll
-        for (int e = 0; e < kPrefetch; ++e) {
-            const int c = tid + e * LsW<W>::threads;
-            if (c < p.n) row[c] = pf.v[q][e];
-        }
-    }
-    w.mask |= pf.want << off;
+    if (want) __builtin_amdgcn_s_waitcnt(0);
+    w.mask |= want << off;
 }
 
 // -------------------------------------------------------------- swap ----
@@ -459,14 +470,15 @@ __device__ __forceinline__ u64 lanes_upto(int f) { return (2ull << f) - 1ull; }
 // Swap of position i with position j (localSearch.cpp:301-410): sets ok
 // when the move improves and is feasible, val when j is a customer
 // position (an evaluated move).
+template <typename RT>
 __device__ __forceinline__ void swap_eval(const DevProblem &p, const LsLds &L, int S, int i, int j,
-                                          int r1, int dn1, float a1, float b1, const float *rowA,
-                                          const float *rowB, const float *rowC, bool &ok, bool &val)
+                                          int r1, int dn1, float a1, float b1, const RT *rowA,
+                                          const RT *rowB, const RT *rowC, bool &ok, bool &val)
 {
     const int jj = j < S - 1 ? j : S - 1;
     const int n2 = L.tour[jj], pn2 = L.tour[jj - 1], sn2 = L.tour[jj < S - 1 ? jj + 1 : jj];
     const float dm = L.dtw[jj - 1], d0 = L.dtw[jj];
-    const float rA = rowA[n2], rC = rowC[n2], rBp = rowB[pn2], rBs = rowB[sn2];
+    const float rA = rget(rowA, n2), rC = rget(rowC, n2), rBp = rget(rowB, pn2), rBs = rget(rowB, sn2);
     val = j < S && n2 != 0;
     float gain;
     if (j == i + 1)
@@ -488,13 +500,14 @@ __device__ __forceinline__ void swap_eval(const DevProblem &p, const LsLds &L, i
 
 // Position i against the current tour; returns the first accepted j or -1
 // and in evc the moves the sequential scan evaluates up to it.
+template <typename RT>
 __device__ __forceinline__ int swap_scan(const DevProblem &p, const LsLds &L, int S, int i,
-                                         const float *rowA, const float *rowB, const float *rowC,
+                                         const RT *rowA, const RT *rowB, const RT *rowC,
                                          int lane, int &evc)
 {
     const int n1 = L.tour[i];
     const int sn1 = L.tour[i + 1], r1 = L.ridx[i];
-    const float a1 = rowA[n1], b1 = rowB[sn1];
+    const float a1 = rget(rowA, n1), b1 = rget(rowB, sn1);
     const int dn1 = L.dem[n1];
     int cnt = 0;
     for (int jb = i + 1; jb < S; jb += kScanUnroll * kLanes) {
@@ -518,8 +531,8 @@ __device__ __forceinline__ int swap_scan(const DevProblem &p, const LsLds &L, in
     return -1;
 }
 
-template <int W>
-__device__ __forceinline__ void ls_swap(const DevProblem &p, LsLds &L, const RowRing &rr, bool pf_on,
+template <int W, typename RT>
+__device__ __forceinline__ void ls_swap(const DevProblem &p, LsLds &L, const RowRing<RT> &rr, bool pf_on,
                                         int S, int tid, long long &ev, int &moves)
 {
     const int lane = tid & 63, w = tid >> 6;
@@ -529,9 +542,8 @@ __device__ __forceinline__ void ls_swap(const DevProblem &p, LsLds &L, const Row
     while (i0 < S) {
         // rows of positions i0-1 .. i0+W (pred, self, succ of each scanned i)
         window_move(win, i0 - 1);
-        rows_ensure<W>(p, L, rr, win, min(W + 2, S - i0 + 1), tid);
-        RowPrefetch<W> pf;
-        prefetch_issue<W>(p, L, pf_on, win, pf, W + 2, S, tid);
+        rows_ensure<W, RT>(p, L, rr, win, min(W + 2, S - i0 + 1), tid);
+        const unsigned pfw = prefetch_issue<W, RT>(p, L, rr, pf_on, win, W + 2, S, tid);
         const int i = i0 + w;
         int j = -1, evc = 0;
         if (i < S && L.tour[i] != 0)
@@ -539,7 +551,7 @@ __device__ __forceinline__ void ls_swap(const DevProblem &p, LsLds &L, const Row
         int *res = L.res + W * parity, *rese = L.res + LsW<W>::res_ev + W * parity;
         parity ^= 1;
         if (lane == 0) { res[w] = j; rese[w] = evc; }
-        prefetch_store<W>(p, rr, win, pf, W + 2, tid);
+        prefetch_wait(win, pfw, W + 2);
         __syncthreads();
         // the sequential scan evaluates the waves' positions in order and
         // stops at the first accepted move
@@ -557,10 +569,10 @@ __device__ __forceinline__ void ls_swap(const DevProblem &p, LsLds &L, const Row
         }
         const int is = i0 + wstar, js = res[wstar];
         if (tid == 0) {
-            const float *rowA = rr.at(is - 1), *rowB = rr.at(is), *rowC = rr.at(is + 1);
+            const RT *rowA = rr.at(is - 1), *rowB = rr.at(is), *rowC = rr.at(is + 1);
             const int n1 = L.tour[is], sn1 = L.tour[is + 1], r1 = L.ridx[is];
             const int n2 = L.tour[js], pn2 = L.tour[js - 1], sn2 = L.tour[js + 1], r2 = L.ridx[js];
-            const float a1 = rowA[n1], b1 = rowB[sn1];
+            const float a1 = rget(rowA, n1), b1 = rget(rowB, sn1);
             const int dn1 = L.dem[n1];
             int load1, load2;
             float dist1, dist2;
@@ -568,15 +580,15 @@ __device__ __forceinline__ void ls_swap(const DevProblem &p, LsLds &L, const Row
                 const int dn2 = L.dem[n2];
                 load1 = L.rl[r1] - dn1 + dn2;
                 load2 = L.rl[r2] - dn2 + dn1;
-                dist1 = (L.rd[r1] - (a1 + b1)) + (rowA[n2] + rowC[n2]);
-                dist2 = (L.rd[r2] - (L.dtw[js - 1] + L.dtw[js])) + (rowB[pn2] + rowB[sn2]);
+                dist1 = (L.rd[r1] - (a1 + b1)) + (rget(rowA, n2) + rget(rowC, n2));
+                dist2 = (L.rd[r2] - (L.dtw[js - 1] + L.dtw[js])) + (rget(rowB, pn2) + rget(rowB, sn2));
             } else {
                 float gain;
                 if (js == is + 1)
-                    gain = -(a1 + L.dtw[js]) + (rowA[n2] + rowB[sn2]);
+                    gain = -(a1 + L.dtw[js]) + (rget(rowA, n2) + rget(rowB, sn2));
                 else
                     gain = -(a1 + b1 + L.dtw[js - 1] + L.dtw[js]) +
-                           (rowA[n2] + rowC[n2] + rowB[pn2] + rowB[sn2]);
+                           (rget(rowA, n2) + rget(rowC, n2) + rget(rowB, pn2) + rget(rowB, sn2));
                 load1 = L.rl[r1];
                 load2 = load1;
                 dist1 = L.rd[r1] + gain;
@@ -604,16 +616,17 @@ __device__ __forceinline__ void ls_swap(const DevProblem &p, LsLds &L, const Row
 
 // Relocate of position i next to position j (neighbourSearch.cpp:280-293,
 // move.cpp:32-51): ok when improving and feasible, val when evaluated.
+template <typename RT>
 __device__ __forceinline__ void relocate_eval(const DevProblem &p, const LsLds &L, int S, int i,
                                               int j, int r1, int dn1, float a1, float b1, float c1,
-                                              const float *rowB, bool &ok, bool &val)
+                                              const RT *rowB, bool &ok, bool &val)
 {
     const int jj = j < S - 1 ? j : S - 2;
     const int n2 = L.tour[jj], pn2 = L.tour[jj - 1], sn2 = L.tour[jj + 1];
     const float dm = L.dtw[jj - 1], d0 = L.dtw[jj];
     const bool before = i > jj;
     const float rm = before ? dm : d0;
-    const float add = rowB[n2] + rowB[before ? pn2 : sn2];
+    const float add = rget(rowB, n2) + rget(rowB, before ? pn2 : sn2);
     val = j < S - 1 && j != i && n2 != 0;
     const float gain = -(a1 + b1 + rm) + (add + c1);
     ok = val && gain < -kEps;
@@ -627,13 +640,14 @@ __device__ __forceinline__ void relocate_eval(const DevProblem &p, const LsLds &
     }
 }
 
+template <typename RT>
 __device__ __forceinline__ int relocate_scan(const DevProblem &p, const LsLds &L, int S, int i,
-                                             const float *rowP, const float *rowB, int lane,
+                                             const RT *rowP, const RT *rowB, int lane,
                                              int &evc)
 {
     const int n1 = L.tour[i];
     const int r1 = L.ridx[i], sn1 = L.tour[i + 1];
-    const float a1 = L.dtw[i - 1], b1 = L.dtw[i], c1 = rowP[sn1];
+    const float a1 = L.dtw[i - 1], b1 = L.dtw[i], c1 = rget(rowP, sn1);
     const int dn1 = L.dem[n1];
     int cnt = 0;
     for (int jb = 1; jb < S - 1; jb += kScanUnroll * kLanes) {
@@ -657,8 +671,8 @@ __device__ __forceinline__ int relocate_scan(const DevProblem &p, const LsLds &L
     return -1;
 }
 
-template <int W>
-__device__ __forceinline__ void ls_relocate(const DevProblem &p, LsLds &L, const RowRing &rr,
+template <int W, typename RT>
+__device__ __forceinline__ void ls_relocate(const DevProblem &p, LsLds &L, const RowRing<RT> &rr,
                                             bool pf_on, int S, int tid, long long &ev, int &moves)
 {
     const int lane = tid & 63, w = tid >> 6;
@@ -668,9 +682,8 @@ __device__ __forceinline__ void ls_relocate(const DevProblem &p, LsLds &L, const
     while (i0 < S - 1 && done < cap) {
         // rows of positions i0-1 .. i0+W-1: d(pred, .) gives c1, d(n1, .) the gains
         window_move(win, i0 - 1);
-        rows_ensure<W>(p, L, rr, win, min(W + 1, S - i0), tid);
-        RowPrefetch<W> pf;
-        prefetch_issue<W>(p, L, pf_on, win, pf, W + 1, S - 1, tid);
+        rows_ensure<W, RT>(p, L, rr, win, min(W + 1, S - i0), tid);
+        const unsigned pfw = prefetch_issue<W, RT>(p, L, rr, pf_on, win, W + 1, S - 1, tid);
         const int i = i0 + w;
         int j = -1, evc = 0;
         if (i < S - 1 && L.tour[i] != 0 && L.rc[L.ridx[i]] >= 2)
@@ -678,7 +691,7 @@ __device__ __forceinline__ void ls_relocate(const DevProblem &p, LsLds &L, const
         int *res = L.res + W * parity, *rese = L.res + LsW<W>::res_ev + W * parity;
         parity ^= 1;
         if (lane == 0) { res[w] = j; rese[w] = evc; }
-        prefetch_store<W>(p, rr, win, pf, W + 1, tid);
+        prefetch_wait(win, pfw, W + 1);
         __syncthreads();
         // the sequential scan evaluates the waves' positions in order and
         // stops at the first accepted move
@@ -698,14 +711,14 @@ __device__ __forceinline__ void ls_relocate(const DevProblem &p, LsLds &L, const
         const int n1 = L.tour[is];
         const int lo = is < js ? is : js, hi = is < js ? js : is;
         if (tid == 0) {
-            const float *rowP = rr.at(is - 1), *rowB = rr.at(is);
+            const RT *rowP = rr.at(is - 1), *rowB = rr.at(is);
             const int r1 = L.ridx[is], sn1 = L.tour[is + 1];
             const int n2 = L.tour[js], pn2 = L.tour[js - 1], sn2 = L.tour[js + 1], r2 = L.ridx[js];
-            const float a1 = L.dtw[is - 1], b1 = L.dtw[is], c1 = rowP[sn1];
+            const float a1 = L.dtw[is - 1], b1 = L.dtw[is], c1 = rget(rowP, sn1);
             const int dn1 = L.dem[n1];
             float rm, add;
-            if (is > js) { rm = L.dtw[js - 1]; add = rowB[n2] + rowB[pn2]; }
-            else { rm = L.dtw[js]; add = rowB[n2] + rowB[sn2]; }
+            if (is > js) { rm = L.dtw[js - 1]; add = rget(rowB, n2) + rget(rowB, pn2); }
+            else { rm = L.dtw[js]; add = rget(rowB, n2) + rget(rowB, sn2); }
             const float gain = -(a1 + b1 + rm) + (add + c1);
             if (r1 != r2) {
                 L.rl[r1] -= dn1;
@@ -739,7 +752,7 @@ __device__ __forceinline__ void ls_relocate(const DevProblem &p, LsLds &L, const
 
 // ------------------------------------------------------------ kernel ----
 
-template <int W>
+template <int W, typename RT>
 __global__ __launch_bounds__(W * 64) void k_local_search(DevBatch bt, int flags, int ring, int prefetch)
 {
     constexpr int NT = LsW<W>::threads;
@@ -748,9 +761,9 @@ __global__ __launch_bounds__(W * 64) void k_local_search(DevBatch bt, int flags,
     const DevProblem p = bt.probs[inst];
     const DevState s = bt.states[inst];
     const int a = blockIdx.x - bt.ant_base[inst], tid = threadIdx.x;
-    LsLds L = ls_bind(p.n, ring, smem);
-    const RowRing rr = {L.rows, ring - 1, p.n};
-    const bool pf_on = prefetch && p.n <= LsW<W>::prefetch_max_n;
+    LsLds L = ls_bind(p.n, ring, (int)sizeof(RT), smem);
+    const RowRing<RT> rr = {reinterpret_cast<RT *>(L.rows), ring - 1, row_stride<RT>(p.n)};
+    const bool pf_on = prefetch != 0;
     const int S = s.sizes[a];
     int *gt = s.tours + (size_t)a * p.T;
     for (int q = tid; q < S; q += NT) L.tour[q] = gt[q];
@@ -773,13 +786,13 @@ __global__ __launch_bounds__(W * 64) void k_local_search(DevBatch bt, int flags,
         ls_edges<W>(p, L, 0, S - 2, tid);
         __syncthreads();
         ls_routes<W>(p, L, S, tid, true);
-        ls_swap<W>(p, L, rr, pf_on, S, tid, ev, moves);
+        ls_swap<W, RT>(p, L, rr, pf_on, S, tid, ev, moves);
     }
     if (flags & kLsRelocate) {
         ls_edges<W>(p, L, 0, S - 2, tid);
         __syncthreads();
         ls_routes<W>(p, L, S, tid, true);
-        ls_relocate<W>(p, L, rr, pf_on, S, tid, ev, moves);
+        ls_relocate<W, RT>(p, L, rr, pf_on, S, tid, ev, moves);
     }
     ls_edges<W>(p, L, 0, S - 2, tid);
     __syncthreads();
@@ -824,13 +837,15 @@ hipError_t launch_local_search(const DevBatch &b, int flags, const LsShape &sh, 
 {
     if (!flags) return hipSuccess;
     if (!ls_shape_valid(sh)) return hipErrorInvalidValue;
-    const size_t lds = local_search_lds_bytes(b.max_n, sh.ring);
-    if (sh.waves == 8)
-        hipLaunchKernelGGL(k_local_search<8>, dim3(b.ants), dim3(8 * kLanes), lds, st, b, flags, sh.ring,
-                           sh.prefetch);
-    else
-        hipLaunchKernelGGL(k_local_search<4>, dim3(b.ants), dim3(4 * kLanes), lds, st, b, flags, sh.ring,
-                           sh.prefetch);
+    const size_t lds = local_search_lds_bytes(b.max_n, sh.ring, sh.row16);
+#define KLS(W, RT) hipLaunchKernelGGL((k_local_search<W, RT>), dim3(b.ants), dim3(W * kLanes), lds, st, b, \
+                                      flags, sh.ring, sh.prefetch)
+    if (sh.waves == 8) {
+        if (sh.row16) KLS(8, unsigned short); else KLS(8, float);
+    } else {
+        if (sh.row16) KLS(4, unsigned short); else KLS(4, float);
+    }
+#undef KLS
     return hipGetLastError();
 }
 

@@ -171,17 +171,19 @@ int aco_profile_get(aco_ctx *ctx, const char *name, double *total_ms, int64_t *l
  * whose launch geometry is fixed in g_aco.cpp:141-177,279-318) ----
  * Local-search kernel shape: waves per ant (4 or 8), rows in the LDS
  * distance-row ring (8, 16 or 32; >= waves + 2) and prefetch of the next
- * round's rows (needs ring >= 2 waves + 2; applies to instances with
- * n <= 256 waves).  Construction: construct_apb > 0 stages the candidate
+ * round's rows (needs ring >= 2 waves + 2).  Construction: construct_apb > 0 stages the candidate
  * table in LDS and runs that many ants per block (1, 2 or 4; candidate
  * roulette without a route-length limit, ants per instance divisible by
- * it); 0 reads candidate rows from L2/HBM.  Every valid shape gives
+ * it); 0 reads candidate rows from L2/HBM.  ls_row16 keeps the
+ * local-search rows as 16-bit integers (exact; only when every distance is
+ * an integer in [0, 65535], e.g. VRPLIB nint).  Every valid shape gives
  * identical results. */
 typedef struct aco_launch_shape {
     int32_t ls_waves;
     int32_t ls_ring;
     int32_t ls_prefetch;
     int32_t construct_apb;
+    int32_t ls_row16;
 } aco_launch_shape;
 
 typedef struct aco_tune_result {

@@ -11,7 +11,8 @@ from helpers import log
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = [(4, 16, 1, 0), (4, 16, 0, 1), (4, 8, 0, 2), (8, 32, 1, 4), (8, 16, 0, 0), (8, 16, 0, 2)]
+SHAPES = [(4, 16, 1, 0, 0), (4, 16, 0, 1, 0), (4, 8, 0, 2, 0), (8, 32, 1, 4, 0), (8, 16, 0, 0, 0),
+          (8, 16, 0, 2, 1), (4, 16, 1, 0, 1), (8, 32, 1, 2, 1)]
 
 
 def _small():
@@ -44,7 +45,7 @@ def test_every_shape_gives_identical_results(which):
     ref, ref_ev = _ls_once(insts, SHAPES[0])
     assert ref_ev[0] > 0
     for sh in SHAPES[1:]:
-        if which == "large" and sh[:3] == (8, 32, 1):
+        if which == "large" and sh[:3] == (8, 32, 1) and not sh[4]:
             # 32 rows of 1101 floats do not fit in 160 KB of LDS
             with pytest.raises(ca.AcoError):
                 _ls_once(insts, sh)
@@ -74,12 +75,17 @@ def test_staged_construction_with_fallbacks(rng):
 def test_invalid_shapes_rejected():
     inst = I.uniform_100(1)
     with ca.GpuACO(inst, n_ants=8, ls_flags=7) as g:   # I-Roulette: no staged construction
-        assert g.launch_shape() == (4, 16, 1, 0)
+        assert g.launch_shape() == (4, 16, 1, 0, 0)
         for bad in [(2, 16, 0), (4, 4, 0), (4, 8, 1), (8, 8, 0), (8, 16, 1), (4, 12, 0),
                     (4, 16, 1, 1), (4, 16, 1, 3)]:
             with pytest.raises(ca.AcoError):
                 g.set_launch_shape(*bad)
-        assert g.launch_shape() == (4, 16, 1, 0)
+        assert g.launch_shape() == (4, 16, 1, 0, 0)
+        g.set_launch_shape(4, 16, 1, 0, 1)       # nint distances: 16-bit rows allowed
+    fl = I.uniform_100(1, dist_mode=ca.DIST_FLOAT)
+    with ca.GpuACO(fl, n_ants=8, ls_flags=7) as g:
+        with pytest.raises(ca.AcoError):         # float distances: no 16-bit rows
+            g.set_launch_shape(4, 16, 1, 0, 1)
     with ca.GpuACO(inst, n_ants=6, k=16, selection=ca.SEL_KNN, ls_flags=7) as g:
         g.set_launch_shape(4, 16, 1, 2)
         with pytest.raises(ca.AcoError):     # 6 ants are not a multiple of 4
@@ -98,7 +104,8 @@ def test_autotune_keeps_the_trajectory():
                 log(f"autotune: {res} -> {g.launch_shape()}")
                 assert any(ms > 0 for _, st, ms in res if st == 0)
                 assert any(ms > 0 for _, st, ms in res if st == 1)
-                assert g.launch_shape()[:3] in [sh[:3] for sh, st, ms in res if st == 1 and ms > 0]
+                assert (g.launch_shape()[:3] + g.launch_shape()[4:]) in \
+                    [sh[:3] + sh[4:] for sh, st, ms in res if st == 1 and ms > 0]
                 assert g.launch_shape()[3] in [sh[3] for sh, st, ms in res if st == 0 and ms > 0]
             g.run(4)
             runs.append([(g.ants(b), g.pheromone(b), g.bsf_records(b)) for b in range(len(insts))])
