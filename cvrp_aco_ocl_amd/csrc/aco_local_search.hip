@@ -40,7 +40,7 @@ template <int W> struct LsW {
 
 struct LsLds {
     int *tour, *tmp, *rl, *dem, *res;
-    float *dtw, *rd, *rows;
+    float *dtw, *tmpf, *rd, *rows;
     short *pos, *rof, *ridx, *zpos, *rc;
     unsigned char *dlb;
 };
@@ -48,7 +48,7 @@ struct LsLds {
 __host__ __device__ constexpr size_t ls_align16(size_t x) { return (x + 15) & ~size_t(15); }
 
 struct LsOffsets {
-    size_t tour, tmp, dtw, rl, rd, dem, rows, res, pos, rof, ridx, zpos, rc, dlb, total;
+    size_t tour, tmp, dtw, tmpf, rl, rd, dem, rows, res, pos, rof, ridx, zpos, rc, dlb, total;
 };
 
 __host__ __device__ inline LsOffsets ls_offsets(int n, int ring, int rowbytes)
@@ -59,6 +59,7 @@ __host__ __device__ inline LsOffsets ls_offsets(int n, int ring, int rowbytes)
     o.tour = at; at = ls_align16(at + 4 * T);
     o.tmp = at;  at = ls_align16(at + 4 * T);
     o.dtw = at;  at = ls_align16(at + 4 * T);
+    o.tmpf = at; at = ls_align16(at + 4 * T);
     o.rl = at;   at = ls_align16(at + 4 * N);
     o.rd = at;   at = ls_align16(at + 4 * N);
     o.dem = at;  at = ls_align16(at + 4 * N);
@@ -79,6 +80,7 @@ __device__ __forceinline__ LsLds ls_bind(int n, int ring, int rowbytes, unsigned
     const LsOffsets o = ls_offsets(n, ring, rowbytes);
     LsLds L;
     L.tour = (int *)(b + o.tour); L.tmp = (int *)(b + o.tmp); L.dtw = (float *)(b + o.dtw);
+    L.tmpf = (float *)(b + o.tmpf);
     L.rl = (int *)(b + o.rl); L.rd = (float *)(b + o.rd); L.dem = (int *)(b + o.dem);
     L.rows = (float *)(b + o.rows); L.res = (int *)(b + o.res);
     L.pos = (short *)(b + o.pos); L.rof = (short *)(b + o.rof); L.ridx = (short *)(b + o.ridx);
@@ -415,12 +417,15 @@ __device__ __forceinline__ void rows_ensure(const DevProblem &p, const LsLds &L,
                                             RowWindow &w, int cnt, int tid)
 {
     bool loaded = false;
+    const int lane = tid & 63;
+    // the window's nodes in one LDS read, picked out per row with readlane
+    const int tv = lane < cnt ? L.tour[w.base + lane] : 0;
 #pragma unroll
     for (int q = 0; q < W + 2; ++q) {
         if (q >= cnt) break;
         if ((w.mask >> q) & 1u) continue;
         const int pos = w.base + q;
-        row_copy_lds<W, RT>(p, L.tour[pos], rr.at(pos), tid);
+        row_copy_lds<W, RT>(p, __builtin_amdgcn_readlane(tv, q), rr.at(pos), tid);
         loaded = true;
     }
     w.mask |= (1u << cnt) - 1u;
@@ -441,12 +446,14 @@ __device__ __forceinline__ unsigned prefetch_issue(const DevProblem &p, const Ls
 {
     unsigned want = 0;
     if (!on) return 0;
+    const int lane = tid & 63, pos0 = w.base + off;
+    const int tv = (lane < W && pos0 + lane < S) ? L.tour[pos0 + lane] : 0;
 #pragma unroll
     for (int q = 0; q < W; ++q) {
-        const int pos = w.base + off + q;
+        const int pos = pos0 + q;
         if (pos >= S || ((w.mask >> (off + q)) & 1u)) continue;
         want |= 1u << q;
-        row_copy_lds<W, RT>(p, L.tour[pos], rr.at(pos), tid);
+        row_copy_lds<W, RT>(p, __builtin_amdgcn_readlane(tv, q), rr.at(pos), tid);
     }
     return want;
 }
@@ -598,10 +605,18 @@ __device__ __forceinline__ void ls_swap(const DevProblem &p, LsLds &L, const Row
             L.tour[js] = n1;
             L.rl[r1] = load1; L.rl[r2] = load2;
             L.rd[r1] = dist1; L.rd[r2] = dist2;
-            L.dtw[is - 1] = Dd(p, L.tour[is - 1], L.tour[is]);
-            L.dtw[is] = Dd(p, L.tour[is], L.tour[is + 1]);
-            L.dtw[js - 1] = Dd(p, L.tour[js - 1], L.tour[js]);
-            L.dtw[js] = Dd(p, L.tour[js], L.tour[js + 1]);
+            // the four new edges, from the rows of pn1, n1, sn1 (symmetric
+            // distances: the same values as the distance matrix)
+            if (js == is + 1) {
+                L.dtw[is - 1] = rget(rowA, n2);   // (pn1, n2)
+                L.dtw[is] = rget(rowB, n2);       // (n2, n1)
+                L.dtw[js] = rget(rowB, sn2);      // (n1, sn2)
+            } else {
+                L.dtw[is - 1] = rget(rowA, n2);   // (pn1, n2)
+                L.dtw[is] = rget(rowC, n2);       // (n2, sn1)
+                L.dtw[js - 1] = rget(rowB, pn2);  // (pn2, n1)
+                L.dtw[js] = rget(rowB, sn2);      // (n1, sn2)
+            }
         }
         __syncthreads();
         window_clear(win, is, is);
@@ -731,18 +746,42 @@ __device__ __forceinline__ void ls_relocate(const DevProblem &p, LsLds &L, const
                 L.rd[r1] = L.rd[r1] + gain;
             }
         }
-        // shift: copy [lo, hi] aside, then write it back moved by one
-        for (int q = lo + tid; q <= hi; q += LsW<W>::threads) L.tmp[q] = L.tour[q];
-        __syncthreads();
+        // the three new edges (removal point, both sides of the insertion),
+        // read by every thread before the shift
+        const RT *rowPs = rr.at(is - 1), *rowBs = rr.at(is);
+        const int sn1s = L.tour[is + 1], n2s = L.tour[js];
+        const int xs = is > js ? L.tour[js - 1] : L.tour[js + 1];
+        const float c1s = rget(rowPs, sn1s), e_n2 = rget(rowBs, n2s), e_x = rget(rowBs, xs);
+        // shift: positions [lo, hi] move by one towards is, n1 lands at js;
+        // route indices and edge weights move with them (no reload)
         for (int q = lo + tid; q <= hi; q += LsW<W>::threads) {
-            int v;
-            if (is < js) v = q == js ? n1 : L.tmp[q + 1];
-            else v = q == js ? n1 : L.tmp[q - 1];
-            L.tour[q] = v;
+            L.tmp[q] = L.tour[q] | ((int)L.ridx[q] << 16);
+            L.tmpf[q] = L.dtw[q];
         }
         __syncthreads();
-        ls_edges<W>(p, L, lo - 1, hi, tid);
-        ls_routes<W>(p, L, S, tid, false);  // refreshes ridx (and pos/zpos)
+        for (int q = lo + tid; q <= hi; q += LsW<W>::threads) {
+            int v, r;
+            if (q == js) {
+                v = n1;
+                r = L.tmp[js] >> 16;          // n1 joins the route of the node at js
+            } else {
+                const int t = L.tmp[is < js ? q + 1 : q - 1];
+                v = t & 0xFFFF;
+                r = t >> 16;
+            }
+            L.tour[q] = v;
+            L.ridx[q] = (short)r;
+        }
+        for (int q = lo - 1 + tid; q <= hi; q += LsW<W>::threads) {
+            float d;
+            if (is < js) {
+                d = q == is - 1 ? c1s : q == js - 1 ? e_n2 : q == js ? e_x : L.tmpf[q + 1];
+            } else {
+                d = q == js - 1 ? e_x : q == js ? e_n2 : q == is ? c1s : L.tmpf[q - 1];
+            }
+            L.dtw[q] = d;
+        }
+        __syncthreads();
         window_clear(win, lo, hi);
         i0 = is;
         ++done;
