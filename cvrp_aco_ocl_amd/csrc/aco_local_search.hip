@@ -189,24 +189,26 @@ __device__ __forceinline__ void group_sync()
 // kept (row positions are distinct, so the group minimum is unique).
 // Returns true with h1..h4 set on an improving move (same in all lanes of
 // the group).
-// Candidate q of a scan: in the row (rp < nn_ls), inside the radius, and
-// for the predecessor variant not adjacent; x is s2 or p2.
+// Candidate q of a scan (q may lie past rend: the caller masks it).  Loads
+// are unconditional (clamped, in-bounds indices) so a lane's candidates
+// have all their loads in flight at once.  ok: in the row (rp < nn_ls),
+// inside the radius and, for the predecessor variant, not adjacent; gain as
+// the reference computes it; x is s2 or p2.
 template <bool SUCC>
-__device__ __forceinline__ bool two_opt_candidate(const DevProblem &p, const LsLds &L, int rend,
-                                                  int q, int n1, int nb1, float radius,
-                                                  const unsigned short *rowpos, int &rp, int &n2,
-                                                  int &x, float &dn)
+__device__ __forceinline__ void two_opt_cand(const DevProblem &p, const LsLds &L, int rend, int pos1,
+                                             int q, int n1, int nb1, float radius,
+                                             const unsigned short *rowpos, bool &ok, float &gain,
+                                             int &rp, int &n2, int &x)
 {
-    n2 = L.tour[q];
+    const int qq = q <= rend ? q : rend;
+    n2 = L.tour[qq];
+    x = SUCC ? (qq == rend ? 0 : L.tour[qq + 1]) : L.tour[qq - 1];
+    const float ew = SUCC ? L.dtw[qq] : L.dtw[qq - 1];
     rp = rowpos[n2];
-    dn = Dd(p, n1, n2);
-    if (rp >= p.nn_ls || !(radius - dn > kEps)) return false;
-    if (SUCC) {
-        x = q == rend ? 0 : L.tour[q + 1];
-        return true;
-    }
-    x = L.tour[q - 1];
-    return !(x == n1 || nb1 == n2);
+    const float dn = Dd(p, n1, n2);
+    const float dx = Dd(p, nb1, x);
+    ok = q <= rend && q != pos1 && rp < p.nn_ls && radius - dn > kEps && (SUCC || !(x == n1 || nb1 == n2));
+    gain = -radius + dn + dx - ew;
 }
 
 template <bool SUCC>
@@ -218,16 +220,19 @@ __device__ __forceinline__ bool two_opt_scan(const DevProblem &p, const LsLds &L
     const unsigned short *rowpos = p.nnpos + (size_t)n1 * p.n;
     u64 best = ~0ull;
     int cnt = 0;
-    for (int q = rbeg + 1 + gl; q <= rend; q += kOptGroup) {
-        if (q == pos1) continue;
-        int rp, n2, x;
-        float dn;
-        if (!two_opt_candidate<SUCC>(p, L, rend, q, n1, nb1, radius, rowpos, rp, n2, x, dn)) continue;
-        ++cnt;
-        const float gain = SUCC ? -radius + dn + Dd(p, nb1, x) - L.dtw[q]
-                                : -radius + dn + Dd(p, nb1, x) - L.dtw[q - 1];
-        if (gain < -kEps) {
-            const u64 key = ((u64)rp << 32) | ((u64)n2 << 16) | (u64)x;
+    for (int q = rbeg + 1 + gl; q <= rend; q += 2 * kOptGroup) {
+        bool ok0, ok1;
+        float g0, g1;
+        int rp0, rp1, a0, a1, x0, x1;
+        two_opt_cand<SUCC>(p, L, rend, pos1, q, n1, nb1, radius, rowpos, ok0, g0, rp0, a0, x0);
+        two_opt_cand<SUCC>(p, L, rend, pos1, q + kOptGroup, n1, nb1, radius, rowpos, ok1, g1, rp1, a1, x1);
+        cnt += ok0 + ok1;
+        if (ok0 && g0 < -kEps) {
+            const u64 key = ((u64)rp0 << 32) | ((u64)a0 << 16) | (u64)x0;
+            best = key < best ? key : best;
+        }
+        if (ok1 && g1 < -kEps) {
+            const u64 key = ((u64)rp1 << 32) | ((u64)a1 << 16) | (u64)x1;
             best = key < best ? key : best;
         }
     }
@@ -240,12 +245,13 @@ __device__ __forceinline__ bool two_opt_scan(const DevProblem &p, const LsLds &L
     // the candidates up to that row position
     const int rps = (int)(best >> 32);
     cnt = 0;
-    for (int q = rbeg + 1 + gl; q <= rend; q += kOptGroup) {
-        if (q == pos1) continue;
-        int rp, n2, x;
-        float dn;
-        if (two_opt_candidate<SUCC>(p, L, rend, q, n1, nb1, radius, rowpos, rp, n2, x, dn) && rp <= rps)
-            ++cnt;
+    for (int q = rbeg + 1 + gl; q <= rend; q += 2 * kOptGroup) {
+        bool ok0, ok1;
+        float g0, g1;
+        int rp0, rp1, a0, a1, x0, x1;
+        two_opt_cand<SUCC>(p, L, rend, pos1, q, n1, nb1, radius, rowpos, ok0, g0, rp0, a0, x0);
+        two_opt_cand<SUCC>(p, L, rend, pos1, q + kOptGroup, n1, nb1, radius, rowpos, ok1, g1, rp1, a1, x1);
+        cnt += (ok0 && rp0 <= rps) + (ok1 && rp1 <= rps);
     }
     evals += cnt;
     const int b2 = (int)((best >> 16) & 0xFFFF), bx = (int)(best & 0xFFFF);
