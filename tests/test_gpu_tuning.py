@@ -115,3 +115,13 @@ def test_autotune_keeps_the_trajectory():
             np.testing.assert_array_equal(x, y)
         np.testing.assert_array_equal(p0[0], p1[0])
         assert [r[:2] for r in r0] == [r[:2] for r in r1]
+
+
+def test_candidate_k_tuning_by_quality():
+    from cvrp_aco_ocl_amd.tuning import tune_candidate_k
+    insts = I.batch(8, count=4)
+    best, res = tune_candidate_k(insts, ks=(8, 16), seconds=0.2, n_ants=16)
+    log(f"k tuning: {res} -> {best}")
+    assert set(res) == {8, 16} and best in res
+    assert all(v["iterations"] > 0 and v["mean_cost"] > 0 for v in res.values())
+    assert res[best]["mean_cost"] == min(v["mean_cost"] for v in res.values())
