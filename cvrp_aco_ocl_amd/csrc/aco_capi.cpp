@@ -895,7 +895,7 @@ int aco_set_launch_shape(aco_ctx *c, const aco_launch_shape *in)
     if (sh.row16 && !c->int16_rows)
         return set_error(ACO_ERR_ARG, "16-bit rows need every distance to be an integer in [0, 65535]");
     if (!ls_shape_valid(sh))
-        return set_error(ACO_ERR_ARG, "invalid shape: waves 4|8, ring 8|16|32 >= waves+2, "
+        return set_error(ACO_ERR_ARG, "invalid shape: waves 4|8|12|16, ring 8|16|32 >= waves+2, "
                                       "prefetch needs ring >= 2*waves+2");
     if (!construct_staged_ok(c->bt, c->prm.selection, c->prm.k, c->m, sh.construct_apb))
         return set_error(ACO_ERR_ARG, "construct_apb must be 0, or 1|2|4 dividing the ants per instance "
@@ -915,8 +915,10 @@ int aco_autotune(aco_ctx *c, int32_t reps, aco_tune_result *results, int32_t cap
     if (rc) return rc;
     if (reps < 1 || reps > 1000) return set_error(ACO_ERR_ARG, "reps must be in [1, 1000]");
     static const int kApb[] = {0, 1, 2, 4};
-    static const LsShape kLs[] = {{4, 16, 1, 0, 0}, {4, 16, 0, 0, 0}, {4, 8, 0, 0, 0}, {8, 32, 1, 0, 0},
-                                  {8, 16, 0, 0, 0}, {4, 16, 1, 0, 1}, {8, 32, 1, 0, 1}, {8, 16, 0, 0, 1}};
+    static const LsShape kLs[] = {{4, 16, 1, 0, 0},  {4, 16, 0, 0, 0},  {4, 8, 0, 0, 0},   {8, 32, 1, 0, 0},
+                                  {8, 16, 0, 0, 0},  {4, 16, 1, 0, 1},  {8, 32, 1, 0, 1},  {8, 16, 0, 0, 1},
+                                  {12, 16, 0, 0, 0}, {12, 16, 0, 0, 1}, {12, 32, 1, 0, 1}, {16, 32, 0, 0, 0},
+                                  {16, 32, 0, 0, 1}};
     constexpr int kNc = (int)(sizeof(kApb) / sizeof(kApb[0]));
     constexpr int kNl = (int)(sizeof(kLs) / sizeof(kLs[0]));
     if (count) *count = kNc + kNl;

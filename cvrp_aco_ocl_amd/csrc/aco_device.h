@@ -17,6 +17,14 @@ namespace cvrp {
 
 constexpr int kLanes = 64;             // wavefront = one ant
 
+// Index of the calling wave in its block, as a wave-uniform value (the
+// compiler cannot prove threadIdx.x >> 6 uniform by itself; without this,
+// loops and branches on it are compiled as divergent).
+__device__ __forceinline__ int wave_id()
+{
+    return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+}
+
 // ---- lane exchange on the DPP network (no LDS traffic) ----
 // DPP controls: quad_perm [1,0,3,2] (lane ^ 1), quad_perm [2,3,0,1]
 // (lane ^ 2), row_shl / row_shr by 4 and 8 (lane +- 4, +- 8 within a row of
@@ -43,6 +51,29 @@ __device__ __forceinline__ unsigned long long dpp_xor8_u64(unsigned long long v,
 {
     const unsigned long long a = dpp_u64<kDppShl8>(v), b = dpp_u64<kDppShr8>(v);
     return (lane & 8) ? b : a;
+}
+
+// Whole-wave shifts by one lane (DPP wave_shr:1 / wave_shl:1): lane l gets
+// lane l-1's (l+1's) value; lane 0 (lane 63), which has no source, gets fill.
+__device__ __forceinline__ int wave_prev(int v, int fill)
+{
+    return __builtin_amdgcn_update_dpp(fill, v, 0x138, 0xF, 0xF, false);
+}
+__device__ __forceinline__ int wave_next(int v, int fill)
+{
+    return __builtin_amdgcn_update_dpp(fill, v, 0x130, 0xF, 0xF, false);
+}
+__device__ __forceinline__ float wave_prev(float v, float fill)
+{
+    return __int_as_float(wave_prev(__float_as_int(v), __float_as_int(fill)));
+}
+__device__ __forceinline__ float wave_next(float v, float fill)
+{
+    return __int_as_float(wave_next(__float_as_int(v), __float_as_int(fill)));
+}
+__device__ __forceinline__ float readlane_f32(float v, int l)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
 
 __device__ __forceinline__ unsigned long long readlane_u64(unsigned long long v, int l)

@@ -364,7 +364,7 @@ template <int RNG, int A>
 __global__ __launch_bounds__(64 * A) void k_construct_knn_lds(DevBatch bt)
 {
     extern __shared__ __align__(16) unsigned char smem[];
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wv = wave_id(), lane = threadIdx.x & 63;
     const int inst = bt.ant_inst[blockIdx.x * A];       // all A ants in one instance
     const DevProblem p = bt.probs[inst];
     const DevState s = bt.states[inst];

@@ -19,23 +19,32 @@
 // discarded.  Distance rows of the positions around i0 sit in an LDS ring
 // and the next round's rows are fetched while the current round scans.
 //
-// Launch shape (LsShape): W, the ring size and prefetch on/off; every shape
-// gives the same tours and the same evaluated-move counts.
+// Launch shape (LsShape): W (4, 8, 12 or 16), the ring size, prefetch on/off
+// and the row type; every shape gives the same tours and the same
+// evaluated-move counts.
 #include "aco_device.h"
 
 namespace cvrp {
 
 using u64 = unsigned long long;
 
-// LDS words of L.res for W waves: [0,W) and [W,2W) per-wave scan results
-// (accepted j) of even and odd rounds, [2W,4W) the moves each wave's scan
-// evaluated up to its result (same parity split), 4W the route count.
-constexpr int kResWords = 64;
+// LDS words of L.res for W waves (W <= 16): [0,W) and [W,2W) per-wave scan
+// results (accepted j) of even and odd rounds, [2W,4W) the moves each wave's
+// scan evaluated up to its result (same parity split), 4W the route count,
+// 4W+1 the lowest wave of the current round with a move (round key).
+constexpr int kResWords = 80;
+
+// Candidates of one scan trip, and the padding of the LDS tour and edge
+// arrays past the tour's end that lets a trip read without clamping.
+constexpr int kScanUnroll = 4;
+constexpr int kScanPad = kScanUnroll * 64 + 4;
 
 template <int W> struct LsW {
+    static_assert(W == 4 || W == 8 || W == 12 || W == 16, "4, 8, 12 or 16 waves per ant");
     static constexpr int threads = W * kLanes;
     static constexpr int res_ev = 2 * W;
     static constexpr int res_r = 4 * W;
+    static constexpr int res_first = 4 * W + 1;
 };
 
 struct LsLds {
@@ -56,9 +65,9 @@ __host__ __device__ inline LsOffsets ls_offsets(int n, int ring, int rowbytes)
     const size_t T = 2 * (size_t)n, N = (size_t)n;
     LsOffsets o;
     size_t at = 0;
-    o.tour = at; at = ls_align16(at + 4 * T);
+    o.tour = at; at = ls_align16(at + 4 * (T + kScanPad));
     o.tmp = at;  at = ls_align16(at + 4 * T);
-    o.dtw = at;  at = ls_align16(at + 4 * T);
+    o.dtw = at;  at = ls_align16(at + 4 * (T + kScanPad));
     o.tmpf = at; at = ls_align16(at + 4 * T);
     o.rl = at;   at = ls_align16(at + 4 * N);
     o.rd = at;   at = ls_align16(at + 4 * N);
@@ -381,21 +390,36 @@ __device__ __forceinline__ const unsigned short *row_src(const DevProblem &p, in
     return p.dist16 + (size_t)v * p.P;
 }
 
-// Copy distance row v into LDS with direct global->LDS loads (one dword per
-// lane per instruction, no registers; completes with vmcnt, so readers wait
-// for vmcnt(0) and a barrier).  Rows of both types start dword aligned and
-// the copy may run up to one padding element past n (within P and within
-// the slot stride).
-template <int W, typename RT>
-__device__ __forceinline__ void row_copy_lds(const DevProblem &p, int v, RT *dst, int tid)
+// Copy distance row v into LDS with direct global->LDS loads issued by one
+// wave (one dword per lane per instruction, no registers; completes with
+// vmcnt, so readers wait for vmcnt(0) and a barrier).  Rows of both types
+// start dword aligned and the copy may run up to one padding element past
+// n (within P and within the slot stride).
+template <typename RT>
+__device__ __forceinline__ void row_copy_wave(const DevProblem &p, int v, RT *dst, int lane)
 {
     const int nd = (p.n * (int)sizeof(RT) + 3) >> 2;
     const unsigned *src = reinterpret_cast<const unsigned *>(row_src(p, v, (RT *)nullptr));
     unsigned *d = reinterpret_cast<unsigned *>(dst);
-    const int lane = tid & 63;
-    for (int c = tid; c < nd; c += LsW<W>::threads)
-        __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(src + c),
-                                         (__attribute__((address_space(3))) void *)(d + (c - lane)), 4, 0, 0);
+    for (int c = 0; c < nd; c += kLanes) {
+        if (c + lane < nd)
+            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(src + c + lane),
+                                             (__attribute__((address_space(3))) void *)(d + c), 4, 0, 0);
+    }
+}
+
+// Copy the rows of the window offsets in `need` (bit q: position base + q):
+// the k-th of them is issued by wave k mod W.
+template <int W, typename RT>
+__device__ __forceinline__ void rows_issue(const DevProblem &p, const LsLds &L, const RowRing<RT> &rr,
+                                           int base, unsigned need, int wave, int lane)
+{
+    int k = 0;
+    for (unsigned m = need; m; m &= m - 1u, ++k) {
+        if (k % W != wave) continue;
+        const int pos = base + __builtin_ctz(m);
+        row_copy_wave<RT>(p, __builtin_amdgcn_readfirstlane(L.tour[pos]), rr.at(pos), lane);
+    }
 }
 
 // Start the window at position base (never moves backwards within a scan).
@@ -416,26 +440,16 @@ __device__ __forceinline__ void window_clear(RowWindow &w, int lo, int hi)
     w.mask &= ~(span << a);
 }
 
-// Make the rows of window offsets [0, cnt) present (cnt <= W+2): all
-// threads load the missing ones; a barrier follows when any was missing.
+// Make the rows of window offsets [0, cnt) present (cnt <= W+2), spread
+// over the waves; a barrier follows when any was missing.
 template <int W, typename RT>
 __device__ __forceinline__ void rows_ensure(const DevProblem &p, const LsLds &L, const RowRing<RT> &rr,
                                             RowWindow &w, int cnt, int tid)
 {
-    bool loaded = false;
-    const int lane = tid & 63;
-    // the window's nodes in one LDS read, picked out per row with readlane
-    const int tv = lane < cnt ? L.tour[w.base + lane] : 0;
-#pragma unroll
-    for (int q = 0; q < W + 2; ++q) {
-        if (q >= cnt) break;
-        if ((w.mask >> q) & 1u) continue;
-        const int pos = w.base + q;
-        row_copy_lds<W, RT>(p, __builtin_amdgcn_readlane(tv, q), rr.at(pos), tid);
-        loaded = true;
-    }
+    const unsigned need = ((1u << cnt) - 1u) & ~w.mask;
     w.mask |= (1u << cnt) - 1u;
-    if (loaded) {
+    if (need) {
+        rows_issue<W, RT>(p, L, rr, w.base, need, wave_id(), tid & 63);
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
     }
@@ -450,17 +464,12 @@ template <int W, typename RT>
 __device__ __forceinline__ unsigned prefetch_issue(const DevProblem &p, const LsLds &L, const RowRing<RT> &rr,
                                                    bool on, const RowWindow &w, int off, int S, int tid)
 {
-    unsigned want = 0;
     if (!on) return 0;
-    const int lane = tid & 63, pos0 = w.base + off;
-    const int tv = (lane < W && pos0 + lane < S) ? L.tour[pos0 + lane] : 0;
-#pragma unroll
-    for (int q = 0; q < W; ++q) {
-        const int pos = pos0 + q;
-        if (pos >= S || ((w.mask >> (off + q)) & 1u)) continue;
-        want |= 1u << q;
-        row_copy_lds<W, RT>(p, __builtin_amdgcn_readlane(tv, q), rr.at(pos), tid);
-    }
+    const int pos0 = w.base + off;
+    const int avail = S - pos0 < W ? S - pos0 : W;
+    if (avail <= 0) return 0;
+    const unsigned want = ((1u << avail) - 1u) & ~(w.mask >> off);
+    if (want) rows_issue<W, RT>(p, L, rr, pos0, want, wave_id(), tid & 63);
     return want;
 }
 
@@ -471,27 +480,73 @@ __device__ __forceinline__ void prefetch_wait(RowWindow &w, unsigned want, int o
     w.mask |= want << off;
 }
 
-// -------------------------------------------------------------- swap ----
+// After a round's barrier: the first wave (in position order) whose scan
+// accepted a move, that move's j (js), and in e the moves the sequential
+// scan evaluates up to and including that wave's position (all W positions
+// when no wave moved).  One LDS read per lane; the rest is uniform.
+template <int W>
+__device__ __forceinline__ int round_pick(const int *res, const int *rese, int lane, int &js, int &e)
+{
+    const int rv = lane < W ? res[lane] : -1;
+    const int re = lane < W ? rese[lane] : 0;
+    const u64 mk = __ballot(rv >= 0);
+    const int wstar = mk ? (int)__ffsll((unsigned long long)mk) - 1 : -1;
+    const int last = wstar < 0 ? W - 1 : wstar;
+    int sum = 0;
+#pragma unroll
+    for (int q = 0; q < W; ++q)
+        if (q <= last) sum += __builtin_amdgcn_readlane(re, q);
+    e = sum;
+    js = wstar < 0 ? -1 : __builtin_amdgcn_readlane(rv, wstar);
+    return wstar;
+}
 
-// Candidate chunks evaluated per loop trip of a scan (independent loads in
-// flight; the first accepted j in scan order is still the one returned).
-constexpr int kScanUnroll = 4;
+// -------------------------------------------------------------- swap ----
 
 // Mask of lanes 0..f.
 __device__ __forceinline__ u64 lanes_upto(int f) { return (2ull << f) - 1ull; }
 
+// Early stop of speculative scans: a wave that accepts a move raises the
+// round key (round << 5 | 31 - wave) with an LDS max.  A wave whose key is
+// exceeded knows that a lower wave of this round has a move, so its own
+// result will be discarded, and stops scanning.  Keys of earlier rounds are
+// smaller, and no wave reaches the next round before every wave has passed
+// this round's barrier, so no reset is needed.
+struct ScanStop {
+    const unsigned *key;
+    unsigned mine;
+};
+__device__ __forceinline__ unsigned round_key(unsigned round, int w) { return (round << 5) | (unsigned)(31 - w); }
+
+// Fresh read of the round key: a volatile load through an LDS pointer (a
+// volatile load through a generic pointer would become a flat load that
+// waits on the memory counter).
+__device__ __forceinline__ unsigned read_key(const unsigned *key)
+{
+    return *(volatile const __attribute__((address_space(3))) unsigned *)(key);
+}
+
+// A trip of a scan covers kScanUnroll chunks of 64 consecutive positions j.
+// Each lane loads the node and the edge weight at its own j once; the
+// values at j-1 and j+1 come from the neighbouring lanes (wave shifts), and
+// at the chunk edges from the neighbouring chunk or, at the trip's edges,
+// from the positions before and after the trip.
+struct TripEdge {
+    float d;    // dtw[jb - 1]
+    float v;    // row value at tour[jb - 1]
+};
+
 // Swap of position i with position j (localSearch.cpp:301-410): sets ok
 // when the move improves and is feasible, val when j is a customer
-// position (an evaluated move).
+// position (an evaluated move).  n2 = tour[j], pn2/sn2 its neighbours,
+// dm/d0 the edges (pn2, n2) and (n2, sn2), rBp/rBs = d(n1, pn2), d(n1, sn2).
 template <typename RT>
 __device__ __forceinline__ void swap_eval(const DevProblem &p, const LsLds &L, int S, int i, int j,
                                           int r1, int dn1, float a1, float b1, const RT *rowA,
-                                          const RT *rowB, const RT *rowC, bool &ok, bool &val)
+                                          const RT *rowC, int n2, float dm, float d0, float rBp,
+                                          float rBs, bool &ok, bool &val)
 {
-    const int jj = j < S - 1 ? j : S - 1;
-    const int n2 = L.tour[jj], pn2 = L.tour[jj - 1], sn2 = L.tour[jj < S - 1 ? jj + 1 : jj];
-    const float dm = L.dtw[jj - 1], d0 = L.dtw[jj];
-    const float rA = rget(rowA, n2), rC = rget(rowC, n2), rBp = rget(rowB, pn2), rBs = rget(rowB, sn2);
+    const float rA = rget(rowA, n2), rC = rget(rowC, n2);
     val = j < S && n2 != 0;
     float gain;
     if (j == i + 1)
@@ -500,7 +555,7 @@ __device__ __forceinline__ void swap_eval(const DevProblem &p, const LsLds &L, i
         gain = -(a1 + b1 + dm + d0) + (rA + rC + rBp + rBs);
     ok = val && gain < -kEps;
     if (ok) {
-        const int r2 = L.ridx[jj];
+        const int r2 = L.ridx[j];
         if (r1 != r2) {
             const int dn2 = L.dem[n2];
             const int load1 = L.rl[r1] - dn1 + dn2, load2 = L.rl[r2] - dn2 + dn1;
@@ -516,19 +571,40 @@ __device__ __forceinline__ void swap_eval(const DevProblem &p, const LsLds &L, i
 template <typename RT>
 __device__ __forceinline__ int swap_scan(const DevProblem &p, const LsLds &L, int S, int i,
                                          const RT *rowA, const RT *rowB, const RT *rowC,
-                                         int lane, int &evc)
+                                         int lane, const ScanStop &stop, int &evc)
 {
     const int n1 = L.tour[i];
     const int sn1 = L.tour[i + 1], r1 = L.ridx[i];
     const float a1 = rget(rowA, n1), b1 = rget(rowB, sn1);
     const int dn1 = L.dem[n1];
     int cnt = 0;
+    TripEdge pe = {L.dtw[i], rget(rowB, n1)};   // position jb - 1 = i
     for (int jb = i + 1; jb < S; jb += kScanUnroll * kLanes) {
+        const unsigned key = read_key(stop.key);
+        int t[kScanUnroll];
+        float d[kScanUnroll], v[kScanUnroll];
+        // past the tour's end the padding reads as depots and zero edges;
+        // every position j <= S-2 that can be accepted sees its true
+        // neighbours
+#pragma unroll
+        for (int u = 0; u < kScanUnroll; ++u) {
+            t[u] = L.tour[jb + u * kLanes + lane];
+            d[u] = L.dtw[jb + u * kLanes + lane];
+        }
+        const int ta = L.tour[jb + kScanUnroll * kLanes];
+#pragma unroll
+        for (int u = 0; u < kScanUnroll; ++u) v[u] = rget(rowB, t[u]);
+        const float va = rget(rowB, ta);
         bool ok[kScanUnroll], val[kScanUnroll];
 #pragma unroll
-        for (int u = 0; u < kScanUnroll; ++u)
-            swap_eval(p, L, S, i, jb + u * kLanes + lane, r1, dn1, a1, b1, rowA, rowB, rowC, ok[u],
+        for (int u = 0; u < kScanUnroll; ++u) {
+            const float dp = u == 0 ? pe.d : readlane_f32(d[u - 1], kLanes - 1);
+            const float vp = u == 0 ? pe.v : readlane_f32(v[u - 1], kLanes - 1);
+            const float vn = u == kScanUnroll - 1 ? va : readlane_f32(v[u + 1], 0);
+            swap_eval(p, L, S, i, jb + u * kLanes + lane, r1, dn1, a1, b1, rowA, rowC, t[u],
+                      wave_prev(d[u], dp), d[u], wave_prev(v[u], vp), wave_next(v[u], vn), ok[u],
                       val[u]);
+        }
 #pragma unroll
         for (int u = 0; u < kScanUnroll; ++u) {
             const u64 mk = __ballot(ok[u]), mv = __ballot(val[u]);
@@ -539,6 +615,9 @@ __device__ __forceinline__ int swap_scan(const DevProblem &p, const LsLds &L, in
             }
             cnt += __popcll(mv);
         }
+        if (key > stop.mine) break;
+        pe.d = readlane_f32(d[kScanUnroll - 1], kLanes - 1);
+        pe.v = readlane_f32(v[kScanUnroll - 1], kLanes - 1);
     }
     evc = cnt;
     return -1;
@@ -546,9 +625,9 @@ __device__ __forceinline__ int swap_scan(const DevProblem &p, const LsLds &L, in
 
 template <int W, typename RT>
 __device__ __forceinline__ void ls_swap(const DevProblem &p, LsLds &L, const RowRing<RT> &rr, bool pf_on,
-                                        int S, int tid, long long &ev, int &moves)
+                                        int S, int tid, unsigned &round, long long &ev, int &moves)
 {
-    const int lane = tid & 63, w = tid >> 6;
+    const int lane = tid & 63, w = wave_id();
     const int cap = kMaxMovesFactor * p.n;
     int done = 0, i0 = 1, parity = 0;
     RowWindow win = {0, 0u};
@@ -559,8 +638,11 @@ __device__ __forceinline__ void ls_swap(const DevProblem &p, LsLds &L, const Row
         const unsigned pfw = prefetch_issue<W, RT>(p, L, rr, pf_on, win, W + 2, S, tid);
         const int i = i0 + w;
         int j = -1, evc = 0;
+        const ScanStop stop = {reinterpret_cast<const unsigned *>(L.res + LsW<W>::res_first),
+                               round_key(++round, w)};
         if (i < S && L.tour[i] != 0)
-            j = swap_scan(p, L, S, i, rr.at(i - 1), rr.at(i), rr.at(i + 1), lane, evc);
+            j = swap_scan(p, L, S, i, rr.at(i - 1), rr.at(i), rr.at(i + 1), lane, stop, evc);
+        if (j >= 0 && lane == 0) atomicMax(reinterpret_cast<unsigned *>(L.res + LsW<W>::res_first), stop.mine);
         int *res = L.res + W * parity, *rese = L.res + LsW<W>::res_ev + W * parity;
         parity ^= 1;
         if (lane == 0) { res[w] = j; rese[w] = evc; }
@@ -568,11 +650,8 @@ __device__ __forceinline__ void ls_swap(const DevProblem &p, LsLds &L, const Row
         __syncthreads();
         // the sequential scan evaluates the waves' positions in order and
         // stops at the first accepted move
-        int wstar = -1, e = 0;
-        for (int q = 0; q < W; ++q) {
-            e += rese[q];
-            if (res[q] >= 0) { wstar = q; break; }
-        }
+        int js, e;
+        const int wstar = round_pick<W>(res, rese, lane, js, e);
         if (tid == 0) ev += e;
         if (wstar < 0) {
             // no barrier: the next round writes the other result buffer, and
@@ -580,7 +659,7 @@ __device__ __forceinline__ void ls_swap(const DevProblem &p, LsLds &L, const Row
             i0 += W;
             continue;
         }
-        const int is = i0 + wstar, js = res[wstar];
+        const int is = i0 + wstar;
         if (tid == 0) {
             const RT *rowA = rr.at(is - 1), *rowB = rr.at(is), *rowC = rr.at(is + 1);
             const int n1 = L.tour[is], sn1 = L.tour[is + 1], r1 = L.ridx[is];
@@ -637,22 +716,21 @@ __device__ __forceinline__ void ls_swap(const DevProblem &p, LsLds &L, const Row
 
 // Relocate of position i next to position j (neighbourSearch.cpp:280-293,
 // move.cpp:32-51): ok when improving and feasible, val when evaluated.
-template <typename RT>
+// n2 = tour[j]; dm/d0 the edges (pn2, n2), (n2, sn2); vp/vn = d(n1, pn2),
+// d(n1, sn2); v = d(n1, n2).
 __device__ __forceinline__ void relocate_eval(const DevProblem &p, const LsLds &L, int S, int i,
                                               int j, int r1, int dn1, float a1, float b1, float c1,
-                                              const RT *rowB, bool &ok, bool &val)
+                                              int n2, float dm, float d0, float v, float vp,
+                                              float vn, bool &ok, bool &val)
 {
-    const int jj = j < S - 1 ? j : S - 2;
-    const int n2 = L.tour[jj], pn2 = L.tour[jj - 1], sn2 = L.tour[jj + 1];
-    const float dm = L.dtw[jj - 1], d0 = L.dtw[jj];
-    const bool before = i > jj;
+    const bool before = i > j;
     const float rm = before ? dm : d0;
-    const float add = rget(rowB, n2) + rget(rowB, before ? pn2 : sn2);
+    const float add = v + (before ? vp : vn);
     val = j < S - 1 && j != i && n2 != 0;
     const float gain = -(a1 + b1 + rm) + (add + c1);
     ok = val && gain < -kEps;
     if (ok) {
-        const int r2 = L.ridx[jj];
+        const int r2 = L.ridx[j];
         if (r1 != r2) {
             const float dist2 = ((L.rd[r2] + p.svc) - rm) + add;
             const float dist1 = ((L.rd[r1] - p.svc) - (a1 + b1)) + c1;
@@ -664,19 +742,40 @@ __device__ __forceinline__ void relocate_eval(const DevProblem &p, const LsLds &
 template <typename RT>
 __device__ __forceinline__ int relocate_scan(const DevProblem &p, const LsLds &L, int S, int i,
                                              const RT *rowP, const RT *rowB, int lane,
-                                             int &evc)
+                                             const ScanStop &stop, int &evc)
 {
     const int n1 = L.tour[i];
     const int r1 = L.ridx[i], sn1 = L.tour[i + 1];
     const float a1 = L.dtw[i - 1], b1 = L.dtw[i], c1 = rget(rowP, sn1);
     const int dn1 = L.dem[n1];
     int cnt = 0;
+    TripEdge pe = {L.dtw[0], rget(rowB, L.tour[0])};   // position jb - 1 = 0
     for (int jb = 1; jb < S - 1; jb += kScanUnroll * kLanes) {
+        const unsigned key = read_key(stop.key);
+        int t[kScanUnroll];
+        float d[kScanUnroll], v[kScanUnroll];
+        // past the tour's end the padding reads as depots and zero edges;
+        // every position j <= S-2 that can be accepted sees its true
+        // neighbours
+#pragma unroll
+        for (int u = 0; u < kScanUnroll; ++u) {
+            t[u] = L.tour[jb + u * kLanes + lane];
+            d[u] = L.dtw[jb + u * kLanes + lane];
+        }
+        const int ta = L.tour[jb + kScanUnroll * kLanes];
+#pragma unroll
+        for (int u = 0; u < kScanUnroll; ++u) v[u] = rget(rowB, t[u]);
+        const float va = rget(rowB, ta);
         bool ok[kScanUnroll], val[kScanUnroll];
 #pragma unroll
-        for (int u = 0; u < kScanUnroll; ++u)
-            relocate_eval(p, L, S, i, jb + u * kLanes + lane, r1, dn1, a1, b1, c1, rowB, ok[u],
-                          val[u]);
+        for (int u = 0; u < kScanUnroll; ++u) {
+            const float dp = u == 0 ? pe.d : readlane_f32(d[u - 1], kLanes - 1);
+            const float vp = u == 0 ? pe.v : readlane_f32(v[u - 1], kLanes - 1);
+            const float vn = u == kScanUnroll - 1 ? va : readlane_f32(v[u + 1], 0);
+            relocate_eval(p, L, S, i, jb + u * kLanes + lane, r1, dn1, a1, b1, c1, t[u],
+                          wave_prev(d[u], dp), d[u], v[u], wave_prev(v[u], vp), wave_next(v[u], vn),
+                          ok[u], val[u]);
+        }
 #pragma unroll
         for (int u = 0; u < kScanUnroll; ++u) {
             const u64 mk = __ballot(ok[u]), mv = __ballot(val[u]);
@@ -687,6 +786,9 @@ __device__ __forceinline__ int relocate_scan(const DevProblem &p, const LsLds &L
             }
             cnt += __popcll(mv);
         }
+        if (key > stop.mine) break;
+        pe.d = readlane_f32(d[kScanUnroll - 1], kLanes - 1);
+        pe.v = readlane_f32(v[kScanUnroll - 1], kLanes - 1);
     }
     evc = cnt;
     return -1;
@@ -694,9 +796,10 @@ __device__ __forceinline__ int relocate_scan(const DevProblem &p, const LsLds &L
 
 template <int W, typename RT>
 __device__ __forceinline__ void ls_relocate(const DevProblem &p, LsLds &L, const RowRing<RT> &rr,
-                                            bool pf_on, int S, int tid, long long &ev, int &moves)
+                                            bool pf_on, int S, int tid, unsigned &round, long long &ev,
+                                            int &moves)
 {
-    const int lane = tid & 63, w = tid >> 6;
+    const int lane = tid & 63, w = wave_id();
     const int cap = kMaxMovesFactor * p.n;
     int done = 0, i0 = 1, parity = 0;
     RowWindow win = {0, 0u};
@@ -707,8 +810,11 @@ __device__ __forceinline__ void ls_relocate(const DevProblem &p, LsLds &L, const
         const unsigned pfw = prefetch_issue<W, RT>(p, L, rr, pf_on, win, W + 1, S - 1, tid);
         const int i = i0 + w;
         int j = -1, evc = 0;
+        const ScanStop stop = {reinterpret_cast<const unsigned *>(L.res + LsW<W>::res_first),
+                               round_key(++round, w)};
         if (i < S - 1 && L.tour[i] != 0 && L.rc[L.ridx[i]] >= 2)
-            j = relocate_scan(p, L, S, i, rr.at(i - 1), rr.at(i), lane, evc);
+            j = relocate_scan(p, L, S, i, rr.at(i - 1), rr.at(i), lane, stop, evc);
+        if (j >= 0 && lane == 0) atomicMax(reinterpret_cast<unsigned *>(L.res + LsW<W>::res_first), stop.mine);
         int *res = L.res + W * parity, *rese = L.res + LsW<W>::res_ev + W * parity;
         parity ^= 1;
         if (lane == 0) { res[w] = j; rese[w] = evc; }
@@ -716,11 +822,8 @@ __device__ __forceinline__ void ls_relocate(const DevProblem &p, LsLds &L, const
         __syncthreads();
         // the sequential scan evaluates the waves' positions in order and
         // stops at the first accepted move
-        int wstar = -1, e = 0;
-        for (int q = 0; q < W; ++q) {
-            e += rese[q];
-            if (res[q] >= 0) { wstar = q; break; }
-        }
+        int js, e;
+        const int wstar = round_pick<W>(res, rese, lane, js, e);
         if (tid == 0) ev += e;
         if (wstar < 0) {
             // no barrier: the next round writes the other result buffer, and
@@ -728,7 +831,7 @@ __device__ __forceinline__ void ls_relocate(const DevProblem &p, LsLds &L, const
             i0 += W;
             continue;
         }
-        const int is = i0 + wstar, js = res[wstar];
+        const int is = i0 + wstar;
         const int n1 = L.tour[is];
         const int lo = is < js ? is : js, hi = is < js ? js : is;
         if (tid == 0) {
@@ -812,13 +915,21 @@ __global__ __launch_bounds__(W * 64) void k_local_search(DevBatch bt, int flags,
     const int S = s.sizes[a];
     int *gt = s.tours + (size_t)a * p.T;
     for (int q = tid; q < S; q += NT) L.tour[q] = gt[q];
+    // padding past the tour's end: depots and zero edges (never written
+    // again; moves stay inside [1, S-2])
+    for (int q = tid; q < kScanPad; q += NT) {
+        L.tour[S + q] = 0;
+        L.dtw[S - 1 + q] = 0.0f;
+    }
     for (int v = tid; v < p.n; v += NT) {
         L.dlb[v] = 0;
         L.dem[v] = p.dem[v];
     }
+    if (tid == 0) L.res[LsW<W>::res_first] = 0;
     __syncthreads();
     long long ev = 0;
     int moves = 0;
+    unsigned round = 0;
     if (flags & kLs2Opt) {
         ls_edges<W>(p, L, 0, S - 2, tid);
         const int R = ls_routes<W>(p, L, S, tid, false);
@@ -831,13 +942,13 @@ __global__ __launch_bounds__(W * 64) void k_local_search(DevBatch bt, int flags,
         ls_edges<W>(p, L, 0, S - 2, tid);
         __syncthreads();
         ls_routes<W>(p, L, S, tid, true);
-        ls_swap<W, RT>(p, L, rr, pf_on, S, tid, ev, moves);
+        ls_swap<W, RT>(p, L, rr, pf_on, S, tid, round, ev, moves);
     }
     if (flags & kLsRelocate) {
         ls_edges<W>(p, L, 0, S - 2, tid);
         __syncthreads();
         ls_routes<W>(p, L, S, tid, true);
-        ls_relocate<W, RT>(p, L, rr, pf_on, S, tid, ev, moves);
+        ls_relocate<W, RT>(p, L, rr, pf_on, S, tid, round, ev, moves);
     }
     ls_edges<W>(p, L, 0, S - 2, tid);
     __syncthreads();
@@ -871,7 +982,7 @@ __global__ __launch_bounds__(W * 64) void k_local_search(DevBatch bt, int flags,
 
 bool ls_shape_valid(const LsShape &sh)
 {
-    if (sh.waves != 4 && sh.waves != 8) return false;
+    if (sh.waves != 4 && sh.waves != 8 && sh.waves != 12 && sh.waves != 16) return false;
     if (sh.ring != 8 && sh.ring != 16 && sh.ring != 32) return false;
     if (sh.ring < sh.waves + 2) return false;
     if (sh.prefetch && sh.ring < 2 * sh.waves + 2) return false;
@@ -885,10 +996,11 @@ hipError_t launch_local_search(const DevBatch &b, int flags, const LsShape &sh, 
     const size_t lds = local_search_lds_bytes(b.max_n, sh.ring, sh.row16);
 #define KLS(W, RT) hipLaunchKernelGGL((k_local_search<W, RT>), dim3(b.ants), dim3(W * kLanes), lds, st, b, \
                                       flags, sh.ring, sh.prefetch)
-    if (sh.waves == 8) {
-        if (sh.row16) KLS(8, unsigned short); else KLS(8, float);
-    } else {
-        if (sh.row16) KLS(4, unsigned short); else KLS(4, float);
+    switch (sh.waves) {
+    case 16: if (sh.row16) KLS(16, unsigned short); else KLS(16, float); break;
+    case 12: if (sh.row16) KLS(12, unsigned short); else KLS(12, float); break;
+    case 8: if (sh.row16) KLS(8, unsigned short); else KLS(8, float); break;
+    default: if (sh.row16) KLS(4, unsigned short); else KLS(4, float); break;
     }
 #undef KLS
     return hipGetLastError();

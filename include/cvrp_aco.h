@@ -169,7 +169,7 @@ int aco_profile_get(aco_ctx *ctx, const char *name, double *total_ms, int64_t *l
 
 /* ---- launch shapes and the autotuner (no counterpart in the reference,
  * whose launch geometry is fixed in g_aco.cpp:141-177,279-318) ----
- * Local-search kernel shape: waves per ant (4 or 8), rows in the LDS
+ * Local-search kernel shape: waves per ant (4, 8, 12 or 16), rows in the LDS
  * distance-row ring (8, 16 or 32; >= waves + 2) and prefetch of the next
  * round's rows (needs ring >= 2 waves + 2).  Construction: construct_apb > 0 stages the candidate
  * table in LDS and runs that many ants per block (1, 2 or 4; candidate

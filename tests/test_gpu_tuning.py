@@ -12,7 +12,8 @@ from helpers import log
 pytestmark = pytest.mark.gpu
 
 SHAPES = [(4, 16, 1, 0, 0), (4, 16, 0, 1, 0), (4, 8, 0, 2, 0), (8, 32, 1, 4, 0), (8, 16, 0, 0, 0),
-          (8, 16, 0, 2, 1), (4, 16, 1, 0, 1), (8, 32, 1, 2, 1)]
+          (8, 16, 0, 2, 1), (4, 16, 1, 0, 1), (8, 32, 1, 2, 1), (12, 16, 0, 0, 0), (12, 32, 1, 4, 1),
+          (16, 32, 0, 0, 1), (16, 32, 0, 2, 0)]
 
 
 def _small():
@@ -45,7 +46,7 @@ def test_every_shape_gives_identical_results(which):
     ref, ref_ev = _ls_once(insts, SHAPES[0])
     assert ref_ev[0] > 0
     for sh in SHAPES[1:]:
-        if which == "large" and sh[:3] == (8, 32, 1) and not sh[4]:
+        if which == "large" and sh[1] == 32 and not sh[4]:
             # 32 rows of 1101 floats do not fit in 160 KB of LDS
             with pytest.raises(ca.AcoError):
                 _ls_once(insts, sh)
@@ -77,6 +78,7 @@ def test_invalid_shapes_rejected():
     with ca.GpuACO(inst, n_ants=8, ls_flags=7) as g:   # I-Roulette: no staged construction
         assert g.launch_shape() == (4, 16, 1, 0, 0)
         for bad in [(2, 16, 0), (4, 4, 0), (4, 8, 1), (8, 8, 0), (8, 16, 1), (4, 12, 0),
+                    (6, 16, 0), (12, 8, 0), (12, 16, 1), (16, 16, 0), (16, 32, 1),
                     (4, 16, 1, 1), (4, 16, 1, 3)]:
             with pytest.raises(ca.AcoError):
                 g.set_launch_shape(*bad)
