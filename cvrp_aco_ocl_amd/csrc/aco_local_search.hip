@@ -536,88 +536,102 @@ struct TripEdge {
     float v;    // row value at tour[jb - 1]
 };
 
-// Swap of position i with position j (localSearch.cpp:301-410): sets ok
-// when the move improves and is feasible, val when j is a customer
-// position (an evaluated move).  n2 = tour[j], pn2/sn2 its neighbours,
-// dm/d0 the edges (pn2, n2) and (n2, sn2), rBp/rBs = d(n1, pn2), d(n1, sn2).
-template <typename RT>
-__device__ __forceinline__ void swap_eval(const DevProblem &p, const LsLds &L, int S, int i, int j,
-                                          int r1, int dn1, float a1, float b1, const RT *rowA,
-                                          const RT *rowC, int n2, float dm, float d0, float rBp,
-                                          float rBs, bool &ok, bool &val)
+// Swap of position i with position j (localSearch.cpp:301-410).  n2 =
+// tour[j], dm/d0 the edges (pn2, n2) and (n2, sn2), rA/rC = d(pn1, n2),
+// d(sn1, n2), rBp/rBs = d(n1, pn2), d(n1, sn2).  The gain is the
+// reference's expression for adjacent and general positions.
+__device__ __forceinline__ float swap_gain(int i, int j, float a1, float b1, float rA, float rC,
+                                           float dm, float d0, float rBp, float rBs)
 {
-    const float rA = rget(rowA, n2), rC = rget(rowC, n2);
-    val = j < S && n2 != 0;
-    float gain;
-    if (j == i + 1)
-        gain = -(a1 + d0) + (rA + rBs);
-    else
-        gain = -(a1 + b1 + dm + d0) + (rA + rC + rBp + rBs);
-    ok = val && gain < -kEps;
-    if (ok) {
-        const int r2 = L.ridx[j];
-        if (r1 != r2) {
-            const int dn2 = L.dem[n2];
-            const int load1 = L.rl[r1] - dn1 + dn2, load2 = L.rl[r2] - dn2 + dn1;
-            const float dist1 = (L.rd[r1] - (a1 + b1)) + (rA + rC);
-            const float dist2 = (L.rd[r2] - (dm + d0)) + (rBp + rBs);
-            ok = !(load1 > p.Q || load2 > p.Q || dist1 > p.D || dist2 > p.D);
-        }
-    }
+    if (j == i + 1) return -(a1 + d0) + (rA + rBs);
+    return -(a1 + b1 + dm + d0) + (rA + rC + rBp + rBs);
+}
+
+// Loads and route lengths stay feasible (checked only for improving moves
+// between two routes).
+__device__ __forceinline__ bool swap_feasible(const DevProblem &p, const LsLds &L, int j, int r1, int dn1,
+                                              float a1, float b1, int n2, float rA, float rC, float dm,
+                                              float d0, float rBp, float rBs)
+{
+    const int r2 = L.ridx[j];
+    if (r1 == r2) return true;
+    const int dn2 = L.dem[n2];
+    const int load1 = L.rl[r1] - dn1 + dn2, load2 = L.rl[r2] - dn2 + dn1;
+    const float dist1 = (L.rd[r1] - (a1 + b1)) + (rA + rC);
+    const float dist2 = (L.rd[r2] - (dm + d0)) + (rBp + rBs);
+    return !(load1 > p.Q || load2 > p.Q || dist1 > p.D || dist2 > p.D);
 }
 
 // Position i against the current tour; returns the first accepted j or -1
-// and in evc the moves the sequential scan evaluates up to it.
+// and in evc the moves the sequential scan evaluates up to it.  A trip's
+// common case (no improving candidate) takes no branch per chunk; the
+// feasibility checks run only for chunks with an improving candidate.
 template <typename RT>
 __device__ __forceinline__ int swap_scan(const DevProblem &p, const LsLds &L, int S, int i,
                                          const RT *rowA, const RT *rowB, const RT *rowC,
                                          int lane, const ScanStop &stop, int &evc)
 {
+    constexpr int U = kScanUnroll;
     const int n1 = L.tour[i];
     const int sn1 = L.tour[i + 1], r1 = L.ridx[i];
     const float a1 = rget(rowA, n1), b1 = rget(rowB, sn1);
     const int dn1 = L.dem[n1];
     int cnt = 0;
     TripEdge pe = {L.dtw[i], rget(rowB, n1)};   // position jb - 1 = i
-    for (int jb = i + 1; jb < S; jb += kScanUnroll * kLanes) {
+    for (int jb = i + 1; jb < S; jb += U * kLanes) {
         const unsigned key = read_key(stop.key);
-        int t[kScanUnroll];
-        float d[kScanUnroll], v[kScanUnroll];
+        int t[U];
+        float d[U], v[U];
         // past the tour's end the padding reads as depots and zero edges;
         // every position j <= S-2 that can be accepted sees its true
         // neighbours
 #pragma unroll
-        for (int u = 0; u < kScanUnroll; ++u) {
+        for (int u = 0; u < U; ++u) {
             t[u] = L.tour[jb + u * kLanes + lane];
             d[u] = L.dtw[jb + u * kLanes + lane];
         }
-        const int ta = L.tour[jb + kScanUnroll * kLanes];
+        const int ta = L.tour[jb + U * kLanes];
 #pragma unroll
-        for (int u = 0; u < kScanUnroll; ++u) v[u] = rget(rowB, t[u]);
+        for (int u = 0; u < U; ++u) v[u] = rget(rowB, t[u]);
         const float va = rget(rowB, ta);
-        bool ok[kScanUnroll], val[kScanUnroll];
+        float rA[U], rC[U], dm[U], rBp[U], rBs[U];
+        bool imp[U], val[U], any = false;
 #pragma unroll
-        for (int u = 0; u < kScanUnroll; ++u) {
+        for (int u = 0; u < U; ++u) {
+            const int j = jb + u * kLanes + lane;
             const float dp = u == 0 ? pe.d : readlane_f32(d[u - 1], kLanes - 1);
             const float vp = u == 0 ? pe.v : readlane_f32(v[u - 1], kLanes - 1);
-            const float vn = u == kScanUnroll - 1 ? va : readlane_f32(v[u + 1], 0);
-            swap_eval(p, L, S, i, jb + u * kLanes + lane, r1, dn1, a1, b1, rowA, rowC, t[u],
-                      wave_prev(d[u], dp), d[u], wave_prev(v[u], vp), wave_next(v[u], vn), ok[u],
-                      val[u]);
+            const float vn = u == U - 1 ? va : readlane_f32(v[u + 1], 0);
+            rA[u] = rget(rowA, t[u]);
+            rC[u] = rget(rowC, t[u]);
+            dm[u] = wave_prev(d[u], dp);
+            rBp[u] = wave_prev(v[u], vp);
+            rBs[u] = wave_next(v[u], vn);
+            val[u] = j < S && t[u] != 0;
+            imp[u] = val[u] && swap_gain(i, j, a1, b1, rA[u], rC[u], dm[u], d[u], rBp[u], rBs[u]) < -kEps;
+            any = any || imp[u];
         }
+        if (__ballot(any)) {
 #pragma unroll
-        for (int u = 0; u < kScanUnroll; ++u) {
-            const u64 mk = __ballot(ok[u]), mv = __ballot(val[u]);
-            if (mk) {
-                const int f = __ffsll(mk) - 1;
-                evc = cnt + __popcll(mv & lanes_upto(f));
-                return jb + u * kLanes + f;
+            for (int u = 0; u < U; ++u) {
+                const int j = jb + u * kLanes + lane;
+                const bool ok = imp[u] && swap_feasible(p, L, j, r1, dn1, a1, b1, t[u], rA[u], rC[u], dm[u],
+                                                        d[u], rBp[u], rBs[u]);
+                const u64 mk = __ballot(ok), mv = __ballot(val[u]);
+                if (mk) {
+                    const int f = __ffsll(mk) - 1;
+                    evc = cnt + __popcll(mv & lanes_upto(f));
+                    return jb + u * kLanes + f;
+                }
+                cnt += __popcll(mv);
             }
-            cnt += __popcll(mv);
+        } else {
+#pragma unroll
+            for (int u = 0; u < U; ++u) cnt += __popcll(__ballot(val[u]));
         }
         if (key > stop.mine) break;
-        pe.d = readlane_f32(d[kScanUnroll - 1], kLanes - 1);
-        pe.v = readlane_f32(v[kScanUnroll - 1], kLanes - 1);
+        pe.d = readlane_f32(d[U - 1], kLanes - 1);
+        pe.v = readlane_f32(v[U - 1], kLanes - 1);
     }
     evc = cnt;
     return -1;
@@ -715,28 +729,24 @@ __device__ __forceinline__ void ls_swap(const DevProblem &p, LsLds &L, const Row
 // ---------------------------------------------------------- relocate ----
 
 // Relocate of position i next to position j (neighbourSearch.cpp:280-293,
-// move.cpp:32-51): ok when improving and feasible, val when evaluated.
-// n2 = tour[j]; dm/d0 the edges (pn2, n2), (n2, sn2); vp/vn = d(n1, pn2),
-// d(n1, sn2); v = d(n1, n2).
-__device__ __forceinline__ void relocate_eval(const DevProblem &p, const LsLds &L, int S, int i,
-                                              int j, int r1, int dn1, float a1, float b1, float c1,
-                                              int n2, float dm, float d0, float v, float vp,
-                                              float vn, bool &ok, bool &val)
+// move.cpp:32-51).  n2 = tour[j]; rm is the edge the insertion removes,
+// (pn2, n2) before j or (n2, sn2) after it; add = d(n1, n2) + d(n1, x)
+// with x = pn2 or sn2.
+__device__ __forceinline__ float relocate_gain(float a1, float b1, float c1, float rm, float add)
 {
-    const bool before = i > j;
-    const float rm = before ? dm : d0;
-    const float add = v + (before ? vp : vn);
-    val = j < S - 1 && j != i && n2 != 0;
-    const float gain = -(a1 + b1 + rm) + (add + c1);
-    ok = val && gain < -kEps;
-    if (ok) {
-        const int r2 = L.ridx[j];
-        if (r1 != r2) {
-            const float dist2 = ((L.rd[r2] + p.svc) - rm) + add;
-            const float dist1 = ((L.rd[r1] - p.svc) - (a1 + b1)) + c1;
-            ok = !(L.rl[r2] + dn1 > p.Q) && !(dist2 > p.D) && !(dist1 > p.D);
-        }
-    }
+    return -(a1 + b1 + rm) + (add + c1);
+}
+
+// Load and route lengths stay feasible (checked only for improving moves
+// between two routes).
+__device__ __forceinline__ bool relocate_feasible(const DevProblem &p, const LsLds &L, int j, int r1, int dn1,
+                                                  float a1, float b1, float c1, float rm, float add)
+{
+    const int r2 = L.ridx[j];
+    if (r1 == r2) return true;
+    const float dist2 = ((L.rd[r2] + p.svc) - rm) + add;
+    const float dist1 = ((L.rd[r1] - p.svc) - (a1 + b1)) + c1;
+    return !(L.rl[r2] + dn1 > p.Q) && !(dist2 > p.D) && !(dist1 > p.D);
 }
 
 template <typename RT>
@@ -744,51 +754,66 @@ __device__ __forceinline__ int relocate_scan(const DevProblem &p, const LsLds &L
                                              const RT *rowP, const RT *rowB, int lane,
                                              const ScanStop &stop, int &evc)
 {
+    constexpr int U = kScanUnroll;
     const int n1 = L.tour[i];
     const int r1 = L.ridx[i], sn1 = L.tour[i + 1];
     const float a1 = L.dtw[i - 1], b1 = L.dtw[i], c1 = rget(rowP, sn1);
     const int dn1 = L.dem[n1];
     int cnt = 0;
     TripEdge pe = {L.dtw[0], rget(rowB, L.tour[0])};   // position jb - 1 = 0
-    for (int jb = 1; jb < S - 1; jb += kScanUnroll * kLanes) {
+    for (int jb = 1; jb < S - 1; jb += U * kLanes) {
         const unsigned key = read_key(stop.key);
-        int t[kScanUnroll];
-        float d[kScanUnroll], v[kScanUnroll];
+        int t[U];
+        float d[U], v[U];
         // past the tour's end the padding reads as depots and zero edges;
         // every position j <= S-2 that can be accepted sees its true
         // neighbours
 #pragma unroll
-        for (int u = 0; u < kScanUnroll; ++u) {
+        for (int u = 0; u < U; ++u) {
             t[u] = L.tour[jb + u * kLanes + lane];
             d[u] = L.dtw[jb + u * kLanes + lane];
         }
-        const int ta = L.tour[jb + kScanUnroll * kLanes];
+        const int ta = L.tour[jb + U * kLanes];
 #pragma unroll
-        for (int u = 0; u < kScanUnroll; ++u) v[u] = rget(rowB, t[u]);
+        for (int u = 0; u < U; ++u) v[u] = rget(rowB, t[u]);
         const float va = rget(rowB, ta);
-        bool ok[kScanUnroll], val[kScanUnroll];
+        float rm[U], add[U];
+        bool imp[U], val[U], any = false;
 #pragma unroll
-        for (int u = 0; u < kScanUnroll; ++u) {
+        for (int u = 0; u < U; ++u) {
+            const int j = jb + u * kLanes + lane;
             const float dp = u == 0 ? pe.d : readlane_f32(d[u - 1], kLanes - 1);
             const float vp = u == 0 ? pe.v : readlane_f32(v[u - 1], kLanes - 1);
-            const float vn = u == kScanUnroll - 1 ? va : readlane_f32(v[u + 1], 0);
-            relocate_eval(p, L, S, i, jb + u * kLanes + lane, r1, dn1, a1, b1, c1, t[u],
-                          wave_prev(d[u], dp), d[u], v[u], wave_prev(v[u], vp), wave_next(v[u], vn),
-                          ok[u], val[u]);
+            const float vn = u == U - 1 ? va : readlane_f32(v[u + 1], 0);
+            // every lane takes part in the lane shifts; each lane then picks
+            const float dmu = wave_prev(d[u], dp), vpu = wave_prev(v[u], vp), vnu = wave_next(v[u], vn);
+            const bool before = i > j;
+            rm[u] = before ? dmu : d[u];
+            add[u] = v[u] + (before ? vpu : vnu);
+            val[u] = j < S - 1 && j != i && t[u] != 0;
+            imp[u] = val[u] && relocate_gain(a1, b1, c1, rm[u], add[u]) < -kEps;
+            any = any || imp[u];
         }
+        if (__ballot(any)) {
 #pragma unroll
-        for (int u = 0; u < kScanUnroll; ++u) {
-            const u64 mk = __ballot(ok[u]), mv = __ballot(val[u]);
-            if (mk) {
-                const int f = __ffsll(mk) - 1;
-                evc = cnt + __popcll(mv & lanes_upto(f));
-                return jb + u * kLanes + f;
+            for (int u = 0; u < U; ++u) {
+                const int j = jb + u * kLanes + lane;
+                const bool ok = imp[u] && relocate_feasible(p, L, j, r1, dn1, a1, b1, c1, rm[u], add[u]);
+                const u64 mk = __ballot(ok), mv = __ballot(val[u]);
+                if (mk) {
+                    const int f = __ffsll(mk) - 1;
+                    evc = cnt + __popcll(mv & lanes_upto(f));
+                    return jb + u * kLanes + f;
+                }
+                cnt += __popcll(mv);
             }
-            cnt += __popcll(mv);
+        } else {
+#pragma unroll
+            for (int u = 0; u < U; ++u) cnt += __popcll(__ballot(val[u]));
         }
         if (key > stop.mine) break;
-        pe.d = readlane_f32(d[kScanUnroll - 1], kLanes - 1);
-        pe.v = readlane_f32(v[kScanUnroll - 1], kLanes - 1);
+        pe.d = readlane_f32(d[U - 1], kLanes - 1);
+        pe.v = readlane_f32(v[U - 1], kLanes - 1);
     }
     evc = cnt;
     return -1;
