@@ -486,6 +486,9 @@ int aco_create_batch(const aco_problem *pbs, int32_t count, const aco_params *pr
         dp.Q = pb->capacity; dp.D = pb->max_dist; dp.svc = pb->service_time;
         dp.alpha = prm.alpha; dp.rho = prm.rho;
         dp.has_d = isfinite(pb->max_dist) ? 1 : 0;
+        dp.dem16 = 1;
+        for (int i = 0; i < n; ++i)
+            if (pb->demand[i] < 0 || pb->demand[i] > 65535) dp.dem16 = 0;
         dp.seed = I.seed;
         dp.tau0_nn = 0.0f;
         if (prm.tau0_rule == ACO_TAU0_NN_TOUR) {
@@ -505,6 +508,7 @@ int aco_create_batch(const aco_problem *pbs, int32_t count, const aco_params *pr
     bt.ants = count * m;
     bt.max_n = bt.max_P = bt.max_nk = 0;
     bt.has_d = 0;
+    bt.dem16 = 1;
     for (int b = 0; b < count; ++b) {
         hp[b] = c->inst[b].dp;
         hs[b] = c->inst[b].ds;
@@ -514,6 +518,7 @@ int aco_create_batch(const aco_problem *pbs, int32_t count, const aco_params *pr
         bt.max_P = std::max(bt.max_P, hp[b].P);
         bt.max_nk = std::max(bt.max_nk, hp[b].n * hp[b].k);
         bt.has_d |= hp[b].has_d;
+        bt.dem16 &= hp[b].dem16;
     }
     void *q = nullptr;
     const size_t tb = sizeof(DevProblem) * count + sizeof(DevState) * count + 4 * ant_inst.size() +

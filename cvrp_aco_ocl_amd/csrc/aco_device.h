@@ -124,6 +124,7 @@ struct DevProblem {
     float D, svc, alpha, rho;
     float tau0_nn;            // 1/(n * L_nn) for the nearest-neighbour tau0 rule
     int has_d;                // max_dist is finite
+    int dem16;                // every demand is in [0, 65535]
     unsigned long long seed;  // Philox key of this instance
     const float *dist;        // n*P
     const float *d0;          // n, distance to the depot (column 0)
@@ -170,6 +171,7 @@ struct DevBatch {
     int B, ants;
     int max_n, max_P, max_nk;
     int has_d;                // some instance has a route-length limit
+    int dem16;                // every instance's demands fit 16 bits
 };
 
 enum { kTau0Arg = 0, kTau0Bsf = 1, kTau0Nn = 2 };

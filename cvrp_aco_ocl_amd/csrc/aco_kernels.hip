@@ -129,16 +129,31 @@ __device__ __forceinline__ bool feasible(const DevProblem &p, const AntLds &L, i
 
 // Full-row greedy pick when the roulette has no positive mass: largest
 // total_info among feasible nodes, ties to the lowest index; 0 = none.
+// The row's values are loaded kGreedyChunk per lane at a time, all issued
+// before any is used, so a chunk costs one memory latency.
+constexpr int kGreedyChunk = 8;
+
 template <bool HAS_D>
 __device__ int greedy_pick(const DevProblem &p, const AntLds &L, const float *tirow,
                            const float *drow, int load, float rdist, int lane)
 {
     u64 best = 0;
-    for (int i = lane; i < p.n; i += kLanes) {
-        const float dci = HAS_D ? drow[i] : 0.0f;
-        if (feasible<HAS_D>(p, L, i, dci, load, rdist)) {
-            const u64 key = ((u64)(f2u(tirow[i]) + 1u) << 32) | (u64)(0xFFFFFFFFu - (u32)i);
-            best = key > best ? key : best;
+    for (int base = 0; base < p.n; base += kGreedyChunk * kLanes) {
+        float tv[kGreedyChunk], dv[kGreedyChunk];
+#pragma unroll
+        for (int u = 0; u < kGreedyChunk; ++u) {
+            const int i = base + u * kLanes + lane;
+            const int ic = i < p.n ? i : p.n - 1;   // in-row index, masked below
+            tv[u] = tirow[ic];
+            dv[u] = HAS_D ? drow[ic] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < kGreedyChunk; ++u) {
+            const int i = base + u * kLanes + lane;
+            if (i < p.n && feasible<HAS_D>(p, L, i, dv[u], load, rdist)) {
+                const u64 key = ((u64)(f2u(tv[u]) + 1u) << 32) | (u64)(0xFFFFFFFFu - (u32)i);
+                best = key > best ? key : best;
+            }
         }
     }
     best = wave_max_u64(best);
@@ -355,11 +370,13 @@ __global__ __launch_bounds__(64) void k_construct_knn(DevBatch bt)
 
 
 // The same roulette with the instance's candidate table staged in LDS, for
-// instances without a route-length limit: A ants per block (one wave each,
-// no block-wide barrier after staging) share the n*k candidate ids (16-bit)
-// and total_info values, so a step's dependent read is an LDS read.  The
-// chosen edge's length is fetched after the pick and added one step later
-// (the same additions in the same order).
+// instances without a route-length limit and with 16-bit demands: A ants
+// per block (one wave each, no block-wide barrier after staging) share the
+// n*k candidate entries {total_info, id | demand << 16}, 8 bytes each, so a
+// step reads its candidates' values, ids and demands with one LDS load and
+// then their visited flags with a second.  The chosen edge's length is
+// fetched after the pick and added one step later (the same additions in
+// the same order).
 template <int RNG, int A>
 __global__ __launch_bounds__(64 * A) void k_construct_knn_lds(DevBatch bt)
 {
@@ -370,14 +387,13 @@ __global__ __launch_bounds__(64 * A) void k_construct_knn_lds(DevBatch bt)
     const DevState s = bt.states[inst];
     const int a = blockIdx.x * A + wv - bt.ant_base[inst];
     const int n = p.n, k = p.k, nk = n * k;
-    float *tik = reinterpret_cast<float *>(smem);
-    int *dem = reinterpret_cast<int *>(tik + nk);
+    uint2 *ent = reinterpret_cast<uint2 *>(smem);
+    int *dem = reinterpret_cast<int *>(ent + nk);
     float *d0 = reinterpret_cast<float *>(dem + n);
-    unsigned short *nid = reinterpret_cast<unsigned short *>(d0 + n);
-    unsigned char *vis = reinterpret_cast<unsigned char *>(nid + ((nk + 7) & ~7)) + wv * ((n + 15) & ~15);
+    unsigned char *vis = reinterpret_cast<unsigned char *>(d0 + n) + wv * ((n + 15) & ~15);
     for (int e = threadIdx.x; e < nk; e += 64 * A) {
-        tik[e] = s.tiknn[e];
-        nid[e] = (unsigned short)p.nnk[e];
+        const int id = p.nnk[e];
+        ent[e] = make_uint2(__float_as_uint(s.tiknn[e]), (unsigned)id | ((unsigned)p.dem[id] << 16));
     }
     for (int i = threadIdx.x; i < n; i += 64 * A) {
         dem[i] = p.dem[i];
@@ -395,14 +411,11 @@ __global__ __launch_bounds__(64 * A) void k_construct_knn_lds(DevBatch bt)
     while (vcnt < n) {
         step++;
         if (RNG == kRngPhilox && ((step - 1) & 63) == 0) ublk = philox_block(seed, iter, (u32)a, (u32)step, lane);
-        int j = 0, dj = 0;
-        float pj = 0.0f;
-        if (lane < k) {
-            j = nid[cur * k + lane];
-            const float tj = tik[cur * k + lane];
-            dj = dem[j];
-            pj = (!vis[j] && load + dj <= p.Q) ? tj : 0.0f;   // the mask of feasible<false>
-        }
+        // lanes >= k read the row's last entry and are masked
+        const uint2 en = ent[cur * k + (lane < k ? lane : k - 1)];
+        const int j = (int)(en.y & 0xFFFFu), dj = (int)(en.y >> 16);
+        const bool ok = (lane < k) & (vis[j] == 0) & (load + dj <= p.Q);   // the mask of feasible<false>
+        const float pj = ok ? __uint_as_float(en.x) : 0.0f;
         const float x = wave_scan_f32(pj, lane);
         const float total = lane_f32(x, k - 1);
         int w, dw;
@@ -456,14 +469,14 @@ __global__ __launch_bounds__(64 * A) void k_construct_knn_lds(DevBatch bt)
 size_t construct_lds_staged_bytes(int n, int k, int apb)
 {
     const size_t nk = (size_t)n * k;
-    return nk * 4 + (size_t)n * 8 + ((nk + 7) & ~(size_t)7) * 2 + (size_t)apb * ((n + 15) & ~15) + 16;
+    return nk * 8 + (size_t)n * 8 + (size_t)apb * ((n + 15) & ~15) + 16;
 }
 
 bool construct_staged_ok(const DevBatch &b, int selection, int k, int ants_per_instance, int apb)
 {
     if (apb == 0) return true;
     if (apb != 1 && apb != 2 && apb != 4) return false;
-    return selection == kSelKnn && !b.has_d && k > 0 && ants_per_instance % apb == 0;
+    return selection == kSelKnn && !b.has_d && b.dem16 && k > 0 && ants_per_instance % apb == 0;
 }
 
 hipError_t launch_construct(const DevBatch &b, int selection, int rng, int apb, int k, hipStream_t st)
