@@ -220,29 +220,43 @@ __device__ __forceinline__ void two_opt_cand(const DevProblem &p, const LsLds &L
     gain = -radius + dn + dx - ew;
 }
 
+// Candidates per lane per trip of a 2-opt scan: positions q, q+8, q+16 and
+// q+24 of the group's lanes, all loads issued before any is used (routes of
+// up to 32 customers take one trip, i.e. one memory latency).
+constexpr int kOptUnroll = 4;
+
 template <bool SUCC>
 __device__ __forceinline__ bool two_opt_scan(const DevProblem &p, const LsLds &L, int rbeg,
                                              int rend, int pos1, int n1, int nb1, float radius,
                                              int gl, long long &evals, int &h1, int &h2, int &h3,
                                              int &h4)
 {
+    constexpr int U = kOptUnroll;
     const unsigned short *rowpos = p.nnpos + (size_t)n1 * p.n;
     u64 best = ~0ull;
     int cnt = 0;
-    for (int q = rbeg + 1 + gl; q <= rend; q += 2 * kOptGroup) {
-        bool ok0, ok1;
-        float g0, g1;
-        int rp0, rp1, a0, a1, x0, x1;
-        two_opt_cand<SUCC>(p, L, rend, pos1, q, n1, nb1, radius, rowpos, ok0, g0, rp0, a0, x0);
-        two_opt_cand<SUCC>(p, L, rend, pos1, q + kOptGroup, n1, nb1, radius, rowpos, ok1, g1, rp1, a1, x1);
-        cnt += ok0 + ok1;
-        if (ok0 && g0 < -kEps) {
-            const u64 key = ((u64)rp0 << 32) | ((u64)a0 << 16) | (u64)x0;
-            best = key < best ? key : best;
-        }
-        if (ok1 && g1 < -kEps) {
-            const u64 key = ((u64)rp1 << 32) | ((u64)a1 << 16) | (u64)x1;
-            best = key < best ? key : best;
+    // the first trip's candidates, kept for the evaluated-move count
+    bool okk[U];
+    int rpk[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) { okk[u] = false; rpk[u] = 0; }
+    for (int q0 = rbeg + 1 + gl; q0 <= rend; q0 += U * kOptGroup) {
+        bool ok[U];
+        float g[U];
+        int rp[U], a[U], x[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            two_opt_cand<SUCC>(p, L, rend, pos1, q0 + u * kOptGroup, n1, nb1, radius, rowpos, ok[u], g[u],
+                               rp[u], a[u], x[u]);
+        const bool first = q0 == rbeg + 1 + gl;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            cnt += ok[u];
+            if (ok[u] && g[u] < -kEps) {
+                const u64 key = ((u64)rp[u] << 32) | ((u64)a[u] << 16) | (u64)x[u];
+                best = key < best ? key : best;
+            }
+            if (first) { okk[u] = ok[u]; rpk[u] = rp[u]; }
         }
     }
     best = group_min_u64(best);
@@ -254,13 +268,21 @@ __device__ __forceinline__ bool two_opt_scan(const DevProblem &p, const LsLds &L
     // the candidates up to that row position
     const int rps = (int)(best >> 32);
     cnt = 0;
-    for (int q = rbeg + 1 + gl; q <= rend; q += 2 * kOptGroup) {
-        bool ok0, ok1;
-        float g0, g1;
-        int rp0, rp1, a0, a1, x0, x1;
-        two_opt_cand<SUCC>(p, L, rend, pos1, q, n1, nb1, radius, rowpos, ok0, g0, rp0, a0, x0);
-        two_opt_cand<SUCC>(p, L, rend, pos1, q + kOptGroup, n1, nb1, radius, rowpos, ok1, g1, rp1, a1, x1);
-        cnt += (ok0 && rp0 <= rps) + (ok1 && rp1 <= rps);
+    if (rend - rbeg <= U * kOptGroup) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) cnt += okk[u] && rpk[u] <= rps;
+    } else {
+        for (int q0 = rbeg + 1 + gl; q0 <= rend; q0 += U * kOptGroup) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                bool ok;
+                float g;
+                int rp, a, x;
+                two_opt_cand<SUCC>(p, L, rend, pos1, q0 + u * kOptGroup, n1, nb1, radius, rowpos, ok, g, rp,
+                                   a, x);
+                cnt += ok && rp <= rps;
+            }
+        }
     }
     evals += cnt;
     const int b2 = (int)((best >> 16) & 0xFFFF), bx = (int)(best & 0xFFFF);
