@@ -520,9 +520,29 @@ int aco_create_batch(const aco_problem *pbs, int32_t count, const aco_params *pr
         bt.has_d |= hp[b].has_d;
         bt.dem16 &= hp[b].dem16;
     }
+    // local-search size classes by n (upper bounds), largest first
+    static const int kClassMax[kLsClasses] = {128, 256, 384, 512, 768, 1 << 30};
+    std::vector<int> ls_order;
+    ls_order.reserve(ant_inst.size());
+    bt.ls_classes = 0;
+    for (int cl = kLsClasses - 1; cl >= 0; --cl) {
+        const int lo = cl > 0 ? kClassMax[cl - 1] : 0;
+        const int first = (int)ls_order.size();
+        int maxn = 0;
+        for (int b = 0; b < count; ++b) {
+            if (hp[b].n <= lo || hp[b].n > kClassMax[cl]) continue;
+            maxn = std::max(maxn, hp[b].n);
+            for (int a = 0; a < m; ++a) ls_order.push_back(b * m + a);
+        }
+        if ((int)ls_order.size() == first) continue;
+        bt.ls_first[bt.ls_classes] = first;
+        bt.ls_maxn[bt.ls_classes] = maxn;
+        ++bt.ls_classes;
+    }
+    bt.ls_first[bt.ls_classes] = (int)ls_order.size();
     void *q = nullptr;
     const size_t tb = sizeof(DevProblem) * count + sizeof(DevState) * count + 4 * ant_inst.size() +
-                      4 * ant_base.size() + 1024;
+                      4 * ant_base.size() + 4 * ls_order.size() + 1280;
     CHK(hipMalloc(&q, tb));
     c->allocs.push_back(q);
     Carver cv{static_cast<unsigned char *>(q)};
@@ -530,7 +550,9 @@ int aco_create_batch(const aco_problem *pbs, int32_t count, const aco_params *pr
     DevState *dstates = cv.take<DevState>(count);
     int *dinst = cv.take<int>(ant_inst.size());
     int *dbase = cv.take<int>(ant_base.size());
+    int *dorder = cv.take<int>(ls_order.size());
     CHK(hipMemcpyAsync(dprobs, hp.data(), sizeof(DevProblem) * count, hipMemcpyHostToDevice, c->stream));
+    CHK(hipMemcpyAsync(dorder, ls_order.data(), 4 * ls_order.size(), hipMemcpyHostToDevice, c->stream));
     CHK(hipMemcpyAsync(dstates, hs.data(), sizeof(DevState) * count, hipMemcpyHostToDevice, c->stream));
     CHK(hipMemcpyAsync(dinst, ant_inst.data(), 4 * ant_inst.size(), hipMemcpyHostToDevice, c->stream));
     CHK(hipMemcpyAsync(dbase, ant_base.data(), 4 * ant_base.size(), hipMemcpyHostToDevice, c->stream));
@@ -539,6 +561,7 @@ int aco_create_batch(const aco_problem *pbs, int32_t count, const aco_params *pr
     bt.states = dstates;
     bt.ant_inst = dinst;
     bt.ant_base = dbase;
+    bt.ls_order = dorder;
 #undef CHK
     *out = c;
     return ACO_OK;

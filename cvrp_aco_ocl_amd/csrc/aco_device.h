@@ -163,6 +163,11 @@ struct DevState {
 // state live in device tables, construction and local-search blocks map
 // to (instance, ant) through ant_inst / ant_base, the per-instance kernels
 // use blockIdx.y as the instance.
+// Local-search size classes of a batch: the ants of instances with similar
+// n are launched together with LDS sized for the class, so ants of small
+// instances pack densely on the CUs.
+constexpr int kLsClasses = 6;
+
 struct DevBatch {
     const DevProblem *probs;  // B
     const DevState *states;   // B
@@ -172,6 +177,10 @@ struct DevBatch {
     int max_n, max_P, max_nk;
     int has_d;                // some instance has a route-length limit
     int dem16;                // every instance's demands fit 16 bits
+    const int *ls_order;              // global ant ids grouped by size class, largest class first
+    int ls_classes;                   // non-empty classes
+    int ls_first[kLsClasses + 1];     // class c: ls_order[ls_first[c] .. ls_first[c+1])
+    int ls_maxn[kLsClasses];          // largest n in class c
 };
 
 enum { kTau0Arg = 0, kTau0Bsf = 1, kTau0Nn = 2 };

@@ -948,14 +948,15 @@ __device__ __forceinline__ void ls_relocate(const DevProblem &p, LsLds &L, const
 // ------------------------------------------------------------ kernel ----
 
 template <int W, typename RT>
-__global__ __launch_bounds__(W * 64) void k_local_search(DevBatch bt, int flags, int ring, int prefetch)
+__global__ __launch_bounds__(W * 64) void k_local_search(DevBatch bt, int first, int flags, int ring, int prefetch)
 {
     constexpr int NT = LsW<W>::threads;
     extern __shared__ __align__(16) unsigned char smem[];
-    const int inst = bt.ant_inst[blockIdx.x];
+    const int g = bt.ls_order[first + blockIdx.x];   // this launch's size class
+    const int inst = bt.ant_inst[g];
     const DevProblem p = bt.probs[inst];
     const DevState s = bt.states[inst];
-    const int a = blockIdx.x - bt.ant_base[inst], tid = threadIdx.x;
+    const int a = g - bt.ant_base[inst], tid = threadIdx.x;
     LsLds L = ls_bind(p.n, ring, (int)sizeof(RT), smem);
     const RowRing<RT> rr = {reinterpret_cast<RT *>(L.rows), ring - 1, row_stride<RT>(p.n)};
     const bool pf_on = prefetch != 0;
@@ -1040,17 +1041,24 @@ hipError_t launch_local_search(const DevBatch &b, int flags, const LsShape &sh, 
 {
     if (!flags) return hipSuccess;
     if (!ls_shape_valid(sh)) return hipErrorInvalidValue;
-    const size_t lds = local_search_lds_bytes(b.max_n, sh.ring, sh.row16);
-#define KLS(W, RT) hipLaunchKernelGGL((k_local_search<W, RT>), dim3(b.ants), dim3(W * kLanes), lds, st, b, \
+    // one launch per size class, largest instances first, each with LDS for
+    // its own largest n
+    for (int cl = 0; cl < b.ls_classes; ++cl) {
+        const int first = b.ls_first[cl], count = b.ls_first[cl + 1] - first;
+        const size_t lds = local_search_lds_bytes(b.ls_maxn[cl], sh.ring, sh.row16);
+#define KLS(W, RT) hipLaunchKernelGGL((k_local_search<W, RT>), dim3(count), dim3(W * kLanes), lds, st, b, first, \
                                       flags, sh.ring, sh.prefetch)
-    switch (sh.waves) {
-    case 16: if (sh.row16) KLS(16, unsigned short); else KLS(16, float); break;
-    case 12: if (sh.row16) KLS(12, unsigned short); else KLS(12, float); break;
-    case 8: if (sh.row16) KLS(8, unsigned short); else KLS(8, float); break;
-    default: if (sh.row16) KLS(4, unsigned short); else KLS(4, float); break;
-    }
+        switch (sh.waves) {
+        case 16: if (sh.row16) KLS(16, unsigned short); else KLS(16, float); break;
+        case 12: if (sh.row16) KLS(12, unsigned short); else KLS(12, float); break;
+        case 8: if (sh.row16) KLS(8, unsigned short); else KLS(8, float); break;
+        default: if (sh.row16) KLS(4, unsigned short); else KLS(4, float); break;
+        }
 #undef KLS
-    return hipGetLastError();
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 }  // namespace cvrp
