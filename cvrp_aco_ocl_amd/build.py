@@ -31,8 +31,8 @@ DEVICE_FLAGS = [
     f"-I{INCLUDE}", f"-I{CSRC}",
 ]
 
-LIB_SOURCES = ["aco_kernels.hip", "aco_local_search.hip", "aco_prior.hip", "aco_capi.cpp",
-               "aco_host.cpp"]
+LIB_SOURCES = ["aco_kernels.hip", "aco_local_search.hip", "aco_prior.hip", "aco_nnlist.hip",
+               "aco_capi.cpp", "aco_host.cpp"]
 CLI_SOURCES = ["g_aco.cpp", "main.cpp"]
 
 

@@ -94,6 +94,87 @@ struct aco_ctx {
             return set_error(ACO_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+// Host-to-device uploads through two pinned buffers used in turn: a chunk
+// is copied into a buffer whose previous transfer has finished and sent
+// asynchronously, so uploads run at DMA speed while the host prepares the
+// next chunk.  The destructor waits for the transfers before freeing.
+struct PinnedStager {
+    static constexpr size_t kBytes = size_t(32) << 20;
+    unsigned char *buf[2] = {nullptr, nullptr};
+    hipEvent_t done[2] = {nullptr, nullptr};
+    bool used[2] = {false, false};
+    int cur = 0;
+    hipStream_t st = nullptr;
+
+    hipError_t init(hipStream_t s)
+    {
+        st = s;
+        for (int i = 0; i < 2; ++i) {
+            hipError_t e = hipHostMalloc((void **)&buf[i], kBytes, hipHostMallocDefault);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&done[i], hipEventDisableTiming);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    }
+    ~PinnedStager()
+    {
+        for (int i = 0; i < 2; ++i) {
+            if (used[i]) (void)hipEventSynchronize(done[i]);
+            if (done[i]) (void)hipEventDestroy(done[i]);
+            if (buf[i]) (void)hipHostFree(buf[i]);
+        }
+    }
+    // the current buffer, once its previous transfer has finished
+    hipError_t take(unsigned char *&p)
+    {
+        p = buf[cur];
+        return used[cur] ? hipEventSynchronize(done[cur]) : hipSuccess;
+    }
+    hipError_t sent()
+    {
+        const hipError_t e = hipEventRecord(done[cur], st);
+        used[cur] = true;
+        cur ^= 1;
+        return e;
+    }
+    hipError_t put(void *dst, const void *src, size_t bytes)
+    {
+        unsigned char *d = static_cast<unsigned char *>(dst);
+        const unsigned char *s = static_cast<const unsigned char *>(src);
+        while (bytes) {
+            const size_t c = bytes < kBytes ? bytes : kBytes;
+            unsigned char *p = nullptr;
+            hipError_t e = take(p);
+            if (e != hipSuccess) return e;
+            memcpy(p, s, c);
+            e = hipMemcpyAsync(d, p, c, hipMemcpyHostToDevice, st);
+            if (e != hipSuccess) return e;
+            if ((e = sent()) != hipSuccess) return e;
+            d += c;
+            s += c;
+            bytes -= c;
+        }
+        return hipSuccess;
+    }
+    // `rows` dense rows of `width` bytes into a device array of row pitch dpitch
+    hipError_t put2d(void *dst, size_t dpitch, const void *src, size_t width, size_t rows)
+    {
+        const size_t per = std::max<size_t>(1, kBytes / width);
+        for (size_t r0 = 0; r0 < rows; r0 += per) {
+            const size_t nr = std::min(per, rows - r0);
+            unsigned char *p = nullptr;
+            hipError_t e = take(p);
+            if (e != hipSuccess) return e;
+            memcpy(p, static_cast<const unsigned char *>(src) + r0 * width, nr * width);
+            e = hipMemcpy2DAsync(static_cast<unsigned char *>(dst) + r0 * dpitch, dpitch, p, width, width, nr,
+                                 hipMemcpyHostToDevice, st);
+            if (e != hipSuccess) return e;
+            if ((e = sent()) != hipSuccess) return e;
+        }
+        return hipSuccess;
+    }
+};
+
 // Bump allocator over one device allocation (256-byte aligned pieces).
 struct Carver {
     unsigned char *base;
@@ -292,9 +373,8 @@ static int check_instance(const aco_problem *pb, int width, std::vector<int> &nn
                     return set_error(ACO_ERR_ARG, "nn_list rows must be sorted by distance");
                 nnl[(size_t)i * width + h] = v;
             }
-    } else {
-        nn_lists(n, pb->dist, width, 0, nnl.data());
     }
+    // without nn_list the lists are built on the device in aco_create
     return ACO_OK;
 }
 
@@ -386,6 +466,24 @@ int aco_create_batch(const aco_problem *pbs, int32_t count, const aco_params *pr
     CHK(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, c->device));
     c->tick_to_s = rate_khz > 0 ? 1.0f / (1000.0f * (float)rate_khz) : 0.0f;
     CHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    {
+        // candidate lists of the instances given without them
+        // (compute_nn_lists, vrpHelper.cpp:188-236), all at once
+        std::vector<int> nv, wv;
+        std::vector<const float *> dv;
+        std::vector<int *> ov;
+        for (int b = 0; b < count; ++b) {
+            if (pbs[b].nn_list) continue;
+            nv.push_back(pbs[b].n);
+            wv.push_back(std::max(prm.k, nn_ls[b]));
+            dv.push_back(pbs[b].dist);
+            ov.push_back(nnl[b].data());
+        }
+        if (!nv.empty()) {
+            const int rc2 = gpu_nn_lists(c->stream, (int)nv.size(), nv.data(), dv.data(), wv.data(), 0, ov.data());
+            if (rc2) return fail(rc2);
+        }
+    }
 
     const int m = c->m, ras = prm.ras_ranks, k = prm.k;
     unsigned long long *clock0 = nullptr;
@@ -399,6 +497,8 @@ int aco_create_batch(const aco_problem *pbs, int32_t count, const aco_params *pr
     tree_rank(kLanes, r64.data());
     for (int l = 0; l < kLanes; ++l) r64inv[r64[l]] = l;
 
+    PinnedStager up;
+    CHK(up.init(c->stream));
     c->inst.resize(count);
     for (int b = 0; b < count; ++b) {
         const aco_problem *pb = &pbs[b];
@@ -455,13 +555,9 @@ int aco_create_batch(const aco_problem *pbs, int32_t count, const aco_params *pr
             }
         }
         DevProblem &dp = I.dp;
-        auto h2d = [&](void *dst, const void *src, size_t bytes_) {
-            return hipMemcpyAsync(dst, src, bytes_, hipMemcpyHostToDevice, c->stream);
-        };
-        CHK(hipMemcpy2DAsync((void *)dp.dist, (size_t)P * 4, pb->dist, (size_t)n * 4, (size_t)n * 4, n,
-                             hipMemcpyHostToDevice, c->stream));
-        CHK(hipMemcpy2DAsync((void *)dp.heur, (size_t)P * 4, heur.data(), (size_t)n * 4, (size_t)n * 4, n,
-                             hipMemcpyHostToDevice, c->stream));
+        auto h2d = [&](void *dst, const void *src, size_t bytes_) { return up.put(dst, src, bytes_); };
+        CHK(up.put2d((void *)dp.dist, (size_t)P * 4, pb->dist, (size_t)n * 4, n));
+        CHK(up.put2d((void *)dp.heur, (size_t)P * 4, heur.data(), (size_t)n * 4, n));
         CHK(h2d((void *)dp.d0, d0.data(), (size_t)n * 4));
         CHK(h2d((void *)dp.dem, pb->demand, (size_t)n * 4));
         CHK(h2d((void *)dp.nnk, nnk.data(), nnk.size() * 4));
@@ -480,7 +576,6 @@ int aco_create_batch(const aco_problem *pbs, int32_t count, const aco_params *pr
         if (fits16) CHK(h2d((void *)dp.dist16, d16.data(), d16.size() * 2));
         CHK(h2d((void *)dp.rank64, r64.data(), kLanes * 4));
         CHK(h2d((void *)dp.rank64inv, r64inv.data(), kLanes * 4));
-        CHK(hipStreamSynchronize(c->stream));  // host vectors die at the end of this iteration
 
         dp.n = n; dp.P = P; dp.k = k; dp.nn_ls = nl; dp.nnP = nnP; dp.m = m; dp.T = I.T; dp.ras = ras;
         dp.Q = pb->capacity; dp.D = pb->max_dist; dp.svc = pb->service_time;

@@ -201,6 +201,10 @@ hipError_t launch_construct(const DevBatch &b, int selection, int rng, int apb, 
 size_t construct_lds_staged_bytes(int n, int k, int apb);
 bool construct_staged_ok(const DevBatch &b, int selection, int k, int ants_per_instance, int apb);
 hipError_t launch_local_search(const DevBatch &b, int flags, const LsShape &sh, hipStream_t st);
+// compute_nn_lists for `count` instances on the device (host arrays in and
+// out, one thread per row, the reference's sort steps); ACO_OK or an error.
+int gpu_nn_lists(hipStream_t st, int count, const int *n, const float *const *dist, const int *width,
+                 int tie, int *const *out);
 hipError_t launch_statistics(const DevBatch &b, float tick_to_s, hipStream_t st);
 hipError_t launch_pheromone_update(const DevBatch &b, hipStream_t st);
 hipError_t launch_gather(const DevBatch &b, hipStream_t st);

@@ -85,6 +85,22 @@ class Instance:
         check(lib.aco_compute_nn_lists(self.n, _ptr(d), nn, tie, _ptr(out)), "aco_compute_nn_lists")
         return out
 
+    @staticmethod
+    def nn_lists_batch(insts, nn, tie: int = 0) -> list:
+        """nn_lists of many instances in one pass on the GPU
+        (aco_compute_nn_lists_batch); nn is one width or one per instance."""
+        insts = list(insts)
+        widths = [nn] * len(insts) if np.isscalar(nn) else list(nn)
+        outs = [np.empty((it.n, w), np.int32) for it, w in zip(insts, widths)]
+        ds = [np.ascontiguousarray(it.dist, np.float32) for it in insts]
+        ns = np.array([it.n for it in insts], np.int32)
+        ws = np.array(widths, np.int32)
+        dptr = (C.c_void_p * len(insts))(*[d.ctypes.data for d in ds])
+        optr = (C.c_void_p * len(insts))(*[o.ctypes.data for o in outs])
+        check(lib.aco_compute_nn_lists_batch(len(insts), _ptr(ns), C.cast(dptr, C.c_void_p), _ptr(ws), tie,
+                                             C.cast(optr, C.c_void_p)), "aco_compute_nn_lists_batch")
+        return outs
+
     def check_solution(self, tour) -> bool:
         t = np.ascontiguousarray(tour, np.int32)
         d = np.ascontiguousarray(self.dist, np.float32)

@@ -218,6 +218,14 @@ int aco_compute_distances(int32_t n, const double *x, const double *y, int32_t m
 /* compute_nn_lists (vrpHelper.cpp:188-236): self and depot excluded;
  * tie 0 = the reference's quicksort order, tie 1 = (distance, index) */
 int aco_compute_nn_lists(int32_t n, const float *dist, int32_t nn, int32_t tie, int32_t *out);
+/* The same lists for `count` instances at once on the current HIP device
+ * (compute_nn_lists, vrpHelper.cpp:188-236; one thread per row, the same
+ * sort steps, so the result equals aco_compute_nn_lists instance by
+ * instance).  n[b], dist[b] (n[b] x n[b] host array), nn[b] (1..n[b]-2)
+ * entries per row into out[b] (n[b] x nn[b] host array).  aco_create uses
+ * it for instances given without nn_list. */
+int aco_compute_nn_lists_batch(int32_t count, const int32_t *n, const float *const *dist,
+                               const int32_t *nn, int32_t tie, int32_t *const *out);
 /* read_instance_file (io.cpp:334-517).  First call with capacity 0 to get
  * *n; then call with capacity >= *n to fill the arrays.  max_dist is
  * INFINITY and service_time 0 when the file has none; edge_weight_type is
