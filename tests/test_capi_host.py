@@ -69,6 +69,15 @@ def test_nn_lists_match_oracle(tie):
     np.testing.assert_array_equal(inst.nn_lists(40, tie), oracle.nn_lists(inst.dist, 40, tie))
 
 
+def test_nn_lists_batch_rejects_bad_arguments_before_device_work():
+    insts = [I.uniform_100(9), I.uniform_100(10)]
+    with pytest.raises(ca.AcoError):
+        ca.Instance.nn_lists_batch(insts, [40, insts[1].n - 1])      # wider than n-2
+    with pytest.raises(ca.AcoError):
+        ca.Instance.nn_lists_batch(insts, 40, tie=2)
+    assert _lib.lib.aco_compute_nn_lists_batch(0, None, None, None, 0, None) == 0
+
+
 def test_read_vrplib_fields():
     inst = ca.Instance.from_vrplib(os.path.join(VRPLIB, "CMT6.vrp"))
     assert inst.n == 51 and inst.capacity == 160
