@@ -144,6 +144,9 @@ def main():
           file=sys.stderr, flush=True)
 
     # ---- per-kernel device time (HIP events on the library stream) ----
+    # (per-phase launches, so each kernel is timed on its own)
+    if shape[5]:
+        g.set_launch_shape(*shape[:5], 0)
     g.profile_enable(True)
     fb_before = g.counters()["fallbacks"]
     ev_before = g.counters()["ls_evals"]
@@ -327,7 +330,8 @@ def main():
             "n": inst.n, "ants": m, "k": k, "capacity": inst.capacity, "seed": args.seed,
             "step": "one ACO iteration (hipGraph replay)",
             "launch_shape": {"ls_waves": shape[0], "ls_ring": shape[1], "ls_prefetch": shape[2],
-                             "construct_ants_per_block": shape[3], "ls_row16": shape[4]},
+                             "construct_ants_per_block": shape[3], "ls_row16": shape[4],
+                             "fused_iteration": shape[5]},
         },
         "roofline": roofline,
         "kernels": per_kernel,

@@ -64,7 +64,7 @@ PRIOR_PARAMS = 64 * 12 + 64 + 64 * 64 + 64 + 64 + 1
 
 class AcoLaunchShape(C.Structure):
     _fields_ = [("ls_waves", C.c_int32), ("ls_ring", C.c_int32), ("ls_prefetch", C.c_int32),
-                ("construct_apb", C.c_int32), ("ls_row16", C.c_int32)]
+                ("construct_apb", C.c_int32), ("ls_row16", C.c_int32), ("fused_iteration", C.c_int32)]
 
 
 class AcoTuneResult(C.Structure):

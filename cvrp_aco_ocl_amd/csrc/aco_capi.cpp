@@ -290,6 +290,12 @@ static int run_pheromone(aco_ctx *c)
 static int run_iteration_steps(aco_ctx *c)
 {
     int rc;
+    if (c->ls.fused) {
+        HIP_TRY(timed(c, "iteration_fused", [&] {
+            return launch_iteration_fused(c->bt, c->prm.rng, c->prm.ls_flags, c->ls, c->tick_to_s, c->stream);
+        }));
+        return ACO_OK;
+    }
     if ((rc = run_construct(c)) != ACO_OK) return rc;
     if ((rc = run_local_search(c)) != ACO_OK) return rc;
     if ((rc = run_statistics(c)) != ACO_OK) return rc;
@@ -637,7 +643,7 @@ int aco_create_batch(const aco_problem *pbs, int32_t count, const aco_params *pr
     bt.ls_first[bt.ls_classes] = (int)ls_order.size();
     void *q = nullptr;
     const size_t tb = sizeof(DevProblem) * count + sizeof(DevState) * count + 4 * ant_inst.size() +
-                      4 * ant_base.size() + 4 * ls_order.size() + 1280;
+                      4 * ant_base.size() + 4 * ls_order.size() + 1536;
     CHK(hipMalloc(&q, tb));
     c->allocs.push_back(q);
     Carver cv{static_cast<unsigned char *>(q)};
@@ -646,6 +652,7 @@ int aco_create_batch(const aco_problem *pbs, int32_t count, const aco_params *pr
     int *dinst = cv.take<int>(ant_inst.size());
     int *dbase = cv.take<int>(ant_base.size());
     int *dorder = cv.take<int>(ls_order.size());
+    int *dfq = cv.take<int>(1);
     CHK(hipMemcpyAsync(dprobs, hp.data(), sizeof(DevProblem) * count, hipMemcpyHostToDevice, c->stream));
     CHK(hipMemcpyAsync(dorder, ls_order.data(), 4 * ls_order.size(), hipMemcpyHostToDevice, c->stream));
     CHK(hipMemcpyAsync(dstates, hs.data(), sizeof(DevState) * count, hipMemcpyHostToDevice, c->stream));
@@ -657,6 +664,7 @@ int aco_create_batch(const aco_problem *pbs, int32_t count, const aco_params *pr
     bt.ant_inst = dinst;
     bt.ant_base = dbase;
     bt.ls_order = dorder;
+    bt.fq_next = dfq;
 #undef CHK
     *out = c;
     return ACO_OK;
@@ -989,6 +997,9 @@ static bool shape_fits(const aco_ctx *c, const LsShape &sh)
 {
     if (!ls_shape_valid(sh) || local_search_lds_bytes(c->max_n, sh.ring, sh.row16) > c->lds_limit) return false;
     if (sh.row16 && !c->int16_rows) return false;
+    if (sh.fused && (!fused_iteration_ok(c->bt, c->prm.selection, c->prm.ls_flags, c->m, sh) ||
+                     fused_lds_bytes(c->max_n, sh) > c->lds_limit))
+        return false;
     if (!construct_staged_ok(c->bt, c->prm.selection, c->prm.k, c->m, sh.construct_apb)) return false;
     return sh.construct_apb == 0 ||
            construct_lds_staged_bytes(c->max_n, c->prm.k, sh.construct_apb) <= c->lds_limit;
@@ -996,7 +1007,7 @@ static bool shape_fits(const aco_ctx *c, const LsShape &sh)
 
 static aco_launch_shape to_public(const LsShape &sh)
 {
-    return aco_launch_shape{sh.waves, sh.ring, sh.prefetch, sh.construct_apb, sh.row16};
+    return aco_launch_shape{sh.waves, sh.ring, sh.prefetch, sh.construct_apb, sh.row16, sh.fused};
 }
 
 int aco_get_launch_shape(aco_ctx *c, aco_launch_shape *out)
@@ -1014,7 +1025,7 @@ int aco_set_launch_shape(aco_ctx *c, const aco_launch_shape *in)
     if (rc) return rc;
     if (!in) return set_error(ACO_ERR_ARG, "null shape");
     const LsShape sh{in->ls_waves, in->ls_ring, in->ls_prefetch ? 1 : 0, in->construct_apb,
-                     in->ls_row16 ? 1 : 0};
+                     in->ls_row16 ? 1 : 0, in->fused_iteration ? 1 : 0};
     if (sh.row16 && !c->int16_rows)
         return set_error(ACO_ERR_ARG, "16-bit rows need every distance to be an integer in [0, 65535]");
     if (!ls_shape_valid(sh))
@@ -1023,6 +1034,9 @@ int aco_set_launch_shape(aco_ctx *c, const aco_launch_shape *in)
     if (!construct_staged_ok(c->bt, c->prm.selection, c->prm.k, c->m, sh.construct_apb))
         return set_error(ACO_ERR_ARG, "construct_apb must be 0, or 1|2|4 dividing the ants per instance "
                                       "for candidate roulette without a route-length limit");
+    if (sh.fused && !fused_iteration_ok(c->bt, c->prm.selection, c->prm.ls_flags, c->m, sh))
+        return set_error(ACO_ERR_ARG, "fused_iteration needs candidate roulette, no route-length limit, local "
+                                      "search, and 4 or 8 waves dividing the ants per instance");
     if (!shape_fits(c, sh))
         return set_error(ACO_ERR_ARG, "shape needs more LDS than the device's " + std::to_string(c->lds_limit) +
                                           " bytes per block");
@@ -1044,15 +1058,16 @@ int aco_autotune(aco_ctx *c, int32_t reps, aco_tune_result *results, int32_t cap
                                   {16, 32, 0, 0, 1}};
     constexpr int kNc = (int)(sizeof(kApb) / sizeof(kApb[0]));
     constexpr int kNl = (int)(sizeof(kLs) / sizeof(kLs[0]));
-    if (count) *count = kNc + kNl;
+    constexpr int kNi = 4;   // stage 2: the per-phase iteration and up to 3 fused shapes
+    if (count) *count = kNc + kNl + kNi;
     HIP_TRY(hipStreamSynchronize(c->stream));
-    // the mutable per-instance state lies between ds.tours and the end of
+    // the mutable per-instance state lies between ds.tau and the end of
     // ds.totals in the instance arena
     struct Span { unsigned char *lo; size_t bytes; };
     std::vector<Span> spans;
     size_t total = 0;
     for (auto &I : c->inst) {
-        unsigned char *lo = reinterpret_cast<unsigned char *>(I.ds.tours);
+        unsigned char *lo = reinterpret_cast<unsigned char *>(I.ds.tau);
         unsigned char *hi = reinterpret_cast<unsigned char *>(I.ds.totals + kTotWords);
         spans.push_back({lo, (size_t)(hi - lo)});
         total += (size_t)(hi - lo);
@@ -1115,6 +1130,44 @@ int aco_autotune(aco_ctx *c, int32_t reps, aco_tune_result *results, int32_t cap
         time_it(built, [&] { return launch_local_search(c->bt, c->prm.ls_flags, kLs[q], c->stream); }, tl[q]);
         if (err == hipSuccess && (best_l < 0 || tl[q] < tl[best_l])) best_l = q;
     }
+    // stage 2: whole iterations, per-phase launches against the fused launch
+    LsShape chosen = c->ls;
+    if (best_l >= 0) {
+        chosen.waves = kLs[best_l].waves;
+        chosen.ring = kLs[best_l].ring;
+        chosen.prefetch = kLs[best_l].prefetch;
+        chosen.row16 = kLs[best_l].row16;
+    }
+    if (best_c >= 0) chosen.construct_apb = kApb[best_c];
+    chosen.fused = 0;
+    LsShape it_sh[kNi];
+    float ti[kNi];
+    int best_i = -1;
+    for (int q = 0; q < kNi; ++q) {
+        ti[q] = -1.0f;
+        it_sh[q] = chosen;
+        it_sh[q].fused = q > 0;
+    }
+    it_sh[2].waves = 4; it_sh[2].ring = 16; it_sh[2].prefetch = 1;
+    it_sh[3].waves = 8; it_sh[3].ring = 16; it_sh[3].prefetch = 0;
+    for (int q = 0; q < kNi && err == hipSuccess && c->prm.ls_flags; ++q) {
+        const LsShape sh = it_sh[q];
+        bool dup = false;
+        for (int r = 1; r < q; ++r)
+            dup |= it_sh[r].waves == sh.waves && it_sh[r].ring == sh.ring && it_sh[r].prefetch == sh.prefetch;
+        if (dup || !shape_fits(c, sh)) continue;
+        time_it(orig, [&] {
+            if (sh.fused) return launch_iteration_fused(c->bt, c->prm.rng, c->prm.ls_flags, sh, c->tick_to_s, c->stream);
+            hipError_t e = launch_construct(c->bt, c->prm.selection, c->prm.rng, sh.construct_apb, c->prm.k, c->stream);
+            if (e == hipSuccess) e = launch_local_search(c->bt, c->prm.ls_flags, sh, c->stream);
+            if (e == hipSuccess) e = launch_statistics(c->bt, c->tick_to_s, c->stream);
+            if (e == hipSuccess) e = launch_pheromone_update(c->bt, c->stream);
+            if (e == hipSuccess) e = launch_gather(c->bt, c->stream);
+            return e;
+        }, ti[q]);
+        if (err == hipSuccess && (best_i < 0 || ti[q] < ti[best_i])) best_i = q;
+    }
+    if (best_i > 0) chosen = it_sh[best_i];
     if (err == hipSuccess) err = copy_all(orig, false);
     // the search clock (best-so-far record times) restarts after tuning
     if (err == hipSuccess) err = launch_stamp_clock(c->bt, c->stream);
@@ -1124,14 +1177,6 @@ int aco_autotune(aco_ctx *c, int32_t reps, aco_tune_result *results, int32_t cap
     (void)hipFree(orig);
     (void)hipFree(built);
     if (err != hipSuccess) return set_error(ACO_ERR_HIP, std::string("aco_autotune: ") + hipGetErrorString(err));
-    LsShape chosen = c->ls;
-    if (best_l >= 0) {
-        chosen.waves = kLs[best_l].waves;
-        chosen.ring = kLs[best_l].ring;
-        chosen.prefetch = kLs[best_l].prefetch;
-        chosen.row16 = kLs[best_l].row16;
-    }
-    if (best_c >= 0) chosen.construct_apb = kApb[best_c];
     c->ls = chosen;
     drop_graph(c);
     int w = 0;
@@ -1145,6 +1190,8 @@ int aco_autotune(aco_ctx *c, int32_t reps, aco_tune_result *results, int32_t cap
         sh.construct_apb = chosen.construct_apb;
         results[w] = aco_tune_result{to_public(sh), 1, tl[q]};
     }
+    for (int q = 0; q < kNi && results && w < capacity; ++q, ++w)
+        results[w] = aco_tune_result{to_public(it_sh[q]), 2, ti[q]};
     return ACO_OK;
 }
 

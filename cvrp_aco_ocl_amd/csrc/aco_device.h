@@ -109,6 +109,7 @@ enum { kLs2Opt = 1, kLsSwap = 2, kLsRelocate = 4 };
 
 // control words (DevState::ctrl)
 enum { kCtrlIter = 0, kCtrlNbsf = 1, kCtrlIterBest = 2, kCtrlFlag = 3, kCtrlOverflow = 4,
+       kCtrlDone = 5,   // fused iteration: ants of the instance finished this launch
        kCtrlWords = 8 };
 // running totals (DevState::totals)
 enum { kTotEvals = 0, kTotMoves = 1, kTotFallbacks = 2, kTotIters = 3, kTotWords = 4 };
@@ -181,6 +182,7 @@ struct DevBatch {
     int ls_classes;                   // non-empty classes
     int ls_first[kLsClasses + 1];     // class c: ls_order[ls_first[c] .. ls_first[c+1])
     int ls_maxn[kLsClasses];          // largest n in class c
+    int *fq_next;                     // fused iteration: next work item (reset before each launch)
 };
 
 enum { kTau0Arg = 0, kTau0Bsf = 1, kTau0Nn = 2 };
@@ -192,6 +194,7 @@ struct LsShape {
     int prefetch;
     int construct_apb;   // construction: ants per block with the candidate table in LDS, 0 = off
     int row16;           // local-search rows as exact 16-bit integers (integer distances <= 65535)
+    int fused = 0;       // one fused launch per iteration (k_iteration_fused)
 };
 
 // ---- host launchers (aco_kernels.hip, aco_local_search.hip) ----
@@ -212,6 +215,12 @@ hipError_t launch_pheromone_init(const DevBatch &b, int mode, float tau0, hipStr
 hipError_t launch_stamp_clock(const DevBatch &b, hipStream_t st);
 size_t construct_lds_bytes(int n, int selection);
 size_t local_search_lds_bytes(int n, int ring, int row16);
+// Fused iteration (aco_kernels.hip): candidate roulette without a route
+// length limit, W = 4 or 8 waves dividing the ants per instance.
+bool fused_iteration_ok(const DevBatch &b, int selection, int ls_flags, int ants_per_instance, const LsShape &sh);
+size_t fused_lds_bytes(int n, const LsShape &sh);
+hipError_t launch_iteration_fused(const DevBatch &b, int rng, int ls_flags, const LsShape &sh, float tick_to_s,
+                                  hipStream_t st);
 bool ls_shape_valid(const LsShape &sh);
 
 }  // namespace cvrp

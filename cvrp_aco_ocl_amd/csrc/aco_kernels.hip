@@ -5,6 +5,7 @@
 // Arithmetic contract: compiled with -ffp-contract=off, IEEE division and
 // denormals kept, so every float result equals the CPU restatement's.
 #include "aco_device.h"
+#include "aco_ls.h"
 
 namespace cvrp {
 
@@ -281,20 +282,15 @@ __global__ __launch_bounds__(64) void k_construct_iroulette(DevBatch bt)
 // r = u * wheel[k-1] and the first lane whose prefix exceeds r wins (the
 // last lane with mass when rounding leaves none).  No mass -> full-row
 // greedy pick, and the depot when nothing is feasible.
+// One ant by one wavefront (lane = its lane); L holds the instance's
+// demands and depot distances and this ant's visited flags (depot set).
+// The step loop synchronises the wave only, so the waves of a block may
+// build different ants.
 template <bool HAS_D, int RNG>
-__global__ __launch_bounds__(64) void k_construct_knn(DevBatch bt)
+__device__ void construct_knn_ant(const DevProblem &p, const DevState &s, int a, const AntLds &L, int lane)
 {
-    extern __shared__ __align__(16) unsigned char smem[];
-    ANT_PROLOGUE
-    const int n = p.n, k = p.k, lane = threadIdx.x;
+    const int n = p.n, k = p.k;
     const u64 seed = p.seed;
-    AntLds L = ant_lds(smem, n, false);
-    for (int i = lane; i < n; i += kLanes) {
-        L.dem[i] = p.dem[i];
-        L.d0[i] = p.d0[i];
-        L.vis[i] = i == 0;
-    }
-    __syncthreads();
     const u32 iter = (u32)s.ctrl[kCtrlIter];
     int z = RNG == kRngParkMiller ? s.pm[a * kLanes] : 0;  // lane 0's stream, held by all lanes
     int *tour = s.tours + (size_t)a * p.T;
@@ -350,7 +346,9 @@ __global__ __launch_bounds__(64) void k_construct_knn(DevBatch bt)
         tdist = tdist + dcw;
         tour_put(tour, tbuf, step, w, lane);
         cur = w;
-        __syncthreads();
+        // lane 0's flag is read by the other lanes next step: LDS accesses of
+        // one wave complete in order
+        __builtin_amdgcn_wave_barrier();
     }
     if (HAS_D) tdist = tdist + L.d0[cur];
     step++;
@@ -368,6 +366,21 @@ __global__ __launch_bounds__(64) void k_construct_knn(DevBatch bt)
     }
 }
 
+template <bool HAS_D, int RNG>
+__global__ __launch_bounds__(64) void k_construct_knn(DevBatch bt)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    ANT_PROLOGUE
+    const int n = p.n, lane = threadIdx.x;
+    AntLds L = ant_lds(smem, n, false);
+    for (int i = lane; i < n; i += kLanes) {
+        L.dem[i] = p.dem[i];
+        L.d0[i] = p.d0[i];
+        L.vis[i] = i == 0;
+    }
+    __syncthreads();
+    construct_knn_ant<HAS_D, RNG>(p, s, a, L, lane);
+}
 
 // The same roulette with the instance's candidate table staged in LDS, for
 // instances without a route-length limit and with 16-bit demands: A ants
@@ -511,34 +524,38 @@ hipError_t launch_construct(const DevBatch &b, int selection, int rng, int apb, 
 // --------------------------------------------------------- statistics ----
 
 constexpr int kStatThreads = 1024;
-constexpr int kStatWaves = kStatThreads / kLanes;
 
+// Block reductions over NT threads (NT / 64 waves); red holds NT/64 + 1.
+template <int NT>
 __device__ u64 block_min_u64(u64 v, u64 *red)
 {
+    constexpr int NW = NT / kLanes;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     v = wave_min_u64(v);
     if (lane == 0) red[wv] = v;
     __syncthreads();
     if (wv == 0) {
-        u64 x = lane < kStatWaves ? red[lane] : ~0ull;
+        u64 x = lane < NW ? red[lane] : ~0ull;
         x = wave_min_u64(x);
-        if (lane == 0) red[kStatWaves] = x;
+        if (lane == 0) red[NW] = x;
     }
     __syncthreads();
-    const u64 r = red[kStatWaves];
+    const u64 r = red[NW];
     __syncthreads();
     return r;
 }
 
+template <int NT>
 __device__ long long block_sum_i64(long long v, long long *red)
 {
+    constexpr int NW = NT / kLanes;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
     if (lane == 0) red[wv] = v;
     __syncthreads();
     long long r = 0;
-    for (int w = 0; w < kStatWaves; ++w) r += red[w];
+    for (int w = 0; w < NW; ++w) r += red[w];
     __syncthreads();
     return r;
 }
@@ -548,10 +565,11 @@ __device__ long long block_sum_i64(long long v, long long *red)
 // (kernel.cl:546-586) stamped with the device clock; then get_elites
 // (kernel.cl:717-741), the deposit amounts (kernel.cl:767-768) and the
 // elite edge tables the pheromone pass reads; finally the iteration count.
-__global__ __launch_bounds__(kStatThreads) void k_statistics(DevBatch bt, float tick_to_s)
+// One instance by the NT threads of the calling block.
+template <int NT>
+__device__ void statistics_instance(const DevProblem &p, const DevState &s, float tick_to_s)
 {
-    const DevProblem p = bt.probs[blockIdx.x];
-    const DevState s = bt.states[blockIdx.x];
+    constexpr int kStatThreads = NT, kStatWaves = NT / kLanes;
     __shared__ u64 red[kStatWaves + 1];
     __shared__ long long red64[kStatWaves];
     __shared__ int sh_elite[kMaxRas];
@@ -563,7 +581,7 @@ __global__ __launch_bounds__(kStatThreads) void k_statistics(DevBatch bt, float 
         const u64 key = ((u64)f2u(s.lens[a]) << 32) | (u64)((a & ~63) | p.rank64[a & 63]);
         kmin = key < kmin ? key : kmin;
     }
-    kmin = block_min_u64(kmin, red);
+    kmin = block_min_u64<NT>(kmin, red);
     const u32 lo = (u32)(kmin & 0xFFFFFFFFull);
     const int best = (int)((lo & ~63u) | (u32)p.rank64inv[lo & 63]);
     const float best_len = s.lens[best];
@@ -611,7 +629,7 @@ __global__ __launch_bounds__(kStatThreads) void k_statistics(DevBatch bt, float 
             const u64 key = ((u64)f2u(v) << 32) | (u64)a;
             km = key < km ? key : km;
         }
-        km = block_min_u64(km, red);
+        km = block_min_u64<NT>(km, red);
         if (tid == 0) sh_elite[g] = (km >> 32) == 0x7f800000ull ? 0 : (int)(km & 0xFFFFFFFFull);
         __syncthreads();
     }
@@ -641,9 +659,9 @@ __global__ __launch_bounds__(kStatThreads) void k_statistics(DevBatch bt, float 
         mv += s.ant_moves[a];
         fb += s.ant_fb[a];
     }
-    ev = block_sum_i64(ev, red64);
-    mv = block_sum_i64(mv, red64);
-    fb = block_sum_i64(fb, red64);
+    ev = block_sum_i64<NT>(ev, red64);
+    mv = block_sum_i64<NT>(mv, red64);
+    fb = block_sum_i64<NT>(fb, red64);
     if (tid == 0) {
         s.totals[kTotEvals] += ev;
         s.totals[kTotMoves] += mv;
@@ -651,6 +669,11 @@ __global__ __launch_bounds__(kStatThreads) void k_statistics(DevBatch bt, float 
         s.totals[kTotIters] += 1;
         s.ctrl[kCtrlIter] += 1;
     }
+}
+
+__global__ __launch_bounds__(kStatThreads) void k_statistics(DevBatch bt, float tick_to_s)
+{
+    statistics_instance<kStatThreads>(bt.probs[blockIdx.x], bt.states[blockIdx.x], tick_to_s);
 }
 
 hipError_t launch_statistics(const DevBatch &b, float tick_to_s, hipStream_t st)
@@ -668,14 +691,13 @@ hipError_t launch_statistics(const DevBatch &b, float tick_to_s, hipStream_t st)
 //   t = (1-rho)*tau;  for g in order: if edge (j,h) is in elite g: t += d_g
 //   tau = t;  total_info = t^alpha * heur
 // The elite-order sum is deterministic (the reference's CAS atomics are not).
-__global__ __launch_bounds__(256) void k_pheromone_update(DevBatch bt)
+// Work item `wave` (row j, 64 float4 of it) of one instance's pass.
+__device__ __forceinline__ int pheromone_items(const DevProblem &p) { return p.n * ((p.P / 4 + 63) / 64); }
+
+__device__ __forceinline__ void pheromone_item(const DevProblem &p, const DevState &s, int wave, int lane)
 {
-    const DevProblem p = bt.probs[blockIdx.y];
-    const DevState s = bt.states[blockIdx.y];
     const float evap = 1.0f - p.rho;
-    const int lane = threadIdx.x & 63;
     const int chunks = (p.P / 4 + 63) / 64;
-    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int j = wave / chunks;
     if (j >= p.n) return;
     const int c4 = (wave - j * chunks) * 64 + lane;
@@ -719,6 +741,12 @@ __global__ __launch_bounds__(256) void k_pheromone_update(DevBatch bt)
     }
     *reinterpret_cast<float4 *>(s.tau + off) = make_float4(v[0], v[1], v[2], v[3]);
     *reinterpret_cast<float4 *>(s.tinfo + off) = make_float4(ti[0], ti[1], ti[2], ti[3]);
+}
+
+__global__ __launch_bounds__(256) void k_pheromone_update(DevBatch bt)
+{
+    pheromone_item(bt.probs[blockIdx.y], bt.states[blockIdx.y], blockIdx.x * 4 + (threadIdx.x >> 6),
+                   threadIdx.x & 63);
 }
 
 hipError_t launch_pheromone_update(const DevBatch &b, hipStream_t st)
@@ -770,6 +798,126 @@ hipError_t launch_pheromone_init(const DevBatch &b, int mode, float tau0, hipStr
     int blocks = (int)((total + 255) / 256);
     if (blocks > 1024) blocks = 1024;
     hipLaunchKernelGGL(k_pheromone_init, dim3(blocks, b.B), dim3(256), 0, st, b, mode, tau0);
+    return hipGetLastError();
+}
+
+// --------------------------------------------------- fused iteration ----
+
+// One launch per iteration (candidate roulette, no route-length limit):
+// blocks of W waves take work items from a queue -- W consecutive ants of
+// one instance, the instances in the local-search size-class order (largest
+// first).  The W waves construct the item's ants, one each
+// (construct_knn_ant), then search them one after another with the whole
+// block (local_search_ant).  The block that finishes an instance's last
+// item -- a device-scope counter per instance, with release/acquire fences
+// around it -- runs that instance's statistics, pheromone pass and
+// candidate gather, the same functions the per-phase kernels run.  No block
+// waits for another; every block leaves when the queue is empty.
+__global__ void k_fused_reset(DevBatch bt)
+{
+    if (threadIdx.x == 0) *bt.fq_next = 0;
+}
+
+template <int W, typename RT, int RNG>
+// (at least 4 waves per SIMD: with construction, search and the last
+// block's passes in one kernel the compiler would otherwise keep enough
+// loop-invariant values live to allow only 2)
+__global__ __launch_bounds__(W * 64, 4) void k_iteration_fused(DevBatch bt, int flags, int ring, int prefetch,
+                                                           float tick_to_s)
+{
+    constexpr int NT = W * 64;
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int sh_item, sh_last;
+    const int tid = threadIdx.x, lane = tid & 63, wv = wave_id();
+    const int items = bt.ants / W;
+    for (;;) {
+        if (tid == 0) sh_item = atomicAdd(bt.fq_next, 1);
+        __syncthreads();
+        // uniform, so the instance's descriptors load into scalar registers
+        const int it = __builtin_amdgcn_readfirstlane(sh_item);
+        __syncthreads();
+        if (it >= items) break;
+        const int g0 = bt.ls_order[it * W];
+        const int inst = bt.ant_inst[g0];
+        const DevProblem p = bt.probs[inst];
+        const DevState s = bt.states[inst];
+        const int a0 = g0 - bt.ant_base[inst];
+        {
+            const int n = p.n;
+            AntLds L;
+            L.dem = reinterpret_cast<int *>(smem);
+            L.d0 = reinterpret_cast<float *>(L.dem + n);
+            L.rank = nullptr;
+            L.vis = reinterpret_cast<unsigned char *>(L.d0 + n) + wv * ((n + 15) & ~15);
+            for (int i = tid; i < n; i += NT) {
+                L.dem[i] = p.dem[i];
+                L.d0[i] = p.d0[i];
+            }
+            for (int i = lane; i < n; i += kLanes) L.vis[i] = i == 0;
+            __syncthreads();
+            construct_knn_ant<false, RNG>(p, s, a0 + wv, L, lane);
+            __syncthreads();
+        }
+#pragma unroll 1
+        for (int q = 0; q < W; ++q) {
+            local_search_ant<W, RT>(p, s, a0 + q, flags, ring, prefetch, smem);
+            __syncthreads();
+        }
+        __threadfence();   // this item's tours, lengths and counters, device-wide
+        if (tid == 0) sh_last = atomicAdd(s.ctrl + kCtrlDone, W) + W == p.m;
+        __syncthreads();
+        if (__builtin_amdgcn_readfirstlane(sh_last)) {
+            __threadfence();   // every other item of the instance
+            statistics_instance<NT>(p, s, tick_to_s);
+            __syncthreads();
+            const int np = pheromone_items(p);
+#pragma unroll 1
+            for (int w = wv; w < np; w += W) pheromone_item(p, s, w, lane);
+            __syncthreads();
+            for (int e = tid; e < p.n * p.k; e += NT) s.tiknn[e] = s.tinfo[(size_t)(e / p.k) * p.P + p.nnk[e]];
+            if (tid == 0) s.ctrl[kCtrlDone] = 0;
+        }
+        __syncthreads();
+    }
+}
+
+bool fused_iteration_ok(const DevBatch &b, int selection, int ls_flags, int ants_per_instance, const LsShape &sh)
+{
+    return selection == kSelKnn && !b.has_d && ls_flags != 0 && (sh.waves == 4 || sh.waves == 8) &&
+           ants_per_instance % sh.waves == 0 && ls_shape_valid(sh);
+}
+
+size_t fused_lds_bytes(int n, const LsShape &sh)
+{
+    const size_t cons = (size_t)n * 8 + (size_t)sh.waves * ((n + 15) & ~15) + 16;
+    const size_t ls = local_search_lds_bytes(n, sh.ring, sh.row16);
+    return cons > ls ? cons : ls;
+}
+
+hipError_t launch_iteration_fused(const DevBatch &b, int rng, int ls_flags, const LsShape &sh, float tick_to_s,
+                                  hipStream_t st)
+{
+    hipLaunchKernelGGL(k_fused_reset, dim3(1), dim3(64), 0, st, b);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    int dev = 0, cus = 0;
+    if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+    if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
+    const size_t lds = fused_lds_bytes(b.max_n, sh);
+    // enough blocks to fill the device; each leaves when the queue is empty
+    const int items = b.ants / sh.waves;
+    int blocks = cus * (sh.waves == 4 ? 4 : 2);
+    if (blocks > items) blocks = items;
+#define KF(W, RT, R) hipLaunchKernelGGL((k_iteration_fused<W, RT, R>), dim3(blocks), dim3(W * kLanes), lds, st, b, \
+                                        ls_flags, sh.ring, sh.prefetch, tick_to_s)
+    if (rng == kRngPhilox) {
+        if (sh.waves == 8) { if (sh.row16) KF(8, unsigned short, kRngPhilox); else KF(8, float, kRngPhilox); }
+        else { if (sh.row16) KF(4, unsigned short, kRngPhilox); else KF(4, float, kRngPhilox); }
+    } else {
+        if (sh.waves == 8) { if (sh.row16) KF(8, unsigned short, kRngParkMiller); else KF(8, float, kRngParkMiller); }
+        else { if (sh.row16) KF(4, unsigned short, kRngParkMiller); else KF(4, float, kRngParkMiller); }
+    }
+#undef KF
     return hipGetLastError();
 }
 

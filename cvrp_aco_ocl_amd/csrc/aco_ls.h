@@ -1,0 +1,1031 @@
+// Local search device code (gfx950): one block of W wavefronts per ant; the
+// ant's tour, per-route tables and a ring of distance rows in LDS.
+//
+//   2-opt     the reference's GPU 2-opt (kernel.cl:126-319): per route,
+//             sequential scan order, first improvement over the nn list
+//             with radius pruning, don't-look bits.  Routes are independent
+//             (the depot's don't-look bit starts clear on route 0 and set on
+//             later routes); a group of 8 lanes searches one route.
+//   swap      LocalSearch::swap (localSearch.cpp:301-410): for each position
+//             i in order, the first j > i whose swap gains more than
+//             EPSILON and keeps loads and route lengths feasible.
+//   relocate  the same scan for insertion moves (neighbourSearch.cpp:280-293,
+//             move.cpp:32-51).
+//
+// swap and relocate run speculatively on W consecutive positions: wave w
+// scans position i0+w against the current tour (4 x 64 j per loop trip).
+// The smallest position with a move is exactly the sequential scan's next
+// move, because no earlier position moved; results of later positions are
+// discarded.  Distance rows of the positions around i0 sit in an LDS ring
+// and the next round's rows are fetched while the current round scans.
+//
+// Launch shape (LsShape): W (4, 8, 12 or 16), the ring size, prefetch on/off
+// and the row type; every shape gives the same tours and the same
+// evaluated-move counts.
+#pragma once
+
+#include "aco_device.h"
+
+namespace cvrp {
+
+using u64 = unsigned long long;
+
+// LDS words of L.res for W waves (W <= 16): [0,W) and [W,2W) per-wave scan
+// results (accepted j) of even and odd rounds, [2W,4W) the moves each wave's
+// scan evaluated up to its result (same parity split), 4W the route count,
+// 4W+1 the lowest wave of the current round with a move (round key).
+constexpr int kResWords = 80;
+
+// Candidates of one scan trip, and the padding of the LDS tour and edge
+// arrays past the tour's end that lets a trip read without clamping.
+constexpr int kScanUnroll = 4;
+constexpr int kScanPad = kScanUnroll * 64 + 4;
+
+template <int W> struct LsW {
+    static_assert(W == 4 || W == 8 || W == 12 || W == 16, "4, 8, 12 or 16 waves per ant");
+    static constexpr int threads = W * kLanes;
+    static constexpr int res_ev = 2 * W;
+    static constexpr int res_r = 4 * W;
+    static constexpr int res_first = 4 * W + 1;
+};
+
+struct LsLds {
+    int *tour, *tmp, *rl, *dem, *res;
+    float *dtw, *tmpf, *rd, *rows;
+    short *pos, *rof, *ridx, *zpos, *rc;
+    unsigned char *dlb;
+};
+
+__host__ __device__ constexpr size_t ls_align16(size_t x) { return (x + 15) & ~size_t(15); }
+
+struct LsOffsets {
+    size_t tour, tmp, dtw, tmpf, rl, rd, dem, rows, res, pos, rof, ridx, zpos, rc, dlb, total;
+};
+
+__host__ __device__ inline LsOffsets ls_offsets(int n, int ring, int rowbytes)
+{
+    const size_t T = 2 * (size_t)n, N = (size_t)n;
+    LsOffsets o;
+    size_t at = 0;
+    o.tour = at; at = ls_align16(at + 4 * (T + kScanPad));
+    o.tmp = at;  at = ls_align16(at + 4 * T);
+    o.dtw = at;  at = ls_align16(at + 4 * (T + kScanPad));
+    o.tmpf = at; at = ls_align16(at + 4 * T);
+    o.rl = at;   at = ls_align16(at + 4 * N);
+    o.rd = at;   at = ls_align16(at + 4 * N);
+    o.dem = at;  at = ls_align16(at + 4 * N);
+    o.rows = at; at = ls_align16(at + (size_t)rowbytes * (size_t)(rowbytes == 4 ? n : (n + 1) & ~1) * (size_t)ring);
+    o.res = at;  at = ls_align16(at + 4 * kResWords);
+    o.pos = at;  at = ls_align16(at + 2 * N);
+    o.rof = at;  at = ls_align16(at + 2 * N);
+    o.ridx = at; at = ls_align16(at + 2 * T);
+    o.zpos = at; at = ls_align16(at + 2 * (N + 1));
+    o.rc = at;   at = ls_align16(at + 2 * N);
+    o.dlb = at;  at = ls_align16(at + N);
+    o.total = at;
+    return o;
+}
+
+__device__ __forceinline__ LsLds ls_bind(int n, int ring, int rowbytes, unsigned char *b)
+{
+    const LsOffsets o = ls_offsets(n, ring, rowbytes);
+    LsLds L;
+    L.tour = (int *)(b + o.tour); L.tmp = (int *)(b + o.tmp); L.dtw = (float *)(b + o.dtw);
+    L.tmpf = (float *)(b + o.tmpf);
+    L.rl = (int *)(b + o.rl); L.rd = (float *)(b + o.rd); L.dem = (int *)(b + o.dem);
+    L.rows = (float *)(b + o.rows); L.res = (int *)(b + o.res);
+    L.pos = (short *)(b + o.pos); L.rof = (short *)(b + o.rof); L.ridx = (short *)(b + o.ridx);
+    L.zpos = (short *)(b + o.zpos); L.rc = (short *)(b + o.rc); L.dlb = b + o.dlb;
+    return L;
+}
+
+
+__device__ __forceinline__ float Dd(const DevProblem &p, int a, int b)
+{
+    return p.dist[(size_t)a * p.P + b];
+}
+
+// dtw[q] = d(tour[q], tour[q+1]) for q in [lo, hi]
+template <int W>
+__device__ __forceinline__ void ls_edges(const DevProblem &p, LsLds &L, int lo, int hi, int tid)
+{
+    for (int q = lo + tid; q <= hi; q += LsW<W>::threads) L.dtw[q] = Dd(p, L.tour[q], L.tour[q + 1]);
+}
+
+// Wave 0: depot positions (zpos), customer positions and route ids (pos,
+// rof) and route index per position (ridx, counted as localSearch.cpp:319-337
+// does).  Then, with tables, the per-route load / distance+service /
+// customer count, one thread per route.  Returns R.
+template <int W>
+__device__ __forceinline__ int ls_routes(const DevProblem &p, LsLds &L, int S, int tid, bool tables)
+{
+    if (tid < kLanes) {
+        const int lane = tid;
+        int zc = 0;
+        for (int base = 0; base < S; base += kLanes) {
+            const int q = base + lane;
+            const int v = q < S ? L.tour[q] : -1;
+            const u64 mz = __ballot(v == 0);
+            const int before = zc + __popcll(mz & ((1ull << lane) - 1ull));
+            if (q < S) {
+                if (v == 0) {
+                    L.zpos[before] = (short)q;
+                } else {
+                    L.pos[v] = (short)q;
+                    L.rof[v] = (short)(before - 1);
+                }
+                L.ridx[q] = (short)(before > 0 ? before - 1 : 0);
+            }
+            zc += __popcll(mz);
+        }
+        if (lane == 0) L.res[LsW<W>::res_r] = zc - 1;
+    }
+    __syncthreads();
+    const int R = L.res[LsW<W>::res_r];
+    if (tables) {
+        for (int r = tid; r < R; r += LsW<W>::threads) {
+            const int beg = L.zpos[r], end = L.zpos[r + 1];
+            int load = 0;
+            float dist = 0.0f;
+            for (int i = beg + 1; i <= end; ++i) {
+                load += L.dem[L.tour[i]];
+                dist += L.dtw[i - 1];
+            }
+            L.rl[r] = load;
+            L.rd[r] = dist + p.svc * (float)(end - beg - 1);
+            L.rc[r] = (short)(end - beg - 1);
+        }
+    }
+    __syncthreads();
+    return R;
+}
+
+// ------------------------------------------------------------- 2-opt ----
+
+// A route is searched by a group of kOptGroup lanes of one wave: the
+// candidate scan is split over the lanes and reduced; moves are applied
+// with the reversals split over the lanes.
+constexpr int kOptGroup = 8;
+
+// Minimum over the aligned group of 8 lanes (lane ^ 1, ^ 2, ^ 4 stay inside
+// the group, all of whose lanes are active).
+__device__ __forceinline__ u64 group_min_u64(u64 v)
+{
+    static_assert(kOptGroup == 8, "the DPP exchanges cover groups of 8 lanes");
+    const int lane = (int)(threadIdx.x & 63);
+    u64 w = dpp_u64<kDppXor1>(v);
+    v = w < v ? w : v;
+    w = dpp_u64<kDppXor2>(v);
+    v = w < v ? w : v;
+    w = dpp_xor4_u64(v, lane);
+    return w < v ? w : v;
+}
+
+__device__ __forceinline__ void group_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// One scan for n1 (SUCC: partner edges (n1,s1),(n2,s2); otherwise
+// (p1,n1),(p2,n2); nb1 is s1 or p1).  The reference walks n1's nn row,
+// skips nodes of other routes and stops at the first in-route node with
+// radius - d <= EPS (kernel.cl:180-230).  Rows are sorted by distance, so
+// the nodes it evaluates are exactly the in-route nodes with
+// radius - d > EPS that appear in the row, and the move it applies is the
+// improving one earliest in the row.  Here the route's own positions are
+// scanned (independent loads, no walk over other routes' neighbours) and
+// the improving candidate with the smallest row position nnpos[n1][n2] is
+// kept (row positions are distinct, so the group minimum is unique).
+// Returns true with h1..h4 set on an improving move (same in all lanes of
+// the group).
+// Candidate q of a scan (q may lie past rend: the caller masks it).  Loads
+// are unconditional (clamped, in-bounds indices) so a lane's candidates
+// have all their loads in flight at once.  ok: in the row (rp < nn_ls),
+// inside the radius and, for the predecessor variant, not adjacent; gain as
+// the reference computes it; x is s2 or p2.
+template <bool SUCC>
+__device__ __forceinline__ void two_opt_cand(const DevProblem &p, const LsLds &L, int rend, int pos1,
+                                             int q, int n1, int nb1, float radius,
+                                             const unsigned short *rowpos, bool &ok, float &gain,
+                                             int &rp, int &n2, int &x)
+{
+    const int qq = q <= rend ? q : rend;
+    n2 = L.tour[qq];
+    x = SUCC ? (qq == rend ? 0 : L.tour[qq + 1]) : L.tour[qq - 1];
+    const float ew = SUCC ? L.dtw[qq] : L.dtw[qq - 1];
+    rp = rowpos[n2];
+    const float dn = Dd(p, n1, n2);
+    const float dx = Dd(p, nb1, x);
+    ok = q <= rend && q != pos1 && rp < p.nn_ls && radius - dn > kEps && (SUCC || !(x == n1 || nb1 == n2));
+    gain = -radius + dn + dx - ew;
+}
+
+// Candidates per lane per trip of a 2-opt scan: positions q, q+8, q+16 and
+// q+24 of the group's lanes, all loads issued before any is used (routes of
+// up to 32 customers take one trip, i.e. one memory latency).
+constexpr int kOptUnroll = 4;
+
+template <bool SUCC>
+__device__ __forceinline__ bool two_opt_scan(const DevProblem &p, const LsLds &L, int rbeg,
+                                             int rend, int pos1, int n1, int nb1, float radius,
+                                             int gl, long long &evals, int &h1, int &h2, int &h3,
+                                             int &h4)
+{
+    constexpr int U = kOptUnroll;
+    const unsigned short *rowpos = p.nnpos + (size_t)n1 * p.n;
+    u64 best = ~0ull;
+    int cnt = 0;
+    // the first trip's candidates, kept for the evaluated-move count
+    bool okk[U];
+    int rpk[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) { okk[u] = false; rpk[u] = 0; }
+    for (int q0 = rbeg + 1 + gl; q0 <= rend; q0 += U * kOptGroup) {
+        bool ok[U];
+        float g[U];
+        int rp[U], a[U], x[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            two_opt_cand<SUCC>(p, L, rend, pos1, q0 + u * kOptGroup, n1, nb1, radius, rowpos, ok[u], g[u],
+                               rp[u], a[u], x[u]);
+        const bool first = q0 == rbeg + 1 + gl;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            cnt += ok[u];
+            if (ok[u] && g[u] < -kEps) {
+                const u64 key = ((u64)rp[u] << 32) | ((u64)a[u] << 16) | (u64)x[u];
+                best = key < best ? key : best;
+            }
+            if (first) { okk[u] = ok[u]; rpk[u] = rp[u]; }
+        }
+    }
+    best = group_min_u64(best);
+    if (best == ~0ull) {
+        evals += cnt;
+        return false;
+    }
+    // the reference's walk stops at the accepted candidate: it evaluated
+    // the candidates up to that row position
+    const int rps = (int)(best >> 32);
+    cnt = 0;
+    if (rend - rbeg <= U * kOptGroup) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) cnt += okk[u] && rpk[u] <= rps;
+    } else {
+        for (int q0 = rbeg + 1 + gl; q0 <= rend; q0 += U * kOptGroup) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                bool ok;
+                float g;
+                int rp, a, x;
+                two_opt_cand<SUCC>(p, L, rend, pos1, q0 + u * kOptGroup, n1, nb1, radius, rowpos, ok, g, rp,
+                                   a, x);
+                cnt += ok && rp <= rps;
+            }
+        }
+    }
+    evals += cnt;
+    const int b2 = (int)((best >> 16) & 0xFFFF), bx = (int)(best & 0xFFFF);
+    if (SUCC) { h1 = n1; h2 = nb1; h3 = b2; h4 = bx; }
+    else { h1 = nb1; h2 = n1; h3 = bx; h4 = b2; }
+    return true;
+}
+
+// kernel.cl:126-277 for route r at positions [rbeg, rend] (rbeg holds the
+// depot), run by the kOptGroup lanes gl = 0.. of one group.
+__device__ __forceinline__ void two_opt_route(const DevProblem &p, LsLds &L, int r, int rbeg,
+                                              int rend, int gl, long long &evals, int &moves_total)
+{
+    bool dlb0 = r > 0;
+    const int num = rend - rbeg + 1;
+    int moves = 0;
+    bool imp = true;
+    while (imp) {
+        imp = false;
+        for (int l = 0; l < num; ++l) {
+            const int pos1 = rbeg + l;
+            const int n1 = L.tour[pos1];
+            if (n1 == 0 ? dlb0 : (L.dlb[n1] != 0)) continue;
+            int h1 = 0, h2 = 0, h3 = 0, h4 = 0;
+            const int s1 = pos1 == rend ? 0 : L.tour[pos1 + 1];
+            // d(n1, s1) is the edge leaving position pos1
+            bool found = two_opt_scan<true>(p, L, rbeg, rend, pos1, n1, s1, L.dtw[pos1], gl, evals,
+                                            h1, h2, h3, h4);
+            if (!found) {
+                const int p1 = pos1 == rbeg ? L.tour[rend] : L.tour[pos1 - 1];
+                // d(p1, n1): the edge entering pos1, or the closing edge
+                // (tour[rend], depot) when n1 is the depot
+                const float rad = pos1 == rbeg ? L.dtw[rend] : L.dtw[pos1 - 1];
+                found = two_opt_scan<false>(p, L, rbeg, rend, pos1, n1, p1, rad, gl, evals, h1, h2,
+                                            h3, h4);
+            }
+            if (!found) {
+                if (n1 == 0) dlb0 = true;
+                else if (gl == 0) L.dlb[n1] = 1;
+                group_sync();
+                continue;
+            }
+            imp = true;
+            const int hs[4] = {h1, h2, h3, h4};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (hs[q] == 0) dlb0 = false;
+                else if (gl == 0) L.dlb[hs[q]] = 0;
+            }
+            int ph1 = h1 == 0 ? rbeg : L.pos[h1];
+            int ph3 = h3 == 0 ? rbeg : L.pos[h3];
+            if (ph3 < ph1) {
+                int t = h1; h1 = h3; h3 = t;
+                t = h2; h2 = h4; h4 = t;
+                t = ph1; ph1 = ph3; ph3 = t;
+            }
+            const int tb = h2 == 0 ? rbeg : L.pos[h2];
+            group_sync();
+            // edge weights: the two new edges, the segment between reversed
+            if (gl == 0) {
+                L.dtw[ph1] = Dd(p, h1, h3);
+                L.dtw[ph3] = Dd(p, h2, h4);
+            }
+            const int ed = (ph3 - ph1 - 1) / 2;
+            for (int t = gl; t < ed; t += kOptGroup) {
+                const int i = ph1 + 1 + t, j = ph3 - 1 - t;
+                const float a = L.dtw[i];
+                L.dtw[i] = L.dtw[j];
+                L.dtw[j] = a;
+            }
+            // tour positions [tb, ph3] reversed
+            const int tp = (ph3 - tb + 1) / 2;
+            for (int t = gl; t < tp; t += kOptGroup) {
+                const int i = tb + t, j = ph3 - t;
+                const int a = L.tour[i], b = L.tour[j];
+                L.tour[i] = b;
+                L.tour[j] = a;
+                L.pos[a] = (short)j;
+                L.pos[b] = (short)i;
+            }
+            group_sync();
+            if (++moves >= kMaxMovesPerRoute) break;
+        }
+        if (moves >= kMaxMovesPerRoute) break;
+    }
+    if (gl == 0) moves_total += moves;
+}
+
+// ------------------------------------------------- distance row cache ----
+
+// Rows d(tour[q], .) live in slot q mod R of an LDS ring of R rows (R = 8,
+// 16 or 32, a launch parameter; a round needs W+2 positions, W+2 more with
+// prefetch).  The window of positions [base, base + R) maps one to one onto the
+// slots; bit o of a validity mask says that the slot of position base + o
+// holds the row of the node now at that position.  Moving the window shifts
+// the mask, a move clears the bits of the positions it changed.  All of it
+// is uniform (every thread holds the same mask), so checks are a few scalar
+// instructions and need no LDS round trip.
+struct RowWindow {
+    int base;
+    unsigned mask;
+};
+
+// Rows hold fp32 distances, or (when every distance of the context is an
+// integer in [0, 65535]) the same values as exact 16-bit integers, copied
+// from the 16-bit distance matrix.
+__device__ __forceinline__ float rget(const float *r, int i) { return r[i]; }
+__device__ __forceinline__ float rget(const unsigned short *r, int i) { return (float)r[i]; }
+
+template <typename RT> struct RowRing {
+    RT *rows;
+    int mask;     // ring size - 1
+    int stride;   // elements per row slot: n rounded up to a whole number of dwords
+    __device__ __forceinline__ int slot(int pos) const { return pos & mask; }
+    __device__ __forceinline__ RT *at(int pos) const { return rows + slot(pos) * stride; }
+};
+
+template <typename RT> __host__ __device__ constexpr int row_stride(int n)
+{
+    return sizeof(RT) == 4 ? n : (n + 1) & ~1;
+}
+
+// Source rows in global memory: fp32 distances, or their 16-bit copy.
+__device__ __forceinline__ const float *row_src(const DevProblem &p, int v, float *) { return p.dist + (size_t)v * p.P; }
+__device__ __forceinline__ const unsigned short *row_src(const DevProblem &p, int v, unsigned short *)
+{
+    return p.dist16 + (size_t)v * p.P;
+}
+
+// Copy distance row v into LDS with direct global->LDS loads issued by one
+// wave (one dword per lane per instruction, no registers; completes with
+// vmcnt, so readers wait for vmcnt(0) and a barrier).  Rows of both types
+// start dword aligned and the copy may run up to one padding element past
+// n (within P and within the slot stride).
+template <typename RT>
+__device__ __forceinline__ void row_copy_wave(const DevProblem &p, int v, RT *dst, int lane)
+{
+    const int nd = (p.n * (int)sizeof(RT) + 3) >> 2;
+    const unsigned *src = reinterpret_cast<const unsigned *>(row_src(p, v, (RT *)nullptr));
+    unsigned *d = reinterpret_cast<unsigned *>(dst);
+    for (int c = 0; c < nd; c += kLanes) {
+        if (c + lane < nd)
+            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(src + c + lane),
+                                             (__attribute__((address_space(3))) void *)(d + c), 4, 0, 0);
+    }
+}
+
+// Copy the rows of the window offsets in `need` (bit q: position base + q):
+// the k-th of them is issued by wave k mod W.
+template <int W, typename RT>
+__device__ __forceinline__ void rows_issue(const DevProblem &p, const LsLds &L, const RowRing<RT> &rr,
+                                           int base, unsigned need, int wave, int lane)
+{
+    int k = 0;
+    for (unsigned m = need; m; m &= m - 1u, ++k) {
+        if (k % W != wave) continue;
+        const int pos = base + __builtin_ctz(m);
+        row_copy_wave<RT>(p, __builtin_amdgcn_readfirstlane(L.tour[pos]), rr.at(pos), lane);
+    }
+}
+
+// Start the window at position base (never moves backwards within a scan).
+__device__ __forceinline__ void window_move(RowWindow &w, int base)
+{
+    const int d = base - w.base;
+    w.mask = d >= 32 ? 0u : (w.mask >> d);
+    w.base = base;
+}
+
+// Positions [lo, hi] changed: their rows are stale.
+__device__ __forceinline__ void window_clear(RowWindow &w, int lo, int hi)
+{
+    const int a = lo - w.base > 0 ? lo - w.base : 0;
+    const int b = hi - w.base < 31 ? hi - w.base : 31;
+    if (a > b) return;
+    const unsigned span = (b - a + 1) >= 32 ? 0xFFFFFFFFu : ((1u << (b - a + 1)) - 1u);
+    w.mask &= ~(span << a);
+}
+
+// Make the rows of window offsets [0, cnt) present (cnt <= W+2), spread
+// over the waves; a barrier follows when any was missing.
+template <int W, typename RT>
+__device__ __forceinline__ void rows_ensure(const DevProblem &p, const LsLds &L, const RowRing<RT> &rr,
+                                            RowWindow &w, int cnt, int tid)
+{
+    const unsigned need = ((1u << cnt) - 1u) & ~w.mask;
+    w.mask |= (1u << cnt) - 1u;
+    if (need) {
+        rows_issue<W, RT>(p, L, rr, w.base, need, wave_id(), tid & 63);
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+    }
+}
+
+// Next round's rows (window offsets [off, off+W), positions < S) are copied
+// into their ring slots before the scan (direct loads, no registers) and
+// waited for before the round's barrier.  Enabled per launch when the ring
+// holds both rounds' positions (2W+2 <= R): the positions of this round and
+// the next never share a slot.
+template <int W, typename RT>
+__device__ __forceinline__ unsigned prefetch_issue(const DevProblem &p, const LsLds &L, const RowRing<RT> &rr,
+                                                   bool on, const RowWindow &w, int off, int S, int tid)
+{
+    if (!on) return 0;
+    const int pos0 = w.base + off;
+    const int avail = S - pos0 < W ? S - pos0 : W;
+    if (avail <= 0) return 0;
+    const unsigned want = ((1u << avail) - 1u) & ~(w.mask >> off);
+    if (want) rows_issue<W, RT>(p, L, rr, pos0, want, wave_id(), tid & 63);
+    return want;
+}
+
+// Before the round's barrier: the prefetched rows have landed.
+__device__ __forceinline__ void prefetch_wait(RowWindow &w, unsigned want, int off)
+{
+    if (want) __builtin_amdgcn_s_waitcnt(0);
+    w.mask |= want << off;
+}
+
+// After a round's barrier: the first wave (in position order) whose scan
+// accepted a move, that move's j (js), and in e the moves the sequential
+// scan evaluates up to and including that wave's position (all W positions
+// when no wave moved).  One LDS read per lane; the rest is uniform.
+template <int W>
+__device__ __forceinline__ int round_pick(const int *res, const int *rese, int lane, int &js, int &e)
+{
+    const int rv = lane < W ? res[lane] : -1;
+    const int re = lane < W ? rese[lane] : 0;
+    const u64 mk = __ballot(rv >= 0);
+    const int wstar = mk ? (int)__ffsll((unsigned long long)mk) - 1 : -1;
+    const int last = wstar < 0 ? W - 1 : wstar;
+    int sum = 0;
+#pragma unroll
+    for (int q = 0; q < W; ++q)
+        if (q <= last) sum += __builtin_amdgcn_readlane(re, q);
+    e = sum;
+    js = wstar < 0 ? -1 : __builtin_amdgcn_readlane(rv, wstar);
+    return wstar;
+}
+
+// -------------------------------------------------------------- swap ----
+
+// Mask of lanes 0..f.
+__device__ __forceinline__ u64 lanes_upto(int f) { return (2ull << f) - 1ull; }
+
+// Early stop of speculative scans: a wave that accepts a move raises the
+// round key (round << 5 | 31 - wave) with an LDS max.  A wave whose key is
+// exceeded knows that a lower wave of this round has a move, so its own
+// result will be discarded, and stops scanning.  Keys of earlier rounds are
+// smaller, and no wave reaches the next round before every wave has passed
+// this round's barrier, so no reset is needed.
+struct ScanStop {
+    const unsigned *key;
+    unsigned mine;
+};
+__device__ __forceinline__ unsigned round_key(unsigned round, int w) { return (round << 5) | (unsigned)(31 - w); }
+
+// Fresh read of the round key: a volatile load through an LDS pointer (a
+// volatile load through a generic pointer would become a flat load that
+// waits on the memory counter).
+__device__ __forceinline__ unsigned read_key(const unsigned *key)
+{
+    return *(volatile const __attribute__((address_space(3))) unsigned *)(key);
+}
+
+// A trip of a scan covers kScanUnroll chunks of 64 consecutive positions j.
+// Each lane loads the node and the edge weight at its own j once; the
+// values at j-1 and j+1 come from the neighbouring lanes (wave shifts), and
+// at the chunk edges from the neighbouring chunk or, at the trip's edges,
+// from the positions before and after the trip.
+struct TripEdge {
+    float d;    // dtw[jb - 1]
+    float v;    // row value at tour[jb - 1]
+};
+
+// Swap of position i with position j (localSearch.cpp:301-410).  n2 =
+// tour[j], dm/d0 the edges (pn2, n2) and (n2, sn2), rA/rC = d(pn1, n2),
+// d(sn1, n2), rBp/rBs = d(n1, pn2), d(n1, sn2).  The gain is the
+// reference's expression for adjacent and general positions.
+__device__ __forceinline__ float swap_gain(int i, int j, float a1, float b1, float rA, float rC,
+                                           float dm, float d0, float rBp, float rBs)
+{
+    if (j == i + 1) return -(a1 + d0) + (rA + rBs);
+    return -(a1 + b1 + dm + d0) + (rA + rC + rBp + rBs);
+}
+
+// Loads and route lengths stay feasible (checked only for improving moves
+// between two routes).
+__device__ __forceinline__ bool swap_feasible(const DevProblem &p, const LsLds &L, int j, int r1, int dn1,
+                                              float a1, float b1, int n2, float rA, float rC, float dm,
+                                              float d0, float rBp, float rBs)
+{
+    const int r2 = L.ridx[j];
+    if (r1 == r2) return true;
+    const int dn2 = L.dem[n2];
+    const int load1 = L.rl[r1] - dn1 + dn2, load2 = L.rl[r2] - dn2 + dn1;
+    const float dist1 = (L.rd[r1] - (a1 + b1)) + (rA + rC);
+    const float dist2 = (L.rd[r2] - (dm + d0)) + (rBp + rBs);
+    return !(load1 > p.Q || load2 > p.Q || dist1 > p.D || dist2 > p.D);
+}
+
+// Position i against the current tour; returns the first accepted j or -1
+// and in evc the moves the sequential scan evaluates up to it.  A trip's
+// common case (no improving candidate) takes no branch per chunk; the
+// feasibility checks run only for chunks with an improving candidate.
+template <typename RT>
+__device__ __forceinline__ int swap_scan(const DevProblem &p, const LsLds &L, int S, int i,
+                                         const RT *rowA, const RT *rowB, const RT *rowC,
+                                         int lane, const ScanStop &stop, int &evc)
+{
+    constexpr int U = kScanUnroll;
+    const int n1 = L.tour[i];
+    const int sn1 = L.tour[i + 1], r1 = L.ridx[i];
+    const float a1 = rget(rowA, n1), b1 = rget(rowB, sn1);
+    const int dn1 = L.dem[n1];
+    int cnt = 0;
+    TripEdge pe = {L.dtw[i], rget(rowB, n1)};   // position jb - 1 = i
+    for (int jb = i + 1; jb < S; jb += U * kLanes) {
+        const unsigned key = read_key(stop.key);
+        int t[U];
+        float d[U], v[U];
+        // past the tour's end the padding reads as depots and zero edges;
+        // every position j <= S-2 that can be accepted sees its true
+        // neighbours
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            t[u] = L.tour[jb + u * kLanes + lane];
+            d[u] = L.dtw[jb + u * kLanes + lane];
+        }
+        const int ta = L.tour[jb + U * kLanes];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = rget(rowB, t[u]);
+        const float va = rget(rowB, ta);
+        float rA[U], rC[U], dm[U], rBp[U], rBs[U];
+        bool imp[U], val[U], any = false;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int j = jb + u * kLanes + lane;
+            const float dp = u == 0 ? pe.d : readlane_f32(d[u - 1], kLanes - 1);
+            const float vp = u == 0 ? pe.v : readlane_f32(v[u - 1], kLanes - 1);
+            const float vn = u == U - 1 ? va : readlane_f32(v[u + 1], 0);
+            rA[u] = rget(rowA, t[u]);
+            rC[u] = rget(rowC, t[u]);
+            dm[u] = wave_prev(d[u], dp);
+            rBp[u] = wave_prev(v[u], vp);
+            rBs[u] = wave_next(v[u], vn);
+            val[u] = j < S && t[u] != 0;
+            imp[u] = val[u] && swap_gain(i, j, a1, b1, rA[u], rC[u], dm[u], d[u], rBp[u], rBs[u]) < -kEps;
+            any = any || imp[u];
+        }
+        if (__ballot(any)) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int j = jb + u * kLanes + lane;
+                const bool ok = imp[u] && swap_feasible(p, L, j, r1, dn1, a1, b1, t[u], rA[u], rC[u], dm[u],
+                                                        d[u], rBp[u], rBs[u]);
+                const u64 mk = __ballot(ok), mv = __ballot(val[u]);
+                if (mk) {
+                    const int f = __ffsll(mk) - 1;
+                    evc = cnt + __popcll(mv & lanes_upto(f));
+                    return jb + u * kLanes + f;
+                }
+                cnt += __popcll(mv);
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < U; ++u) cnt += __popcll(__ballot(val[u]));
+        }
+        if (key > stop.mine) break;
+        pe.d = readlane_f32(d[U - 1], kLanes - 1);
+        pe.v = readlane_f32(v[U - 1], kLanes - 1);
+    }
+    evc = cnt;
+    return -1;
+}
+
+template <int W, typename RT>
+__device__ __forceinline__ void ls_swap(const DevProblem &p, LsLds &L, const RowRing<RT> &rr, bool pf_on,
+                                        int S, int tid, unsigned &round, long long &ev, int &moves)
+{
+    const int lane = tid & 63, w = wave_id();
+    const int cap = kMaxMovesFactor * p.n;
+    int done = 0, i0 = 1, parity = 0;
+    RowWindow win = {0, 0u};
+    while (i0 < S) {
+        // rows of positions i0-1 .. i0+W (pred, self, succ of each scanned i)
+        window_move(win, i0 - 1);
+        rows_ensure<W, RT>(p, L, rr, win, min(W + 2, S - i0 + 1), tid);
+        const unsigned pfw = prefetch_issue<W, RT>(p, L, rr, pf_on, win, W + 2, S, tid);
+        const int i = i0 + w;
+        int j = -1, evc = 0;
+        const ScanStop stop = {reinterpret_cast<const unsigned *>(L.res + LsW<W>::res_first),
+                               round_key(++round, w)};
+        if (i < S && L.tour[i] != 0)
+            j = swap_scan(p, L, S, i, rr.at(i - 1), rr.at(i), rr.at(i + 1), lane, stop, evc);
+        if (j >= 0 && lane == 0) atomicMax(reinterpret_cast<unsigned *>(L.res + LsW<W>::res_first), stop.mine);
+        int *res = L.res + W * parity, *rese = L.res + LsW<W>::res_ev + W * parity;
+        parity ^= 1;
+        if (lane == 0) { res[w] = j; rese[w] = evc; }
+        prefetch_wait(win, pfw, W + 2);
+        __syncthreads();
+        // the sequential scan evaluates the waves' positions in order and
+        // stops at the first accepted move
+        int js, e;
+        const int wstar = round_pick<W>(res, rese, lane, js, e);
+        if (tid == 0) ev += e;
+        if (wstar < 0) {
+            // no barrier: the next round writes the other result buffer, and
+            // rows it loads belong to positions no wave still reads
+            i0 += W;
+            continue;
+        }
+        const int is = i0 + wstar;
+        if (tid == 0) {
+            const RT *rowA = rr.at(is - 1), *rowB = rr.at(is), *rowC = rr.at(is + 1);
+            const int n1 = L.tour[is], sn1 = L.tour[is + 1], r1 = L.ridx[is];
+            const int n2 = L.tour[js], pn2 = L.tour[js - 1], sn2 = L.tour[js + 1], r2 = L.ridx[js];
+            const float a1 = rget(rowA, n1), b1 = rget(rowB, sn1);
+            const int dn1 = L.dem[n1];
+            int load1, load2;
+            float dist1, dist2;
+            if (r1 != r2) {
+                const int dn2 = L.dem[n2];
+                load1 = L.rl[r1] - dn1 + dn2;
+                load2 = L.rl[r2] - dn2 + dn1;
+                dist1 = (L.rd[r1] - (a1 + b1)) + (rget(rowA, n2) + rget(rowC, n2));
+                dist2 = (L.rd[r2] - (L.dtw[js - 1] + L.dtw[js])) + (rget(rowB, pn2) + rget(rowB, sn2));
+            } else {
+                float gain;
+                if (js == is + 1)
+                    gain = -(a1 + L.dtw[js]) + (rget(rowA, n2) + rget(rowB, sn2));
+                else
+                    gain = -(a1 + b1 + L.dtw[js - 1] + L.dtw[js]) +
+                           (rget(rowA, n2) + rget(rowC, n2) + rget(rowB, pn2) + rget(rowB, sn2));
+                load1 = L.rl[r1];
+                load2 = load1;
+                dist1 = L.rd[r1] + gain;
+                dist2 = dist1;
+            }
+            L.tour[is] = n2;
+            L.tour[js] = n1;
+            L.rl[r1] = load1; L.rl[r2] = load2;
+            L.rd[r1] = dist1; L.rd[r2] = dist2;
+            // the four new edges, from the rows of pn1, n1, sn1 (symmetric
+            // distances: the same values as the distance matrix)
+            if (js == is + 1) {
+                L.dtw[is - 1] = rget(rowA, n2);   // (pn1, n2)
+                L.dtw[is] = rget(rowB, n2);       // (n2, n1)
+                L.dtw[js] = rget(rowB, sn2);      // (n1, sn2)
+            } else {
+                L.dtw[is - 1] = rget(rowA, n2);   // (pn1, n2)
+                L.dtw[is] = rget(rowC, n2);       // (n2, sn1)
+                L.dtw[js - 1] = rget(rowB, pn2);  // (pn2, n1)
+                L.dtw[js] = rget(rowB, sn2);      // (n1, sn2)
+            }
+        }
+        __syncthreads();
+        window_clear(win, is, is);
+        window_clear(win, js, js);
+        i0 = is;  // re-examine the position that moved (the reference's i--)
+        if (++done >= cap) break;
+    }
+    if (tid == 0) moves += done;
+}
+
+// ---------------------------------------------------------- relocate ----
+
+// Relocate of position i next to position j (neighbourSearch.cpp:280-293,
+// move.cpp:32-51).  n2 = tour[j]; rm is the edge the insertion removes,
+// (pn2, n2) before j or (n2, sn2) after it; add = d(n1, n2) + d(n1, x)
+// with x = pn2 or sn2.
+__device__ __forceinline__ float relocate_gain(float a1, float b1, float c1, float rm, float add)
+{
+    return -(a1 + b1 + rm) + (add + c1);
+}
+
+// Load and route lengths stay feasible (checked only for improving moves
+// between two routes).
+__device__ __forceinline__ bool relocate_feasible(const DevProblem &p, const LsLds &L, int j, int r1, int dn1,
+                                                  float a1, float b1, float c1, float rm, float add)
+{
+    const int r2 = L.ridx[j];
+    if (r1 == r2) return true;
+    const float dist2 = ((L.rd[r2] + p.svc) - rm) + add;
+    const float dist1 = ((L.rd[r1] - p.svc) - (a1 + b1)) + c1;
+    return !(L.rl[r2] + dn1 > p.Q) && !(dist2 > p.D) && !(dist1 > p.D);
+}
+
+template <typename RT>
+__device__ __forceinline__ int relocate_scan(const DevProblem &p, const LsLds &L, int S, int i,
+                                             const RT *rowP, const RT *rowB, int lane,
+                                             const ScanStop &stop, int &evc)
+{
+    constexpr int U = kScanUnroll;
+    const int n1 = L.tour[i];
+    const int r1 = L.ridx[i], sn1 = L.tour[i + 1];
+    const float a1 = L.dtw[i - 1], b1 = L.dtw[i], c1 = rget(rowP, sn1);
+    const int dn1 = L.dem[n1];
+    int cnt = 0;
+    TripEdge pe = {L.dtw[0], rget(rowB, L.tour[0])};   // position jb - 1 = 0
+    for (int jb = 1; jb < S - 1; jb += U * kLanes) {
+        const unsigned key = read_key(stop.key);
+        int t[U];
+        float d[U], v[U];
+        // past the tour's end the padding reads as depots and zero edges;
+        // every position j <= S-2 that can be accepted sees its true
+        // neighbours
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            t[u] = L.tour[jb + u * kLanes + lane];
+            d[u] = L.dtw[jb + u * kLanes + lane];
+        }
+        const int ta = L.tour[jb + U * kLanes];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = rget(rowB, t[u]);
+        const float va = rget(rowB, ta);
+        float rm[U], add[U];
+        bool imp[U], val[U], any = false;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int j = jb + u * kLanes + lane;
+            const float dp = u == 0 ? pe.d : readlane_f32(d[u - 1], kLanes - 1);
+            const float vp = u == 0 ? pe.v : readlane_f32(v[u - 1], kLanes - 1);
+            const float vn = u == U - 1 ? va : readlane_f32(v[u + 1], 0);
+            // every lane takes part in the lane shifts; each lane then picks
+            const float dmu = wave_prev(d[u], dp), vpu = wave_prev(v[u], vp), vnu = wave_next(v[u], vn);
+            const bool before = i > j;
+            rm[u] = before ? dmu : d[u];
+            add[u] = v[u] + (before ? vpu : vnu);
+            val[u] = j < S - 1 && j != i && t[u] != 0;
+            imp[u] = val[u] && relocate_gain(a1, b1, c1, rm[u], add[u]) < -kEps;
+            any = any || imp[u];
+        }
+        if (__ballot(any)) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int j = jb + u * kLanes + lane;
+                const bool ok = imp[u] && relocate_feasible(p, L, j, r1, dn1, a1, b1, c1, rm[u], add[u]);
+                const u64 mk = __ballot(ok), mv = __ballot(val[u]);
+                if (mk) {
+                    const int f = __ffsll(mk) - 1;
+                    evc = cnt + __popcll(mv & lanes_upto(f));
+                    return jb + u * kLanes + f;
+                }
+                cnt += __popcll(mv);
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < U; ++u) cnt += __popcll(__ballot(val[u]));
+        }
+        if (key > stop.mine) break;
+        pe.d = readlane_f32(d[U - 1], kLanes - 1);
+        pe.v = readlane_f32(v[U - 1], kLanes - 1);
+    }
+    evc = cnt;
+    return -1;
+}
+
+template <int W, typename RT>
+__device__ __forceinline__ void ls_relocate(const DevProblem &p, LsLds &L, const RowRing<RT> &rr,
+                                            bool pf_on, int S, int tid, unsigned &round, long long &ev,
+                                            int &moves)
+{
+    const int lane = tid & 63, w = wave_id();
+    const int cap = kMaxMovesFactor * p.n;
+    int done = 0, i0 = 1, parity = 0;
+    RowWindow win = {0, 0u};
+    while (i0 < S - 1 && done < cap) {
+        // rows of positions i0-1 .. i0+W-1: d(pred, .) gives c1, d(n1, .) the gains
+        window_move(win, i0 - 1);
+        rows_ensure<W, RT>(p, L, rr, win, min(W + 1, S - i0), tid);
+        const unsigned pfw = prefetch_issue<W, RT>(p, L, rr, pf_on, win, W + 1, S - 1, tid);
+        const int i = i0 + w;
+        int j = -1, evc = 0;
+        const ScanStop stop = {reinterpret_cast<const unsigned *>(L.res + LsW<W>::res_first),
+                               round_key(++round, w)};
+        if (i < S - 1 && L.tour[i] != 0 && L.rc[L.ridx[i]] >= 2)
+            j = relocate_scan(p, L, S, i, rr.at(i - 1), rr.at(i), lane, stop, evc);
+        if (j >= 0 && lane == 0) atomicMax(reinterpret_cast<unsigned *>(L.res + LsW<W>::res_first), stop.mine);
+        int *res = L.res + W * parity, *rese = L.res + LsW<W>::res_ev + W * parity;
+        parity ^= 1;
+        if (lane == 0) { res[w] = j; rese[w] = evc; }
+        prefetch_wait(win, pfw, W + 1);
+        __syncthreads();
+        // the sequential scan evaluates the waves' positions in order and
+        // stops at the first accepted move
+        int js, e;
+        const int wstar = round_pick<W>(res, rese, lane, js, e);
+        if (tid == 0) ev += e;
+        if (wstar < 0) {
+            // no barrier: the next round writes the other result buffer, and
+            // rows it loads belong to positions no wave still reads
+            i0 += W;
+            continue;
+        }
+        const int is = i0 + wstar;
+        const int n1 = L.tour[is];
+        const int lo = is < js ? is : js, hi = is < js ? js : is;
+        if (tid == 0) {
+            const RT *rowP = rr.at(is - 1), *rowB = rr.at(is);
+            const int r1 = L.ridx[is], sn1 = L.tour[is + 1];
+            const int n2 = L.tour[js], pn2 = L.tour[js - 1], sn2 = L.tour[js + 1], r2 = L.ridx[js];
+            const float a1 = L.dtw[is - 1], b1 = L.dtw[is], c1 = rget(rowP, sn1);
+            const int dn1 = L.dem[n1];
+            float rm, add;
+            if (is > js) { rm = L.dtw[js - 1]; add = rget(rowB, n2) + rget(rowB, pn2); }
+            else { rm = L.dtw[js]; add = rget(rowB, n2) + rget(rowB, sn2); }
+            const float gain = -(a1 + b1 + rm) + (add + c1);
+            if (r1 != r2) {
+                L.rl[r1] -= dn1;
+                L.rl[r2] += dn1;
+                L.rd[r1] = ((L.rd[r1] - p.svc) - (a1 + b1)) + c1;
+                L.rd[r2] = ((L.rd[r2] + p.svc) - rm) + add;
+                L.rc[r1] = (short)(L.rc[r1] - 1);
+                L.rc[r2] = (short)(L.rc[r2] + 1);
+            } else {
+                L.rd[r1] = L.rd[r1] + gain;
+            }
+        }
+        // the three new edges (removal point, both sides of the insertion),
+        // read by every thread before the shift
+        const RT *rowPs = rr.at(is - 1), *rowBs = rr.at(is);
+        const int sn1s = L.tour[is + 1], n2s = L.tour[js];
+        const int xs = is > js ? L.tour[js - 1] : L.tour[js + 1];
+        const float c1s = rget(rowPs, sn1s), e_n2 = rget(rowBs, n2s), e_x = rget(rowBs, xs);
+        // shift: positions [lo, hi] move by one towards is, n1 lands at js;
+        // route indices and edge weights move with them (no reload)
+        for (int q = lo + tid; q <= hi; q += LsW<W>::threads) {
+            L.tmp[q] = L.tour[q] | ((int)L.ridx[q] << 16);
+            L.tmpf[q] = L.dtw[q];
+        }
+        __syncthreads();
+        for (int q = lo + tid; q <= hi; q += LsW<W>::threads) {
+            int v, r;
+            if (q == js) {
+                v = n1;
+                r = L.tmp[js] >> 16;          // n1 joins the route of the node at js
+            } else {
+                const int t = L.tmp[is < js ? q + 1 : q - 1];
+                v = t & 0xFFFF;
+                r = t >> 16;
+            }
+            L.tour[q] = v;
+            L.ridx[q] = (short)r;
+        }
+        for (int q = lo - 1 + tid; q <= hi; q += LsW<W>::threads) {
+            float d;
+            if (is < js) {
+                d = q == is - 1 ? c1s : q == js - 1 ? e_n2 : q == js ? e_x : L.tmpf[q + 1];
+            } else {
+                d = q == js - 1 ? e_x : q == js ? e_n2 : q == is ? c1s : L.tmpf[q - 1];
+            }
+            L.dtw[q] = d;
+        }
+        __syncthreads();
+        window_clear(win, lo, hi);
+        i0 = is;
+        ++done;
+    }
+    if (tid == 0) moves += done;
+}
+
+// ---------------------------------------------------------- one ant ----
+
+// Local search of ant a of one instance by the W waves of the calling block,
+// LDS at smem (local_search_lds_bytes).  The caller synchronises the block
+// before reusing smem.
+template <int W, typename RT>
+__device__ void local_search_ant(const DevProblem &p, const DevState &s, int a, int flags, int ring,
+                                 int prefetch, unsigned char *smem)
+{
+    constexpr int NT = LsW<W>::threads;
+    const int tid = threadIdx.x;
+    LsLds L = ls_bind(p.n, ring, (int)sizeof(RT), smem);
+    const RowRing<RT> rr = {reinterpret_cast<RT *>(L.rows), ring - 1, row_stride<RT>(p.n)};
+    const bool pf_on = prefetch != 0;
+    const int S = s.sizes[a];
+    int *gt = s.tours + (size_t)a * p.T;
+    for (int q = tid; q < S; q += NT) L.tour[q] = gt[q];
+    // padding past the tour's end: depots and zero edges (never written
+    // again; moves stay inside [1, S-2])
+    for (int q = tid; q < kScanPad; q += NT) {
+        L.tour[S + q] = 0;
+        L.dtw[S - 1 + q] = 0.0f;
+    }
+    for (int v = tid; v < p.n; v += NT) {
+        L.dlb[v] = 0;
+        L.dem[v] = p.dem[v];
+    }
+    if (tid == 0) L.res[LsW<W>::res_first] = 0;
+    __syncthreads();
+    long long ev = 0;
+    int moves = 0;
+    unsigned round = 0;
+    if (flags & kLs2Opt) {
+        ls_edges<W>(p, L, 0, S - 2, tid);
+        const int R = ls_routes<W>(p, L, S, tid, false);
+        const int grp = tid / kOptGroup, gl = tid % kOptGroup;
+        for (int r = grp; r < R; r += NT / kOptGroup)
+            two_opt_route(p, L, r, L.zpos[r], L.zpos[r + 1] - 1, gl, ev, moves);
+        __syncthreads();
+    }
+    if (flags & kLsSwap) {
+        ls_edges<W>(p, L, 0, S - 2, tid);
+        __syncthreads();
+        ls_routes<W>(p, L, S, tid, true);
+        ls_swap<W, RT>(p, L, rr, pf_on, S, tid, round, ev, moves);
+    }
+    if (flags & kLsRelocate) {
+        ls_edges<W>(p, L, 0, S - 2, tid);
+        __syncthreads();
+        ls_routes<W>(p, L, S, tid, true);
+        ls_relocate<W, RT>(p, L, rr, pf_on, S, tid, round, ev, moves);
+    }
+    ls_edges<W>(p, L, 0, S - 2, tid);
+    __syncthreads();
+    for (int q = tid; q < S; q += NT) gt[q] = L.tour[q];
+    // tour length in position order (vrpHelper.cpp:244-253)
+    if (tid == 0) {
+        float len = 0.0f;
+        for (int q = 0; q < S - 1; ++q) len += L.dtw[q];
+        s.lens[a] = len;
+    }
+    __shared__ long long red_ev[W];
+    __shared__ int red_mv[W];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        ev += __shfl_xor(ev, o);
+        moves += __shfl_xor(moves, o);
+    }
+    if ((tid & 63) == 0) {
+        red_ev[tid >> 6] = ev;
+        red_mv[tid >> 6] = moves;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        long long e = 0;
+        int mv = 0;
+        for (int q = 0; q < W; ++q) { e += red_ev[q]; mv += red_mv[q]; }
+        s.ant_evals[a] = e;
+        s.ant_moves[a] = mv;
+    }
+}
+
+}  // namespace cvrp

@@ -277,27 +277,27 @@ class GpuACO:
     # -- launch shapes and the autotuner ---------------------------------
     def launch_shape(self) -> tuple:
         """(local-search waves, ring rows, prefetch, construction ants per
-        block, 16-bit local-search rows)."""
+        block, 16-bit local-search rows, fused iteration)."""
         sh = _lib.AcoLaunchShape()
         check(lib.aco_get_launch_shape(self.h, C.byref(sh)), "aco_get_launch_shape")
-        return (sh.ls_waves, sh.ls_ring, sh.ls_prefetch, sh.construct_apb, sh.ls_row16)
+        return (sh.ls_waves, sh.ls_ring, sh.ls_prefetch, sh.construct_apb, sh.ls_row16, sh.fused_iteration)
 
     def set_launch_shape(self, waves: int, ring: int, prefetch: int, construct_apb: int = 0,
-                         row16: int = 0):
-        sh = _lib.AcoLaunchShape(waves, ring, prefetch, construct_apb, row16)
+                         row16: int = 0, fused: int = 0):
+        sh = _lib.AcoLaunchShape(waves, ring, prefetch, construct_apb, row16, fused)
         check(lib.aco_set_launch_shape(self.h, C.byref(sh)), "aco_set_launch_shape")
 
     def autotune(self, reps: int = 3):
         """Times every construction and local-search candidate on the current
         state and keeps the fastest of each (state and results unchanged).
         Returns [(shape, stage, ms or -1)], stage 0 = construction,
-        1 = local search."""
+        1 = local search, 2 = whole iteration (per-phase or fused)."""
         res = (_lib.AcoTuneResult * 32)()
         cnt = C.c_int32()
         check(lib.aco_autotune(self.h, reps, C.cast(res, C.c_void_p), 32, C.byref(cnt)),
               "aco_autotune")
         return [((r.shape.ls_waves, r.shape.ls_ring, r.shape.ls_prefetch, r.shape.construct_apb,
-                  r.shape.ls_row16), r.stage, r.ms) for r in res[: cnt.value]]
+                  r.shape.ls_row16, r.shape.fused_iteration), r.stage, r.ms) for r in res[: cnt.value]]
 
     def profile_enable(self, on: bool = True):
         check(lib.aco_profile_enable(self.h, int(on)), "aco_profile_enable")

@@ -176,7 +176,13 @@ int aco_profile_get(aco_ctx *ctx, const char *name, double *total_ms, int64_t *l
  * roulette without a route-length limit, ants per instance divisible by
  * it); 0 reads candidate rows from L2/HBM.  ls_row16 keeps the
  * local-search rows as 16-bit integers (exact; only when every distance is
- * an integer in [0, 65535], e.g. VRPLIB nint).  Every valid shape gives
+ * an integer in [0, 65535], e.g. VRPLIB nint).  fused_iteration runs
+ * aco_iteration as one launch (after a one-thread queue reset): blocks of
+ * ls_waves (4 or 8, dividing the ants per instance) take W ants of one
+ * instance at a time, build them (one wave each) and search them, and the
+ * block finishing an instance runs its statistics and pheromone update;
+ * candidate roulette without a route-length limit only.  The per-phase
+ * entry points (aco_construct, ...) are unaffected.  Every valid shape gives
  * identical results. */
 typedef struct aco_launch_shape {
     int32_t ls_waves;
@@ -184,12 +190,14 @@ typedef struct aco_launch_shape {
     int32_t ls_prefetch;
     int32_t construct_apb;
     int32_t ls_row16;
+    int32_t fused_iteration;
 } aco_launch_shape;
 
 typedef struct aco_tune_result {
     aco_launch_shape shape;
-    int32_t stage;       /* 0: construction candidate, 1: local-search candidate */
-    float ms;            /* mean launch time of that stage, -1 if it does not apply */
+    int32_t stage;       /* 0: construction candidate, 1: local-search candidate,
+                            2: whole iteration, per-phase launches or fused */
+    float ms;            /* mean time of that stage, -1 if it does not apply */
 } aco_tune_result;
 
 int aco_get_launch_shape(aco_ctx *ctx, aco_launch_shape *shape);
@@ -197,10 +205,11 @@ int aco_get_launch_shape(aco_ctx *ctx, aco_launch_shape *shape);
 int aco_set_launch_shape(aco_ctx *ctx, const aco_launch_shape *shape);
 /* After aco_init: times every construction candidate `reps` times from the
  * current state, then every local-search candidate on the same constructed
- * tours, keeps the fastest of each and
- * restores the state, so the colony's results do not change; the clock of
- * the best-so-far records restarts afterwards.  results (optional,
- * capacity entries) receives every candidate's time. */
+ * tours, keeps the fastest of each, then times whole iterations with those
+ * (per-phase launches) against the fused launch with up to three shapes,
+ * keeps the fastest and restores the state, so the colony's results do not
+ * change; the clock of the best-so-far records restarts afterwards.
+ * results (optional, capacity entries) receives every candidate's time. */
 int aco_autotune(aco_ctx *ctx, int32_t reps, aco_tune_result *results, int32_t capacity,
                  int32_t *count);
 

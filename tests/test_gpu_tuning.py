@@ -76,13 +76,13 @@ def test_staged_construction_with_fallbacks(rng):
 def test_invalid_shapes_rejected():
     inst = I.uniform_100(1)
     with ca.GpuACO(inst, n_ants=8, ls_flags=7) as g:   # I-Roulette: no staged construction
-        assert g.launch_shape() == (4, 16, 1, 0, 0)
+        assert g.launch_shape() == (4, 16, 1, 0, 0, 0)
         for bad in [(2, 16, 0), (4, 4, 0), (4, 8, 1), (8, 8, 0), (8, 16, 1), (4, 12, 0),
                     (6, 16, 0), (12, 8, 0), (12, 16, 1), (16, 16, 0), (16, 32, 1),
                     (4, 16, 1, 1), (4, 16, 1, 3)]:
             with pytest.raises(ca.AcoError):
                 g.set_launch_shape(*bad)
-        assert g.launch_shape() == (4, 16, 1, 0, 0)
+        assert g.launch_shape() == (4, 16, 1, 0, 0, 0)
         g.set_launch_shape(4, 16, 1, 0, 1)       # nint distances: 16-bit rows allowed
     fl = I.uniform_100(1, dist_mode=ca.DIST_FLOAT)
     with ca.GpuACO(fl, n_ants=8, ls_flags=7) as g:
@@ -106,9 +106,14 @@ def test_autotune_keeps_the_trajectory():
                 log(f"autotune: {res} -> {g.launch_shape()}")
                 assert any(ms > 0 for _, st, ms in res if st == 0)
                 assert any(ms > 0 for _, st, ms in res if st == 1)
-                assert (g.launch_shape()[:3] + g.launch_shape()[4:]) in \
-                    [sh[:3] + sh[4:] for sh, st, ms in res if st == 1 and ms > 0]
-                assert g.launch_shape()[3] in [sh[3] for sh, st, ms in res if st == 0 and ms > 0]
+                chosen = g.launch_shape()
+                if chosen[5]:
+                    assert chosen in [sh for sh, st, ms in res if st == 2 and ms > 0]
+                else:
+                    assert (chosen[:3] + chosen[4:5]) in \
+                        [sh[:3] + sh[4:5] for sh, st, ms in res if st == 1 and ms > 0]
+                    assert chosen[3] in [sh[3] for sh, st, ms in res if st == 0 and ms > 0]
+                assert any(ms > 0 for _, st, ms in res if st == 2)
             g.run(4)
             runs.append([(g.ants(b), g.pheromone(b), g.bsf_records(b)) for b in range(len(insts))])
     for b in range(len(insts)):
