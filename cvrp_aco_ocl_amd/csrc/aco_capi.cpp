@@ -99,16 +99,17 @@ struct aco_ctx {
 // asynchronously, so uploads run at DMA speed while the host prepares the
 // next chunk.  The destructor waits for the transfers before freeing.
 struct PinnedStager {
-    static constexpr size_t kBytes = size_t(32) << 20;
+    size_t kBytes = size_t(32) << 20;   // per buffer
     unsigned char *buf[2] = {nullptr, nullptr};
     hipEvent_t done[2] = {nullptr, nullptr};
     bool used[2] = {false, false};
     int cur = 0;
     hipStream_t st = nullptr;
 
-    hipError_t init(hipStream_t s)
+    hipError_t init(hipStream_t s, size_t bytes)
     {
         st = s;
+        kBytes = bytes;
         for (int i = 0; i < 2; ++i) {
             hipError_t e = hipHostMalloc((void **)&buf[i], kBytes, hipHostMallocDefault);
             if (e == hipSuccess) e = hipEventCreateWithFlags(&done[i], hipEventDisableTiming);
@@ -503,8 +504,14 @@ int aco_create_batch(const aco_problem *pbs, int32_t count, const aco_params *pr
     tree_rank(kLanes, r64.data());
     for (int l = 0; l < kLanes; ++l) r64inv[r64[l]] = l;
 
+    // pinned buffers sized for the largest instance's uploads (about 24 n^2
+    // bytes), between 1 and 32 MB each
+    size_t stage_bytes = size_t(1) << 20;
+    for (int b = 0; b < count; ++b)
+        stage_bytes = std::max(stage_bytes, (size_t)24 * pbs[b].n * pbs[b].n + (size_t(64) << 10));
+    stage_bytes = std::min(stage_bytes, size_t(32) << 20);
     PinnedStager up;
-    CHK(up.init(c->stream));
+    CHK(up.init(c->stream, stage_bytes));
     c->inst.resize(count);
     for (int b = 0; b < count; ++b) {
         const aco_problem *pb = &pbs[b];
