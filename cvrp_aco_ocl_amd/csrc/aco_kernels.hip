@@ -819,10 +819,11 @@ __global__ void k_fused_reset(DevBatch bt)
 }
 
 template <int W, typename RT, int RNG>
-// (at least 4 waves per SIMD: with construction, search and the last
-// block's passes in one kernel the compiler would otherwise keep enough
-// loop-invariant values live to allow only 2)
-__global__ __launch_bounds__(W * 64, 4) void k_iteration_fused(DevBatch bt, int flags, int ring, int prefetch,
+// (a floor of 3 (W = 4) or 4 (W = 8) waves per SIMD: with construction,
+// search and the last block's passes in one kernel the compiler would
+// otherwise keep enough loop-invariant values live to allow only 2; 3 is
+// what LDS allows 4-wave blocks at n = 500 anyway)
+__global__ __launch_bounds__(W * 64, W == 4 ? 3 : 4) void k_iteration_fused(DevBatch bt, int flags, int ring, int prefetch,
                                                            float tick_to_s)
 {
     constexpr int NT = W * 64;
@@ -906,7 +907,7 @@ hipError_t launch_iteration_fused(const DevBatch &b, int rng, int ls_flags, cons
     const size_t lds = fused_lds_bytes(b.max_n, sh);
     // enough blocks to fill the device; each leaves when the queue is empty
     const int items = b.ants / sh.waves;
-    int blocks = cus * (sh.waves == 4 ? 4 : 2);
+    int blocks = cus * (sh.waves == 4 ? 3 : 2);
     if (blocks > items) blocks = items;
 #define KF(W, RT, R) hipLaunchKernelGGL((k_iteration_fused<W, RT, R>), dim3(blocks), dim3(W * kLanes), lds, st, b, \
                                         ls_flags, sh.ring, sh.prefetch, tick_to_s)
