@@ -28,6 +28,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "ant-solutions/s + time-to-2%-gap, learned-prior vs plain ACO, 1 MI355X"
+STAGES = {0: "construct", 1: "local_search", 2: "iteration"}
 HBM_PEAK_GBS = 8000.0
 WORKLOAD = dict(customers=480, clusters=8, sigma=6.0, routes=30, m=480, k=30, ls=7,
                 iterations_full=1000)
@@ -49,6 +50,8 @@ def parse():
                     help="learned-prior vs plain leg: evaluation instances (0 = skip)")
     ap.add_argument("--prior-iterations", type=int, default=2000,
                     help="plain iterations that fix the reference cost C*")
+    ap.add_argument("--batch-instances", type=int, default=256,
+                    help="instances of the batched leg (0 = skip)")
     ap.add_argument("--n1000-seconds", type=float, default=10.0,
                     help="GPU vs CPU oracle on the 1001-node instance (0 = skip)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -305,10 +308,42 @@ def main():
                  "cpu_ant_solutions_per_s": round(c_rate, 2), "cpu_threads": threads,
                  "cpu_sample": f"{its} iterations in {el:.2f}s",
                  "ratio": round(g_rate / c_rate, 2), "launch_shape": gb_shape,
-                 "autotune_ms": {f"{'construct' if st == 0 else 'local_search'}{sh}": round(ms, 3)
-                                 for sh, st, ms in tuned}}
+                 "autotune_ms": {f"{STAGES[st]}{sh}": round(ms, 3) for sh, st, ms in tuned}}
         print(f"# n=1001: GPU {g_rate:.0f} vs CPU {c_rate:.1f} ant-sol/s ({g_rate / c_rate:.1f}x)",
               file=sys.stderr, flush=True)
+
+    # ---- 256 uneven instances batched in one context (parity-test config,
+    # reported for its throughput; not the metric's workload) ----
+    batch = None
+    if args.batch_instances > 0:
+        binst = instances.batch(0, count=args.batch_instances)
+        ns = [it.n for it in binst]
+        t0 = time.perf_counter()
+        gq = ca.GpuACO(binst, n_ants=64, k=16, seed=1, selection=ca.SEL_KNN, rng=ca.RNG_PHILOX,
+                       ls_flags=7)
+        gq.init_aco()
+        gq.synchronize()
+        create_s = time.perf_counter() - t0
+        tuned = gq.autotune(reps=2)
+        gq.run(2)
+        gq.synchronize()
+        K = 10
+        t0 = time.perf_counter()
+        gq.run(K)
+        gq.synchronize()
+        el = time.perf_counter() - t0
+        bshape = gq.launch_shape()
+        gq.close()
+        batch = {"workload": f"{len(binst)} instances, n {min(ns)}..{max(ns)} (uniform or 4-12 "
+                             "clusters, demands 1..50, 5-40 routes), 64 ants each, k=16 Philox "
+                             "roulette, 2-opt+swap+relocate, one batched context",
+                 "ant_solutions_per_s": round(len(binst) * 64 * K / el, 1),
+                 "ms_per_iteration": round(1e3 * el / K, 3), "create_and_init_s": round(create_s, 3),
+                 "launch_shape": bshape,
+                 "iteration_ms": {f"{'fused' if sh[5] else 'per-phase'}{sh}": round(ms, 3)
+                                  for sh, st, ms in tuned if st == 2 and ms > 0}}
+        print(f"# batch: {batch['ant_solutions_per_s']:.0f} ant-sol/s, {batch['ms_per_iteration']:.2f} "
+              f"ms/iter, shape {bshape}", file=sys.stderr, flush=True)
 
     out = {
         "metric": METRIC,
@@ -339,6 +374,7 @@ def main():
         "quality": quality,
         "learned_prior": prior_leg,
         "n1001": n1000,
+        "batch256": batch,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
