@@ -50,8 +50,8 @@ template <int W> struct LsW {
 };
 
 struct LsLds {
-    int *tour, *tmp, *rl, *dem, *res;
-    float *dtw, *tmpf, *rd, *rows;
+    int *tour, *rl, *dem, *res;
+    float *dtw, *rd, *rows;
     short *pos, *rof, *ridx, *zpos, *rc;
     unsigned char *dlb;
 };
@@ -59,7 +59,7 @@ struct LsLds {
 __host__ __device__ constexpr size_t ls_align16(size_t x) { return (x + 15) & ~size_t(15); }
 
 struct LsOffsets {
-    size_t tour, tmp, dtw, tmpf, rl, rd, dem, rows, res, pos, rof, ridx, zpos, rc, dlb, total;
+    size_t tour, dtw, rl, rd, dem, rows, res, pos, rof, ridx, zpos, rc, dlb, total;
 };
 
 __host__ __device__ inline LsOffsets ls_offsets(int n, int ring, int rowbytes)
@@ -68,9 +68,7 @@ __host__ __device__ inline LsOffsets ls_offsets(int n, int ring, int rowbytes)
     LsOffsets o;
     size_t at = 0;
     o.tour = at; at = ls_align16(at + 4 * (T + kScanPad));
-    o.tmp = at;  at = ls_align16(at + 4 * T);
     o.dtw = at;  at = ls_align16(at + 4 * (T + kScanPad));
-    o.tmpf = at; at = ls_align16(at + 4 * T);
     o.rl = at;   at = ls_align16(at + 4 * N);
     o.rd = at;   at = ls_align16(at + 4 * N);
     o.dem = at;  at = ls_align16(at + 4 * N);
@@ -90,8 +88,7 @@ __device__ __forceinline__ LsLds ls_bind(int n, int ring, int rowbytes, unsigned
 {
     const LsOffsets o = ls_offsets(n, ring, rowbytes);
     LsLds L;
-    L.tour = (int *)(b + o.tour); L.tmp = (int *)(b + o.tmp); L.dtw = (float *)(b + o.dtw);
-    L.tmpf = (float *)(b + o.tmpf);
+    L.tour = (int *)(b + o.tour); L.dtw = (float *)(b + o.dtw);
     L.rl = (int *)(b + o.rl); L.rd = (float *)(b + o.rd); L.dem = (int *)(b + o.dem);
     L.rows = (float *)(b + o.rows); L.res = (int *)(b + o.res);
     L.pos = (short *)(b + o.pos); L.rof = (short *)(b + o.rof); L.ridx = (short *)(b + o.ridx);
@@ -909,36 +906,41 @@ __device__ __forceinline__ void ls_relocate(const DevProblem &p, LsLds &L, const
         const int sn1s = L.tour[is + 1], n2s = L.tour[js];
         const int xs = is > js ? L.tour[js - 1] : L.tour[js + 1];
         const float c1s = rget(rowPs, sn1s), e_n2 = rget(rowBs, n2s), e_x = rget(rowBs, xs);
-        // shift: positions [lo, hi] move by one towards is, n1 lands at js;
-        // route indices and edge weights move with them (no reload)
-        for (int q = lo + tid; q <= hi; q += LsW<W>::threads) {
-            L.tmp[q] = L.tour[q] | ((int)L.ridx[q] << 16);
-            L.tmpf[q] = L.dtw[q];
-        }
-        __syncthreads();
-        for (int q = lo + tid; q <= hi; q += LsW<W>::threads) {
-            int v, r;
-            if (q == js) {
-                v = n1;
-                r = L.tmp[js] >> 16;          // n1 joins the route of the node at js
-            } else {
-                const int t = L.tmp[is < js ? q + 1 : q - 1];
-                v = t & 0xFFFF;
-                r = t >> 16;
+        // shift, in place: positions [lo, hi] move by one towards is, n1
+        // lands at js (in the route of the node that was at js); route
+        // indices and the edge weights of [lo-1, hi] move with them (no
+        // reload).  Chunks of NT positions are read into registers, then
+        // written; they are taken in the direction of the move (upwards when
+        // is < js, downwards otherwise), so a source position is read before
+        // its own chunk is written.
+        {
+            constexpr int NT = LsW<W>::threads;
+            const bool up = is < js;
+            const int first = lo - 1, last = hi;
+            const int nch = (last - first + NT) / NT;
+            for (int c = 0; c < nch; ++c) {
+                const int q = (up ? first + c * NT : last - (c + 1) * NT + 1) + tid;
+                const bool in_t = q >= lo && q <= hi && q >= first, in_d = q >= first && q <= last;
+                int v = 0, r = 0;
+                float d = 0.0f;
+                if (in_t) {
+                    const int src = q == js ? js : up ? q + 1 : q - 1;
+                    v = q == js ? n1 : L.tour[src];
+                    r = L.ridx[src];
+                }
+                if (in_d) {
+                    if (up) d = q == is - 1 ? c1s : q == js - 1 ? e_n2 : q == js ? e_x : L.dtw[q + 1];
+                    else d = q == js - 1 ? e_x : q == js ? e_n2 : q == is ? c1s : L.dtw[q - 1];
+                }
+                __syncthreads();
+                if (in_t) {
+                    L.tour[q] = v;
+                    L.ridx[q] = (short)r;
+                }
+                if (in_d) L.dtw[q] = d;
+                __syncthreads();
             }
-            L.tour[q] = v;
-            L.ridx[q] = (short)r;
         }
-        for (int q = lo - 1 + tid; q <= hi; q += LsW<W>::threads) {
-            float d;
-            if (is < js) {
-                d = q == is - 1 ? c1s : q == js - 1 ? e_n2 : q == js ? e_x : L.tmpf[q + 1];
-            } else {
-                d = q == js - 1 ? e_x : q == js ? e_n2 : q == is ? c1s : L.tmpf[q - 1];
-            }
-            L.dtw[q] = d;
-        }
-        __syncthreads();
         window_clear(win, lo, hi);
         i0 = is;
         ++done;

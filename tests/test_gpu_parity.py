@@ -319,3 +319,50 @@ def test_errors_from_device_calls():
             bad = np.zeros((9, 5), np.int32)
             bad[:, 2] = 9999  # node id out of range
             g.set_ants(bad, np.full(9, 5, np.int32), np.zeros(9, np.float32))
+
+
+def _uniform(n_customers, seed, span=1000):
+    r = np.random.default_rng(seed)
+    x = r.integers(0, span + 1, size=n_customers + 1).astype(np.float64)
+    y = r.integers(0, span + 1, size=n_customers + 1).astype(np.float64)
+    x[0] = y[0] = span / 2
+    dem = np.concatenate([[0], r.integers(1, 51, size=n_customers)]).astype(np.int32)
+    return ca.Instance.from_coords(x, y, dem, int(math.ceil(dem.sum() / 80)), ca.DIST_NINT)
+
+
+def test_large_instance_matches_oracle():
+    """n = 2001: the default local-search shape falls back to the 8-row fp32
+    ring to fit LDS; one whole iteration equals the oracle's."""
+    inst = _uniform(2000, 31)
+    m = 4
+    P = oracle_problem(inst, k=24)
+    col = oracle.Colony(P, m, selection=ca.SEL_KNN, rng=ca.RNG_PHILOX, ls_flags=7, seed=13)
+    col.init()
+    with ca.GpuACO(inst, n_ants=m, k=24, seed=13, selection=ca.SEL_KNN, rng=ca.RNG_PHILOX,
+                   ls_flags=7) as g:
+        g.init_aco()
+        log(f"n={inst.n} shape {g.launch_shape()}")
+        g.run_aco_iteration()
+        col.iteration()
+        gt, gs, gl = g.ants()
+        ot, osz, ol = col.ants()
+        np.testing.assert_array_equal(gs, osz)
+        for a in range(m + 1):
+            np.testing.assert_array_equal(gt[a, : gs[a]], ot[a, : osz[a]], err_msg=f"ant {a}")
+            assert inst.check_solution(gt[a, : gs[a]])
+        np.testing.assert_array_equal(gl, ol)
+
+
+def test_too_large_for_local_search_is_a_clean_error():
+    """Past what LDS holds, aco_create fails with a message; without local
+    search the same instance runs."""
+    inst = _uniform(3000, 5)
+    with pytest.raises(ca.AcoError) as e:
+        ca.GpuACO(inst, n_ants=4, k=16, selection=ca.SEL_KNN, rng=ca.RNG_PHILOX, ls_flags=7)
+    assert "LDS" in str(e.value)
+    with ca.GpuACO(inst, n_ants=4, k=16, selection=ca.SEL_KNN, rng=ca.RNG_PHILOX, ls_flags=0) as g:
+        g.init_aco()
+        g.run(1)
+        tours, sizes, lens = g.ants()
+        for a in range(4):
+            assert inst.check_solution(tours[a, : sizes[a]])
