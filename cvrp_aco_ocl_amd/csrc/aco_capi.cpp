@@ -1062,7 +1062,7 @@ int aco_autotune(aco_ctx *c, int32_t reps, aco_tune_result *results, int32_t cap
     static const LsShape kLs[] = {{4, 16, 1, 0, 0},  {4, 16, 0, 0, 0},  {4, 8, 0, 0, 0},   {8, 32, 1, 0, 0},
                                   {8, 16, 0, 0, 0},  {4, 16, 1, 0, 1},  {8, 32, 1, 0, 1},  {8, 16, 0, 0, 1},
                                   {12, 16, 0, 0, 0}, {12, 16, 0, 0, 1}, {12, 32, 1, 0, 1}, {16, 32, 0, 0, 0},
-                                  {16, 32, 0, 0, 1}};
+                                  {16, 32, 0, 0, 1}, {4, 8, 0, 0, 1}};
     constexpr int kNc = (int)(sizeof(kApb) / sizeof(kApb[0]));
     constexpr int kNl = (int)(sizeof(kLs) / sizeof(kLs[0]));
     constexpr int kNi = 4;   // stage 2: the per-phase iteration and up to 3 fused shapes
