@@ -128,6 +128,21 @@ __device__ __forceinline__ bool feasible(const DevProblem &p, const AntLds &L, i
     return true;
 }
 
+// The same mask without branches (every condition evaluated), so the loads
+// feeding it are issued together.
+template <bool HAS_D>
+__device__ __forceinline__ bool feasible_mask(const DevProblem &p, const AntLds &L, int i, float dci,
+                                              int load, float rdist)
+{
+    bool f = (L.vis[i] == 0) & (load + L.dem[i] <= p.Q);
+    if (HAS_D) {
+        float t = rdist + (dci + p.svc);
+        t = t + L.d0[i];
+        f = f & (t <= p.D);
+    }
+    return f;
+}
+
 // Full-row greedy pick when the roulette has no positive mass: largest
 // total_info among feasible nodes, ties to the lowest index; 0 = none.
 // The row's values are loaded kGreedyChunk per lane at a time, all issued
@@ -300,16 +315,16 @@ __device__ void construct_knn_ant(const DevProblem &p, const DevState &s, int a,
     while (vcnt < n) {
         step++;
         if (RNG == kRngPhilox && ((step - 1) & 63) == 0) ublk = philox_block(seed, iter, (u32)a, (u32)step, lane);
-        int j = 0;
-        float pj = 0.0f, dcj = 0.0f;
-        if (lane < k) {
-            // candidate id, its total_info and (with a length limit) d(cur, id):
-            // one contiguous read each
-            j = p.nnk[cur * k + lane];
-            const float tj = s.tiknn[cur * k + lane];
-            if (HAS_D) dcj = p.nnkd[cur * k + lane];
-            pj = feasible<HAS_D>(p, L, j, dcj, load, rdist) ? tj : 0.0f;
-        }
+        // candidate id, its total_info and (with a length limit) d(cur, id):
+        // one contiguous read each, issued together; lanes >= k read the row's
+        // last entry and are masked.  total_info is finite and >= 0, so the
+        // product by 1 or 0 is exact and keeps its load unconditional.
+        const int e = cur * k + (lane < k ? lane : k - 1);
+        const int j = p.nnk[e];
+        const float tj = s.tiknn[e];
+        const float dcj = HAS_D ? p.nnkd[e] : 0.0f;
+        const bool ok = (lane < k) & feasible_mask<HAS_D>(p, L, j, dcj, load, rdist);
+        const float pj = tj * (ok ? 1.0f : 0.0f);
         const float x = wave_scan_f32(pj, lane);
         const float total = lane_f32(x, k - 1);
         int w;
@@ -428,7 +443,9 @@ __global__ __launch_bounds__(64 * A) void k_construct_knn_lds(DevBatch bt)
         const uint2 en = ent[cur * k + (lane < k ? lane : k - 1)];
         const int j = (int)(en.y & 0xFFFFu), dj = (int)(en.y >> 16);
         const bool ok = (lane < k) & (vis[j] == 0) & (load + dj <= p.Q);   // the mask of feasible<false>
-        const float pj = ok ? __uint_as_float(en.x) : 0.0f;
+        // total_info is finite and >= 0, so x * 1 = x and x * 0 = 0 exactly; the
+        // product keeps its load unconditional (one 8-byte LDS read with the id)
+        const float pj = __uint_as_float(en.x) * (ok ? 1.0f : 0.0f);
         const float x = wave_scan_f32(pj, lane);
         const float total = lane_f32(x, k - 1);
         int w, dw;
