@@ -835,11 +835,11 @@ __global__ void k_fused_reset(DevBatch bt)
     if (threadIdx.x == 0) *bt.fq_next = 0;
 }
 
-template <int W, typename RT, int RNG>
 // (a floor of 3 (W = 4) or 4 (W = 8) waves per SIMD: with construction,
 // search and the last block's passes in one kernel the compiler would
 // otherwise keep enough loop-invariant values live to allow only 2; 3 is
 // what LDS allows 4-wave blocks at n = 500 anyway)
+template <int W, typename RT, int RNG>
 __global__ __launch_bounds__(W * 64, W == 4 ? 3 : 4) void k_iteration_fused(DevBatch bt, int flags, int ring, int prefetch,
                                                            float tick_to_s)
 {
