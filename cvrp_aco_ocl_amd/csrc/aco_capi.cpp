@@ -462,8 +462,8 @@ int aco_create_batch(const aco_problem *pbs, int32_t count, const aco_params *pr
     // fits, else an 8-row ring
     c->lds_limit = prop.sharedMemPerBlock;
     c->max_n = max_n;
-    c->ls = LsShape{4, 16, 1, 0};
-    if (local_search_lds_bytes(max_n, 16, 0) > prop.sharedMemPerBlock) c->ls = LsShape{4, 8, 0, 0, 0};
+    c->ls = LsShape{4, 16, 1, 0, 0, 0};
+    if (local_search_lds_bytes(max_n, 16, 0) > prop.sharedMemPerBlock) c->ls = LsShape{4, 8, 0, 0, 0, 0};
     const size_t lds_need = std::max(construct_lds_bytes(max_n, prm.selection),
                                      prm.ls_flags ? local_search_lds_bytes(max_n, c->ls.ring, 0) : 0);
     if (lds_need > prop.sharedMemPerBlock)
