@@ -27,6 +27,7 @@ import numpy as np
 
 from . import instances as I
 from . import prior as PR
+from .solver import Instance
 
 DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
 # exponent of the prior probability in eta (eta * p^gamma), chosen on held-out
@@ -55,8 +56,8 @@ def labelled_edges(count, seed, a, tag):
     for lo in range(0, count, a.batch):
         chunk = [I.training(i, seed=seed) for i in range(lo, min(count, lo + a.batch))]
         best = PR.solve_plain(chunk, a.iterations, n_ants=a.ants, k=a.k, seed=seed + 1 + lo)
-        for inst, (tour, length) in zip(chunk, best):
-            nn = inst.nn_lists(a.k)
+        lists = Instance.nn_lists_batch(chunk, a.k)   # the chunk's candidate lists in one GPU pass
+        for inst, nn, (tour, length) in zip(chunk, lists, best):
             feats.append(PR.features(inst, nn, a.k))
             labels.append(PR.edge_labels(nn, a.k, tour))
             costs.append(length)
