@@ -179,3 +179,22 @@ def test_apply_equals_dense_prior(ca):
     with ca.GpuACO(insts, n_ants=16, k=20, seed=5, tau0_rule=ca.TAU0_NN_TOUR) as g:
         g.init_aco()
         assert not np.array_equal(g.pheromone(0)[1], runs[0][0][0])
+
+
+def test_labelled_edges_of_training_instances(ca):
+    """The training set builder: features of every candidate edge (lists from
+    the batched GPU builder) equal the per-instance ones, labels mark the
+    plain colony's best tours."""
+    import argparse
+    from cvrp_aco_ocl_amd import instances as I, prior as PR, train_prior as TP
+    a = argparse.Namespace(batch=3, iterations=5, ants=16, k=16)
+    f, y, costs, _ = TP.labelled_edges(3, 7, a, "test")
+    insts = [I.training(i, seed=7) for i in range(3)]
+    assert f.shape == (sum(it.n for it in insts) * 16, 12) and y.shape == (f.shape[0],)
+    assert set(np.unique(y)) <= {0.0, 1.0} and y.sum() > 0
+    off = 0
+    for it in insts:
+        ref = PR.features(it, it.nn_lists(16), 16)
+        np.testing.assert_array_equal(f[off: off + ref.shape[0]], ref)
+        off += ref.shape[0]
+    assert len(costs) == 3 and all(c > 0 for c in costs)
