@@ -50,6 +50,8 @@ def parse():
                     help="learned-prior vs plain leg: evaluation instances (0 = skip)")
     ap.add_argument("--prior-iterations", type=int, default=2000,
                     help="plain iterations that fix the reference cost C*")
+    ap.add_argument("--mlp-probe", type=int, default=0, metavar="INSTANCES",
+                    help="only run the learned prior's MLP on that many evaluation instances")
     ap.add_argument("--batch-instances", type=int, default=256,
                     help="instances of the batched leg (0 = skip)")
     ap.add_argument("--n1000-seconds", type=float, default=10.0,
@@ -70,6 +72,32 @@ def copy_probe(mib: int) -> None:
     torch.cuda.synchronize()
     print(json.dumps({"copy_probe_bytes_read": nbytes, "copy_probe_bytes_written": nbytes,
                       "launches": 10}))
+
+
+def mlp_probe(instances_count: int) -> None:
+    """The learned prior's dense layers on the evaluation set's candidate
+    edges (instances of 500 customers, k = 16): 3 forward passes over every
+    edge, then one training epoch (batches of 16384), for a kernel trace and
+    MFMA counters.  Labels are the candidate rank < 2 (the timing does not
+    depend on them)."""
+    import cvrp_aco_ocl_amd as ca
+    from cvrp_aco_ocl_amd import instances, prior as PR
+    insts = instances.evaluation(instances_count)
+    lists = ca.Instance.nn_lists_batch(insts, 16)
+    feat = np.concatenate([PR.features(it, nn, 16) for it, nn in zip(insts, lists)])
+    label = (feat[:, 1] < 2.0 / 16.0).astype(np.float32)
+    net = PR.PriorNet.load()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        net.forward(feat)
+    t_fwd = (time.perf_counter() - t0) / 3
+    t0 = time.perf_counter()
+    net.train(feat, label, epochs=1, batch=16384)
+    t_train = time.perf_counter() - t0
+    net.close()
+    print(json.dumps({"mlp_probe_edges": int(len(feat)), "forward_wall_s": round(t_fwd, 4),
+                      "train_epoch_wall_s": round(t_train, 4),
+                      "forward_flop_per_edge": 2 * (12 * 64 + 64 * 64 + 64)}))
 
 
 def make_solver(ca, inst, seed):
@@ -117,6 +145,9 @@ def main():
         sys.exit(2)
     if args.copy_probe:
         copy_probe(args.copy_probe)
+        return
+    if args.mlp_probe:
+        mlp_probe(args.mlp_probe)
         return
     import cvrp_aco_ocl_amd as ca
     from cvrp_aco_ocl_amd import instances
