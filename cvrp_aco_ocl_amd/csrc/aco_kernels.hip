@@ -703,36 +703,19 @@ hipError_t launch_statistics(const DevBatch &b, float tick_to_s, hipStream_t st)
 
 // Fused pheromone_evaporation + pheromone_deposit + compute_total_info
 // (kernel.cl:685-692, 747-776, 782-789) in one pass over n x P.  Each
-// wavefront owns 64 float4 of one row j, so the row's elite successors
-// (succ[g][j], <= ras values) are wave-uniform.  Per element:
+// wavefront owns 128 float4 of one row j (two per lane, both loaded before
+// either is used), so the row's elite successors (succ[g][j], <= ras values)
+// are wave-uniform.  Per element:
 //   t = (1-rho)*tau;  for g in order: if edge (j,h) is in elite g: t += d_g
 //   tau = t;  total_info = t^alpha * heur
 // The elite-order sum is deterministic (the reference's CAS atomics are not).
-// Work item `wave` (row j, 64 float4 of it) of one instance's pass.
-__device__ __forceinline__ int pheromone_items(const DevProblem &p) { return p.n * ((p.P / 4 + 63) / 64); }
+constexpr int kPhF4 = 128;   // float4 per work item (row segment)
 
-__device__ __forceinline__ void pheromone_item(const DevProblem &p, const DevState &s, int wave, int lane)
+__device__ __forceinline__ int pheromone_items(const DevProblem &p) { return p.n * ((p.P / 4 + kPhF4 - 1) / kPhF4); }
+
+__device__ __forceinline__ void pheromone_f4(const DevProblem &p, const DevState &s, int j, int c4, float evap,
+                                             const int *sc, const float *dl, float4 t, float4 h, size_t off)
 {
-    const float evap = 1.0f - p.rho;
-    const int chunks = (p.P / 4 + 63) / 64;
-    const int j = wave / chunks;
-    if (j >= p.n) return;
-    const int c4 = (wave - j * chunks) * 64 + lane;
-    if (c4 * 4 >= p.P) return;
-    int sc[kMaxRas];
-    float dl[kMaxRas];
-#pragma unroll
-    for (int g = 0; g < kMaxRas; ++g) {
-        sc[g] = -1;
-        dl[g] = 0.0f;
-        if (g < p.ras) {
-            sc[g] = j != 0 ? s.succ[g * p.n + j] : -1;
-            dl[g] = s.delta[g];
-        }
-    }
-    const size_t off = (size_t)j * p.P + (size_t)c4 * 4;
-    float4 t = *reinterpret_cast<const float4 *>(s.tau + off);
-    const float4 h = *reinterpret_cast<const float4 *>(p.heur + off);
     u32 ff[kMaxRas];
 #pragma unroll
     for (int g = 0; g < kMaxRas; ++g) {
@@ -760,6 +743,40 @@ __device__ __forceinline__ void pheromone_item(const DevProblem &p, const DevSta
     *reinterpret_cast<float4 *>(s.tinfo + off) = make_float4(ti[0], ti[1], ti[2], ti[3]);
 }
 
+// Work item `wave` (row j, kPhF4 float4 of it) of one instance's pass.
+__device__ __forceinline__ void pheromone_item(const DevProblem &p, const DevState &s, int wave, int lane)
+{
+    const float evap = 1.0f - p.rho;
+    const int chunks = (p.P / 4 + kPhF4 - 1) / kPhF4;
+    const int j = wave / chunks;
+    if (j >= p.n) return;
+    const int c4a = (wave - j * chunks) * kPhF4 + lane, c4b = c4a + 64;
+    const bool a_in = c4a * 4 < p.P, b_in = c4b * 4 < p.P;
+    int sc[kMaxRas];
+    float dl[kMaxRas];
+#pragma unroll
+    for (int g = 0; g < kMaxRas; ++g) {
+        sc[g] = -1;
+        dl[g] = 0.0f;
+        if (g < p.ras) {
+            sc[g] = j != 0 ? s.succ[g * p.n + j] : -1;
+            dl[g] = s.delta[g];
+        }
+    }
+    const size_t offa = (size_t)j * p.P + (size_t)c4a * 4, offb = offa + 256;
+    float4 ta = make_float4(0, 0, 0, 0), ha = ta, tb = ta, hb = ta;
+    if (a_in) {
+        ta = *reinterpret_cast<const float4 *>(s.tau + offa);
+        ha = *reinterpret_cast<const float4 *>(p.heur + offa);
+    }
+    if (b_in) {
+        tb = *reinterpret_cast<const float4 *>(s.tau + offb);
+        hb = *reinterpret_cast<const float4 *>(p.heur + offb);
+    }
+    if (a_in) pheromone_f4(p, s, j, c4a, evap, sc, dl, ta, ha, offa);
+    if (b_in) pheromone_f4(p, s, j, c4b, evap, sc, dl, tb, hb, offb);
+}
+
 __global__ __launch_bounds__(256) void k_pheromone_update(DevBatch bt)
 {
     pheromone_item(bt.probs[blockIdx.y], bt.states[blockIdx.y], blockIdx.x * 4 + (threadIdx.x >> 6),
@@ -768,7 +785,7 @@ __global__ __launch_bounds__(256) void k_pheromone_update(DevBatch bt)
 
 hipError_t launch_pheromone_update(const DevBatch &b, hipStream_t st)
 {
-    const int chunks = (b.max_P / 4 + 63) / 64;
+    const int chunks = (b.max_P / 4 + kPhF4 - 1) / kPhF4;
     const int waves = b.max_n * chunks;
     hipLaunchKernelGGL(k_pheromone_update, dim3((waves + 3) / 4, b.B), dim3(256), 0, st, b);
     return hipGetLastError();
