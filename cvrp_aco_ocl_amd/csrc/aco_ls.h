@@ -523,9 +523,6 @@ __device__ __forceinline__ int round_pick(const int *res, const int *rese, int l
 
 // -------------------------------------------------------------- swap ----
 
-// Mask of lanes 0..f.
-__device__ __forceinline__ u64 lanes_upto(int f) { return (2ull << f) - 1ull; }
-
 // Early stop of speculative scans: a wave that accepts a move raises the
 // round key (round << 5 | 31 - wave) with an LDS max.  A wave whose key is
 // exceeded knows that a lower wave of this round has a move, so its own
@@ -546,27 +543,44 @@ __device__ __forceinline__ unsigned read_key(const unsigned *key)
     return *(volatile const __attribute__((address_space(3))) unsigned *)(key);
 }
 
-// A trip of a scan covers kScanUnroll chunks of 64 consecutive positions j.
-// Each lane loads the node and the edge weight at its own j once; the
-// values at j-1 and j+1 come from the neighbouring lanes (wave shifts), and
-// at the chunk edges from the neighbouring chunk or, at the trip's edges,
-// from the positions before and after the trip.
-struct TripEdge {
-    float d;    // dtw[jb - 1]
-    float v;    // row value at tour[jb - 1]
+// Customers at tour positions [1, x], x >= 1: ridx[x] counts the depots at
+// positions [1, x-1].  Swaps and the relocate shift move each node's route
+// index with it and keep the order of depots, so this stays exact while
+// the search runs.  The scans count their evaluated moves with it: the
+// candidates a sequential scan evaluates up to position j are the
+// customers between its first candidate and j.
+__device__ __forceinline__ int customers_upto(const LsLds &L, int x)
+{
+    return x - L.ridx[x] - (L.tour[x] == 0 ? 1 : 0);
+}
+
+// A trip of a scan covers kScanUnroll chunks of 64 consecutive candidates,
+// one per lane.  Each lane reads its candidate's neighbours in the tour and
+// edge arrays directly (past the tour's end the padding reads as depots and
+// zero edges), and the gain and the mask of improving candidates are formed
+// without branches; only a chunk with an improving candidate takes the
+// feasibility checks.
+//
+// Row entries are summed in their own type before one conversion: fp32 rows
+// add in the order the reference's expression does; 16-bit rows hold
+// integers, whose sums below 2^24 are exact, so converting the integer sum
+// gives the same float as adding the converted entries.
+template <typename RT> struct RowSum;
+template <> struct RowSum<float> {
+    using T = float;
+    static __device__ __forceinline__ float f(float x) { return x; }
+};
+template <> struct RowSum<unsigned short> {
+    using T = unsigned;
+    static __device__ __forceinline__ float f(unsigned x) { return (float)x; }
 };
 
 // Swap of position i with position j (localSearch.cpp:301-410).  n2 =
 // tour[j], dm/d0 the edges (pn2, n2) and (n2, sn2), rA/rC = d(pn1, n2),
 // d(sn1, n2), rBp/rBs = d(n1, pn2), d(n1, sn2).  The gain is the
-// reference's expression for adjacent and general positions.
-__device__ __forceinline__ float swap_gain(int i, int j, float a1, float b1, float rA, float rC,
-                                           float dm, float d0, float rBp, float rBs)
-{
-    if (j == i + 1) return -(a1 + d0) + (rA + rBs);
-    return -(a1 + b1 + dm + d0) + (rA + rC + rBp + rBs);
-}
-
+// reference's expression: -(a1 + b1 + dm + d0) + (rA + rC + rBp + rBs), or
+// -(a1 + d0) + (rA + rBs) for adjacent positions (j = i + 1).
+//
 // Loads and route lengths stay feasible (checked only for improving moves
 // between two routes).
 __device__ __forceinline__ bool swap_feasible(const DevProblem &p, const LsLds &L, int j, int r1, int dn1,
@@ -583,77 +597,71 @@ __device__ __forceinline__ bool swap_feasible(const DevProblem &p, const LsLds &
 }
 
 // Position i against the current tour; returns the first accepted j or -1
-// and in evc the moves the sequential scan evaluates up to it.  A trip's
-// common case (no improving candidate) takes no branch per chunk; the
-// feasibility checks run only for chunks with an improving candidate.
+// and in evc the moves the sequential scan evaluates up to it.
 template <typename RT>
 __device__ __forceinline__ int swap_scan(const DevProblem &p, const LsLds &L, int S, int i,
                                          const RT *rowA, const RT *rowB, const RT *rowC,
                                          int lane, const ScanStop &stop, int &evc)
 {
+    using V = typename RowSum<RT>::T;
     constexpr int U = kScanUnroll;
     const int n1 = L.tour[i];
     const int sn1 = L.tour[i + 1], r1 = L.ridx[i];
     const float a1 = rget(rowA, n1), b1 = rget(rowB, sn1);
     const int dn1 = L.dem[n1];
-    int cnt = 0;
-    TripEdge pe = {L.dtw[i], rget(rowB, n1)};   // position jb - 1 = i
     for (int jb = i + 1; jb < S; jb += U * kLanes) {
         const unsigned key = read_key(stop.key);
-        int t[U];
-        float d[U], v[U];
-        // past the tour's end the padding reads as depots and zero edges;
-        // every position j <= S-2 that can be accepted sees its true
-        // neighbours
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            t[u] = L.tour[jb + u * kLanes + lane];
-            d[u] = L.dtw[jb + u * kLanes + lane];
-        }
-        const int ta = L.tour[jb + U * kLanes];
-#pragma unroll
-        for (int u = 0; u < U; ++u) v[u] = rget(rowB, t[u]);
-        const float va = rget(rowB, ta);
-        float rA[U], rC[U], dm[U], rBp[U], rBs[U];
-        bool imp[U], val[U], any = false;
+        int tp[U], t[U], tn[U];
+        float dm[U], d0[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int j = jb + u * kLanes + lane;
-            const float dp = u == 0 ? pe.d : readlane_f32(d[u - 1], kLanes - 1);
-            const float vp = u == 0 ? pe.v : readlane_f32(v[u - 1], kLanes - 1);
-            const float vn = u == U - 1 ? va : readlane_f32(v[u + 1], 0);
-            rA[u] = rget(rowA, t[u]);
-            rC[u] = rget(rowC, t[u]);
-            dm[u] = wave_prev(d[u], dp);
-            rBp[u] = wave_prev(v[u], vp);
-            rBs[u] = wave_next(v[u], vn);
-            val[u] = j < S && t[u] != 0;
-            imp[u] = val[u] && swap_gain(i, j, a1, b1, rA[u], rC[u], dm[u], d[u], rBp[u], rBs[u]) < -kEps;
-            any = any || imp[u];
+            tp[u] = L.tour[j - 1];
+            t[u] = L.tour[j];
+            tn[u] = L.tour[j + 1];
+            dm[u] = L.dtw[j - 1];
+            d0[u] = L.dtw[j];
+        }
+        V rA[U], rC[U], rBp[U], rBs[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            rA[u] = rowA[t[u]];
+            rC[u] = rowC[t[u]];
+            rBp[u] = rowB[tp[u]];
+            rBs[u] = rowB[tn[u]];
+        }
+        bool imp[U], any = false;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            float g = -(a1 + b1 + dm[u] + d0[u]) + RowSum<RT>::f(rA[u] + rC[u] + rBp[u] + rBs[u]);
+            if (u == 0) {
+                // j = i + 1 (adjacent) is lane 0 of the first chunk of the first trip
+                const float adj = -(a1 + d0[u]) + RowSum<RT>::f(rA[u] + rBs[u]);
+                g = jb + lane == i + 1 ? adj : g;
+            }
+            // depots are no candidates; j >= S reads a padding depot
+            imp[u] = (t[u] != 0) & (g < -kEps);
+            any = any | imp[u];
         }
         if (__ballot(any)) {
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int j = jb + u * kLanes + lane;
-                const bool ok = imp[u] && swap_feasible(p, L, j, r1, dn1, a1, b1, t[u], rA[u], rC[u], dm[u],
-                                                        d[u], rBp[u], rBs[u]);
-                const u64 mk = __ballot(ok), mv = __ballot(val[u]);
+                const bool ok = imp[u] && swap_feasible(p, L, j, r1, dn1, a1, b1, t[u], RowSum<RT>::f(rA[u]),
+                                                        RowSum<RT>::f(rC[u]), dm[u], d0[u],
+                                                        RowSum<RT>::f(rBp[u]), RowSum<RT>::f(rBs[u]));
+                const u64 mk = __ballot(ok);
                 if (mk) {
-                    const int f = __ffsll(mk) - 1;
-                    evc = cnt + __popcll(mv & lanes_upto(f));
-                    return jb + u * kLanes + f;
+                    const int js = jb + u * kLanes + (__ffsll(mk) - 1);
+                    evc = customers_upto(L, js) - customers_upto(L, i);
+                    return js;
                 }
-                cnt += __popcll(mv);
             }
-        } else {
-#pragma unroll
-            for (int u = 0; u < U; ++u) cnt += __popcll(__ballot(val[u]));
         }
+        // stopped early: a lower wave has a move and this result is dropped
         if (key > stop.mine) break;
-        pe.d = readlane_f32(d[U - 1], kLanes - 1);
-        pe.v = readlane_f32(v[U - 1], kLanes - 1);
     }
-    evc = cnt;
+    evc = customers_upto(L, S - 1) - customers_upto(L, i);
     return -1;
 }
 
@@ -769,73 +777,72 @@ __device__ __forceinline__ bool relocate_feasible(const DevProblem &p, const LsL
     return !(L.rl[r2] + dn1 > p.Q) && !(dist2 > p.D) && !(dist1 > p.D);
 }
 
+// Candidates are taken by edge: inserting n1 on edge q = (tour[q],
+// tour[q+1]) is candidate j = q + 1 (before tour[j], rm = edge j - 1) when
+// q < i and candidate j = q (after tour[j], rm = edge j) when q > i; edges
+// i - 1 and i touch n1 itself (j = i).  j grows with q, so edges are taken
+// in the sequential scan's order, and every candidate's gain has the same
+// form: add = d(n1, tour[q]) + d(n1, tour[q+1]), rm = the edge's weight.
 template <typename RT>
 __device__ __forceinline__ int relocate_scan(const DevProblem &p, const LsLds &L, int S, int i,
                                              const RT *rowP, const RT *rowB, int lane,
                                              const ScanStop &stop, int &evc)
 {
+    using V = typename RowSum<RT>::T;
     constexpr int U = kScanUnroll;
     const int n1 = L.tour[i];
     const int r1 = L.ridx[i], sn1 = L.tour[i + 1];
     const float a1 = L.dtw[i - 1], b1 = L.dtw[i], c1 = rget(rowP, sn1);
     const int dn1 = L.dem[n1];
-    int cnt = 0;
-    TripEdge pe = {L.dtw[0], rget(rowB, L.tour[0])};   // position jb - 1 = 0
-    for (int jb = 1; jb < S - 1; jb += U * kLanes) {
+    for (int qb = 0; qb < S - 1; qb += U * kLanes) {
         const unsigned key = read_key(stop.key);
-        int t[U];
-        float d[U], v[U];
-        // past the tour's end the padding reads as depots and zero edges;
-        // every position j <= S-2 that can be accepted sees its true
-        // neighbours
+        int t0[U], t1[U];
+        float rm[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            t[u] = L.tour[jb + u * kLanes + lane];
-            d[u] = L.dtw[jb + u * kLanes + lane];
+            const int q = qb + u * kLanes + lane;
+            t0[u] = L.tour[q];
+            t1[u] = L.tour[q + 1];
+            rm[u] = L.dtw[q];
         }
-        const int ta = L.tour[jb + U * kLanes];
-#pragma unroll
-        for (int u = 0; u < U; ++u) v[u] = rget(rowB, t[u]);
-        const float va = rget(rowB, ta);
-        float rm[U], add[U];
-        bool imp[U], val[U], any = false;
+        V v0[U], v1[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int j = jb + u * kLanes + lane;
-            const float dp = u == 0 ? pe.d : readlane_f32(d[u - 1], kLanes - 1);
-            const float vp = u == 0 ? pe.v : readlane_f32(v[u - 1], kLanes - 1);
-            const float vn = u == U - 1 ? va : readlane_f32(v[u + 1], 0);
-            // every lane takes part in the lane shifts; each lane then picks
-            const float dmu = wave_prev(d[u], dp), vpu = wave_prev(v[u], vp), vnu = wave_next(v[u], vn);
-            const bool before = i > j;
-            rm[u] = before ? dmu : d[u];
-            add[u] = v[u] + (before ? vpu : vnu);
-            val[u] = j < S - 1 && j != i && t[u] != 0;
-            imp[u] = val[u] && relocate_gain(a1, b1, c1, rm[u], add[u]) < -kEps;
-            any = any || imp[u];
+            v0[u] = rowB[t0[u]];
+            v1[u] = rowB[t1[u]];
+        }
+        float add[U];
+        bool imp[U], any = false;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int q = qb + u * kLanes + lane;
+            // the node at j; q >= S - 1 reads the closing depot or padding
+            const int tj = q < i ? t1[u] : t0[u];
+            add[u] = RowSum<RT>::f(v0[u] + v1[u]);
+            imp[u] = (tj != 0) & (tj != n1) & (relocate_gain(a1, b1, c1, rm[u], add[u]) < -kEps);
+            any = any | imp[u];
         }
         if (__ballot(any)) {
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const int j = jb + u * kLanes + lane;
+                const int q = qb + u * kLanes + lane;
+                const int j = q < i ? q + 1 : q;
                 const bool ok = imp[u] && relocate_feasible(p, L, j, r1, dn1, a1, b1, c1, rm[u], add[u]);
-                const u64 mk = __ballot(ok), mv = __ballot(val[u]);
+                const u64 mk = __ballot(ok);
                 if (mk) {
-                    const int f = __ffsll(mk) - 1;
-                    evc = cnt + __popcll(mv & lanes_upto(f));
-                    return jb + u * kLanes + f;
+                    const int qf = qb + u * kLanes + (__ffsll(mk) - 1);
+                    const int js = qf < i ? qf + 1 : qf;
+                    // customers in [1, js] other than n1 itself
+                    evc = customers_upto(L, js) - (i <= js ? 1 : 0);
+                    return js;
                 }
-                cnt += __popcll(mv);
             }
-        } else {
-#pragma unroll
-            for (int u = 0; u < U; ++u) cnt += __popcll(__ballot(val[u]));
         }
+        // stopped early: a lower wave has a move and this result is dropped
         if (key > stop.mine) break;
-        pe.d = readlane_f32(d[U - 1], kLanes - 1);
-        pe.v = readlane_f32(v[U - 1], kLanes - 1);
     }
-    evc = cnt;
+    // every customer position in [1, S-2] but i
+    evc = customers_upto(L, S - 2) - 1;
     return -1;
 }
 
