@@ -519,7 +519,9 @@ int aco_create_batch(const aco_problem *pbs, int32_t count, const aco_params *pr
         const int n = pb->n, width = std::max(k, nn_ls[b]), nl = nn_ls[b];
         const int nnP = (nl + 3) & ~3;
         I.n = n;
-        I.P = (n + 3) & ~3;
+        // row pitch: every row of the fp32 and 16-bit matrices starts on a
+        // 16-byte boundary (local search copies rows in 16-byte pieces)
+        I.P = (n + 7) & ~7;
         I.T = 2 * n;
         I.h_dist.assign(pb->dist, pb->dist + (size_t)n * n);
         for (float d : I.h_dist)

@@ -13,7 +13,13 @@ __global__ __launch_bounds__(W * 64) void k_local_search(DevBatch bt, int first,
     extern __shared__ __align__(16) unsigned char smem[];
     const int g = bt.ls_order[first + blockIdx.x];   // this launch's size class
     const int inst = bt.ant_inst[g];
-    local_search_ant<W, RT>(bt.probs[inst], bt.states[inst], g - bt.ant_base[inst], flags, ring, prefetch, smem);
+    // copies in registers, read once before the first barrier: fields read
+    // through the global records would be loaded again after every barrier,
+    // and each such load waits for all outstanding memory operations,
+    // in-flight row copies included
+    const DevProblem p = bt.probs[inst];
+    const DevState s = bt.states[inst];
+    local_search_ant<W, RT>(p, s, g - bt.ant_base[inst], flags, ring, prefetch, smem);
 }
 
 bool ls_shape_valid(const LsShape &sh)

@@ -72,7 +72,8 @@ __host__ __device__ inline LsOffsets ls_offsets(int n, int ring, int rowbytes)
     o.rl = at;   at = ls_align16(at + 4 * N);
     o.rd = at;   at = ls_align16(at + 4 * N);
     o.dem = at;  at = ls_align16(at + 4 * N);
-    o.rows = at; at = ls_align16(at + (size_t)rowbytes * (size_t)(rowbytes == 4 ? n : (n + 1) & ~1) * (size_t)ring);
+    // row slots of a whole number of 16-byte pieces (the row copy's unit)
+    o.rows = at; at = ls_align16(at + (size_t)rowbytes * (size_t)(rowbytes == 4 ? (n + 3) & ~3 : (n + 7) & ~7) * (size_t)ring);
     o.res = at;  at = ls_align16(at + 4 * kResWords);
     o.pos = at;  at = ls_align16(at + 2 * N);
     o.rof = at;  at = ls_align16(at + 2 * N);
@@ -393,14 +394,14 @@ __device__ __forceinline__ float rget(const unsigned short *r, int i) { return (
 template <typename RT> struct RowRing {
     RT *rows;
     int mask;     // ring size - 1
-    int stride;   // elements per row slot: n rounded up to a whole number of dwords
+    int stride;   // elements per row slot: n rounded up to a whole number of 16-byte pieces
     __device__ __forceinline__ int slot(int pos) const { return pos & mask; }
     __device__ __forceinline__ RT *at(int pos) const { return rows + slot(pos) * stride; }
 };
 
 template <typename RT> __host__ __device__ constexpr int row_stride(int n)
 {
-    return sizeof(RT) == 4 ? n : (n + 1) & ~1;
+    return sizeof(RT) == 4 ? (n + 3) & ~3 : (n + 7) & ~7;
 }
 
 // Source rows in global memory: fp32 distances, or their 16-bit copy.
@@ -411,20 +412,22 @@ __device__ __forceinline__ const unsigned short *row_src(const DevProblem &p, in
 }
 
 // Copy distance row v into LDS with direct global->LDS loads issued by one
-// wave (one dword per lane per instruction, no registers; completes with
-// vmcnt, so readers wait for vmcnt(0) and a barrier).  Rows of both types
-// start dword aligned and the copy may run up to one padding element past
-// n (within P and within the slot stride).
+// wave: 16 bytes per lane per instruction (one instruction for a 16-bit
+// row of up to 512 nodes), no registers; completes with vmcnt, so readers
+// wait for vmcnt(0) and a barrier.  Source rows start on 16-byte
+// boundaries (the row pitch P is a multiple of 8 and the matrices are
+// 256-byte aligned) and so do the ring slots; the copy may run past n up
+// to the next 16-byte boundary, which stays within P and within the slot.
 template <typename RT>
 __device__ __forceinline__ void row_copy_wave(const DevProblem &p, int v, RT *dst, int lane)
 {
-    const int nd = (p.n * (int)sizeof(RT) + 3) >> 2;
-    const unsigned *src = reinterpret_cast<const unsigned *>(row_src(p, v, (RT *)nullptr));
-    unsigned *d = reinterpret_cast<unsigned *>(dst);
-    for (int c = 0; c < nd; c += kLanes) {
-        if (c + lane < nd)
+    const int nq = (p.n * (int)sizeof(RT) + 15) >> 4;
+    const uint4 *src = reinterpret_cast<const uint4 *>(row_src(p, v, (RT *)nullptr));
+    uint4 *d = reinterpret_cast<uint4 *>(dst);
+    for (int c = 0; c < nq; c += kLanes) {
+        if (c + lane < nq)
             __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(src + c + lane),
-                                             (__attribute__((address_space(3))) void *)(d + c), 4, 0, 0);
+                                             (__attribute__((address_space(3))) void *)(d + c), 16, 0, 0);
     }
 }
 
