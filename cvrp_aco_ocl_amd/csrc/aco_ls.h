@@ -685,7 +685,8 @@ __device__ __forceinline__ void ls_swap(const DevProblem &p, LsLds &L, const Row
         int j = -1, evc = 0;
         const ScanStop stop = {reinterpret_cast<const unsigned *>(L.res + LsW<W>::res_first),
                                round_key(++round, w)};
-        if (i < S && L.tour[i] != 0)
+        // uniform per wave (scalar branch; padding past S reads as depots)
+        if (i < S && __builtin_amdgcn_readfirstlane(L.tour[i]) != 0)
             j = swap_scan(p, L, S, i, rr.at(i - 1), rr.at(i), rr.at(i + 1), lane, stop, evc);
         if (j >= 0 && lane == 0) atomicMax(reinterpret_cast<unsigned *>(L.res + LsW<W>::res_first), stop.mine);
         int *res = L.res + W * parity, *rese = L.res + LsW<W>::res_ev + W * parity;
@@ -867,7 +868,11 @@ __device__ __forceinline__ void ls_relocate(const DevProblem &p, LsLds &L, const
         int j = -1, evc = 0;
         const ScanStop stop = {reinterpret_cast<const unsigned *>(L.res + LsW<W>::res_first),
                                round_key(++round, w)};
-        if (i < S - 1 && L.tour[i] != 0 && L.rc[L.ridx[i]] >= 2)
+        // uniform per wave (scalar branches); the route must keep a customer
+        bool scan = i < S - 1;
+        if (scan) scan = __builtin_amdgcn_readfirstlane(L.tour[i]) != 0;
+        if (scan) scan = __builtin_amdgcn_readfirstlane(L.rc[L.ridx[i]]) >= 2;
+        if (scan)
             j = relocate_scan(p, L, S, i, rr.at(i - 1), rr.at(i), lane, stop, evc);
         if (j >= 0 && lane == 0) atomicMax(reinterpret_cast<unsigned *>(L.res + LsW<W>::res_first), stop.mine);
         int *res = L.res + W * parity, *rese = L.res + LsW<W>::res_ev + W * parity;
@@ -972,7 +977,8 @@ __device__ void local_search_ant(const DevProblem &p, const DevState &s, int a, 
     LsLds L = ls_bind(p.n, ring, (int)sizeof(RT), smem);
     const RowRing<RT> rr = {reinterpret_cast<RT *>(L.rows), ring - 1, row_stride<RT>(p.n)};
     const bool pf_on = prefetch != 0;
-    const int S = s.sizes[a];
+    // uniform: loop bounds, row windows and masks stay in scalar registers
+    const int S = __builtin_amdgcn_readfirstlane(s.sizes[a]);
     int *gt = s.tours + (size_t)a * p.T;
     for (int q = tid; q < S; q += NT) L.tour[q] = gt[q];
     // padding past the tour's end: depots and zero edges (never written
