@@ -185,8 +185,9 @@ __device__ __forceinline__ void group_sync()
     __builtin_amdgcn_wave_barrier();
 }
 
-// One scan for n1 (SUCC: partner edges (n1,s1),(n2,s2); otherwise
-// (p1,n1),(p2,n2); nb1 is s1 or p1).  The reference walks n1's nn row,
+// The two scans for n1: the successor scan (partner edges (n1,s1),(n2,s2),
+// radius d(n1,s1)) and, when it finds no move, the predecessor scan (edges
+// (p1,n1),(p2,n2), radius d(p1,n1)).  The reference walks n1's nn row,
 // skips nodes of other routes and stops at the first in-route node with
 // radius - d <= EPS (kernel.cl:180-230).  Rows are sorted by distance, so
 // the nodes it evaluates are exactly the in-route nodes with
@@ -194,29 +195,34 @@ __device__ __forceinline__ void group_sync()
 // improving one earliest in the row.  Here the route's own positions are
 // scanned (independent loads, no walk over other routes' neighbours) and
 // the improving candidate with the smallest row position nnpos[n1][n2] is
-// kept (row positions are distinct, so the group minimum is unique).
-// Returns true with h1..h4 set on an improving move (same in all lanes of
-// the group).
-// Candidate q of a scan (q may lie past rend: the caller masks it).  Loads
-// are unconditional (clamped, in-bounds indices) so a lane's candidates
-// have all their loads in flight at once.  ok: in the row (rp < nn_ls),
-// inside the radius and, for the predecessor variant, not adjacent; gain as
-// the reference computes it; x is s2 or p2.
-template <bool SUCC>
-__device__ __forceinline__ void two_opt_cand(const DevProblem &p, const LsLds &L, int rend, int pos1,
-                                             int q, int n1, int nb1, float radius,
-                                             const unsigned short *rowpos, bool &ok, float &gain,
-                                             int &rp, int &n2, int &x)
+// kept (row positions are distinct, so the group minimum is unique).  Both
+// scans take the same candidates n2, with the same row positions and
+// d(n1, n2), so one pass evaluates both: one round of memory latency per
+// position instead of two.
+//
+// Candidate q (q may lie past rend: the caller masks it).  Loads are
+// unconditional (clamped, in-bounds indices) so a lane's candidates have
+// all their loads in flight at once.  ok*: in the row (rp < nn_ls), inside
+// the radius and, for the predecessor scan, not adjacent; gains as the
+// reference computes them; xs = s2, xp = p2.
+__device__ __forceinline__ void two_opt_cand(const DevProblem &p, const LsLds &L, int rend, int pos1, int q,
+                                             int n1, int s1, int p1, float rads, float radp,
+                                             const unsigned short *rowpos, bool &oks, bool &okp, float &gs,
+                                             float &gp, int &rp, int &n2, int &xs, int &xp)
 {
     const int qq = q <= rend ? q : rend;
     n2 = L.tour[qq];
-    x = SUCC ? (qq == rend ? 0 : L.tour[qq + 1]) : L.tour[qq - 1];
-    const float ew = SUCC ? L.dtw[qq] : L.dtw[qq - 1];
+    xs = qq == rend ? 0 : L.tour[qq + 1];
+    xp = L.tour[qq - 1];
+    const float ews = L.dtw[qq], ewp = L.dtw[qq - 1];
     rp = rowpos[n2];
     const float dn = Dd(p, n1, n2);
-    const float dx = Dd(p, nb1, x);
-    ok = q <= rend && q != pos1 && rp < p.nn_ls && radius - dn > kEps && (SUCC || !(x == n1 || nb1 == n2));
-    gain = -radius + dn + dx - ew;
+    const float dxs = Dd(p, s1, xs), dxp = Dd(p, p1, xp);
+    const bool base = (q <= rend) & (q != pos1) & (rp < p.nn_ls);
+    oks = base & (rads - dn > kEps);
+    okp = base & (radp - dn > kEps) & !(xp == n1 || p1 == n2);
+    gs = -rads + dn + dxs - ews;
+    gp = -radp + dn + dxp - ewp;
 }
 
 // Candidates per lane per trip of a 2-opt scan: positions q, q+8, q+16 and
@@ -224,70 +230,80 @@ __device__ __forceinline__ void two_opt_cand(const DevProblem &p, const LsLds &L
 // up to 32 customers take one trip, i.e. one memory latency).
 constexpr int kOptUnroll = 4;
 
-template <bool SUCC>
-__device__ __forceinline__ bool two_opt_scan(const DevProblem &p, const LsLds &L, int rbeg,
-                                             int rend, int pos1, int n1, int nb1, float radius,
-                                             int gl, long long &evals, int &h1, int &h2, int &h3,
-                                             int &h4)
+// Returns 0 (no move), 1 (successor scan's move) or 2 (predecessor scan's
+// move) with h1..h4 set (same in all lanes of the group).  The evaluated
+// moves are the sequential pair's: the successor scan's up to its move, or
+// all of them plus the predecessor scan's up to its move.
+__device__ __forceinline__ int two_opt_scans(const DevProblem &p, const LsLds &L, int rbeg, int rend, int pos1,
+                                             int n1, int s1, int p1, float rads, float radp, int gl,
+                                             long long &evals, int &h1, int &h2, int &h3, int &h4)
 {
     constexpr int U = kOptUnroll;
     const unsigned short *rowpos = p.nnpos + (size_t)n1 * p.n;
-    u64 best = ~0ull;
-    int cnt = 0;
+    u64 bs = ~0ull, bp = ~0ull;
+    int cs = 0, cp = 0;
     // the first trip's candidates, kept for the evaluated-move count
-    bool okk[U];
-    int rpk[U];
+    bool oks_k[U], okp_k[U];
+    int rp_k[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) { okk[u] = false; rpk[u] = 0; }
+    for (int u = 0; u < U; ++u) { oks_k[u] = okp_k[u] = false; rp_k[u] = 0; }
     for (int q0 = rbeg + 1 + gl; q0 <= rend; q0 += U * kOptGroup) {
-        bool ok[U];
-        float g[U];
-        int rp[U], a[U], x[U];
+        bool oks[U], okp[U];
+        float gs[U], gp[U];
+        int rp[U], a[U], xs[U], xp[U];
 #pragma unroll
         for (int u = 0; u < U; ++u)
-            two_opt_cand<SUCC>(p, L, rend, pos1, q0 + u * kOptGroup, n1, nb1, radius, rowpos, ok[u], g[u],
-                               rp[u], a[u], x[u]);
+            two_opt_cand(p, L, rend, pos1, q0 + u * kOptGroup, n1, s1, p1, rads, radp, rowpos, oks[u], okp[u],
+                         gs[u], gp[u], rp[u], a[u], xs[u], xp[u]);
         const bool first = q0 == rbeg + 1 + gl;
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            cnt += ok[u];
-            if (ok[u] && g[u] < -kEps) {
-                const u64 key = ((u64)rp[u] << 32) | ((u64)a[u] << 16) | (u64)x[u];
-                best = key < best ? key : best;
-            }
-            if (first) { okk[u] = ok[u]; rpk[u] = rp[u]; }
+            cs += oks[u];
+            cp += okp[u];
+            const u64 kr = ((u64)rp[u] << 32) | ((u64)a[u] << 16);
+            if (oks[u] && gs[u] < -kEps) bs = (kr | (u64)xs[u]) < bs ? (kr | (u64)xs[u]) : bs;
+            if (okp[u] && gp[u] < -kEps) bp = (kr | (u64)xp[u]) < bp ? (kr | (u64)xp[u]) : bp;
+            if (first) { oks_k[u] = oks[u]; okp_k[u] = okp[u]; rp_k[u] = rp[u]; }
         }
     }
-    best = group_min_u64(best);
-    if (best == ~0ull) {
-        evals += cnt;
-        return false;
+    int kind = 0;
+    u64 best = group_min_u64(bs);
+    if (best != ~0ull) {
+        kind = 1;
+    } else {
+        best = group_min_u64(bp);
+        if (best != ~0ull) kind = 2;
+    }
+    if (kind == 0) {
+        evals += cs + cp;
+        return 0;
     }
     // the reference's walk stops at the accepted candidate: it evaluated
     // the candidates up to that row position
     const int rps = (int)(best >> 32);
-    cnt = 0;
+    int cnt = 0;
     if (rend - rbeg <= U * kOptGroup) {
 #pragma unroll
-        for (int u = 0; u < U; ++u) cnt += okk[u] && rpk[u] <= rps;
+        for (int u = 0; u < U; ++u) cnt += (kind == 1 ? oks_k[u] : okp_k[u]) && rp_k[u] <= rps;
     } else {
         for (int q0 = rbeg + 1 + gl; q0 <= rend; q0 += U * kOptGroup) {
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                bool ok;
-                float g;
-                int rp, a, x;
-                two_opt_cand<SUCC>(p, L, rend, pos1, q0 + u * kOptGroup, n1, nb1, radius, rowpos, ok, g, rp,
-                                   a, x);
-                cnt += ok && rp <= rps;
+                bool oks, okp;
+                float gs, gp;
+                int rp, a, xs, xp;
+                two_opt_cand(p, L, rend, pos1, q0 + u * kOptGroup, n1, s1, p1, rads, radp, rowpos, oks, okp, gs,
+                             gp, rp, a, xs, xp);
+                cnt += (kind == 1 ? oks : okp) && rp <= rps;
             }
         }
     }
-    evals += cnt;
+    // the predecessor scan runs after the whole successor scan
+    evals += cnt + (kind == 2 ? cs : 0);
     const int b2 = (int)((best >> 16) & 0xFFFF), bx = (int)(best & 0xFFFF);
-    if (SUCC) { h1 = n1; h2 = nb1; h3 = b2; h4 = bx; }
-    else { h1 = nb1; h2 = n1; h3 = bx; h4 = b2; }
-    return true;
+    if (kind == 1) { h1 = n1; h2 = s1; h3 = b2; h4 = bx; }
+    else { h1 = p1; h2 = n1; h3 = bx; h4 = b2; }
+    return kind;
 }
 
 // kernel.cl:126-277 for route r at positions [rbeg, rend] (rbeg holds the
@@ -307,17 +323,13 @@ __device__ __forceinline__ void two_opt_route(const DevProblem &p, LsLds &L, int
             if (n1 == 0 ? dlb0 : (L.dlb[n1] != 0)) continue;
             int h1 = 0, h2 = 0, h3 = 0, h4 = 0;
             const int s1 = pos1 == rend ? 0 : L.tour[pos1 + 1];
-            // d(n1, s1) is the edge leaving position pos1
-            bool found = two_opt_scan<true>(p, L, rbeg, rend, pos1, n1, s1, L.dtw[pos1], gl, evals,
-                                            h1, h2, h3, h4);
-            if (!found) {
-                const int p1 = pos1 == rbeg ? L.tour[rend] : L.tour[pos1 - 1];
-                // d(p1, n1): the edge entering pos1, or the closing edge
-                // (tour[rend], depot) when n1 is the depot
-                const float rad = pos1 == rbeg ? L.dtw[rend] : L.dtw[pos1 - 1];
-                found = two_opt_scan<false>(p, L, rbeg, rend, pos1, n1, p1, rad, gl, evals, h1, h2,
-                                            h3, h4);
-            }
+            const int p1 = pos1 == rbeg ? L.tour[rend] : L.tour[pos1 - 1];
+            // d(n1, s1) is the edge leaving position pos1; d(p1, n1) the edge
+            // entering it, or the closing edge (tour[rend], depot) when n1 is
+            // the depot
+            const float rads = L.dtw[pos1], radp = pos1 == rbeg ? L.dtw[rend] : L.dtw[pos1 - 1];
+            const bool found =
+                two_opt_scans(p, L, rbeg, rend, pos1, n1, s1, p1, rads, radp, gl, evals, h1, h2, h3, h4) != 0;
             if (!found) {
                 if (n1 == 0) dlb0 = true;
                 else if (gl == 0) L.dlb[n1] = 1;
