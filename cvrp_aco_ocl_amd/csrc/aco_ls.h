@@ -31,9 +31,8 @@ namespace cvrp {
 using u64 = unsigned long long;
 
 // LDS words of L.res for W waves (W <= 16): [0,W) and [W,2W) per-wave scan
-// results (accepted j) of even and odd rounds, [2W,4W) the moves each wave's
-// scan evaluated up to its result (same parity split), 4W the route count,
-// 4W+1 the lowest wave of the current round with a move (round key).
+// results (accepted j) of even and odd rounds, 4W the route count, 4W+1 the
+// lowest wave of the current round with a move (round key).
 constexpr int kResWords = 80;
 
 // Candidates of one scan trip, and the padding of the LDS tour and edge
@@ -44,7 +43,6 @@ constexpr int kScanPad = kScanUnroll * 64 + 4;
 template <int W> struct LsW {
     static_assert(W == 4 || W == 8 || W == 12 || W == 16, "4, 8, 12 or 16 waves per ant");
     static constexpr int threads = W * kLanes;
-    static constexpr int res_ev = 2 * W;
     static constexpr int res_r = 4 * W;
     static constexpr int res_first = 4 * W + 1;
 };
@@ -443,15 +441,21 @@ __device__ __forceinline__ void row_copy_wave(const DevProblem &p, int v, RT *ds
     }
 }
 
+// Window offsets q < 32 with q mod W == wave.
+template <int W> __device__ __forceinline__ unsigned wave_offsets(int wave)
+{
+    constexpr unsigned every = W == 4 ? 0x11111111u : W == 8 ? 0x01010101u : W == 12 ? 0x01001001u : 0x00010001u;
+    return every << wave;
+}
+
 // Copy the rows of the window offsets in `need` (bit q: position base + q):
-// the k-th of them is issued by wave k mod W.
+// offset q is issued by wave q mod W, so a run of up to W offsets is one row
+// per wave and each wave finds its own with a few scalar instructions.
 template <int W, typename RT>
 __device__ __forceinline__ void rows_issue(const DevProblem &p, const LsLds &L, const RowRing<RT> &rr,
                                            int base, unsigned need, int wave, int lane)
 {
-    int k = 0;
-    for (unsigned m = need; m; m &= m - 1u, ++k) {
-        if (k % W != wave) continue;
+    for (unsigned m = need & wave_offsets<W>(wave); m; m &= m - 1u) {
         const int pos = base + __builtin_ctz(m);
         row_copy_wave<RT>(p, __builtin_amdgcn_readfirstlane(L.tour[pos]), rr.at(pos), lane);
     }
@@ -515,27 +519,6 @@ __device__ __forceinline__ void prefetch_wait(RowWindow &w, unsigned want, int o
     w.mask |= want << off;
 }
 
-// After a round's barrier: the first wave (in position order) whose scan
-// accepted a move, that move's j (js), and in e the moves the sequential
-// scan evaluates up to and including that wave's position (all W positions
-// when no wave moved).  One LDS read per lane; the rest is uniform.
-template <int W>
-__device__ __forceinline__ int round_pick(const int *res, const int *rese, int lane, int &js, int &e)
-{
-    const int rv = lane < W ? res[lane] : -1;
-    const int re = lane < W ? rese[lane] : 0;
-    const u64 mk = __ballot(rv >= 0);
-    const int wstar = mk ? (int)__ffsll((unsigned long long)mk) - 1 : -1;
-    const int last = wstar < 0 ? W - 1 : wstar;
-    int sum = 0;
-#pragma unroll
-    for (int q = 0; q < W; ++q)
-        if (q <= last) sum += __builtin_amdgcn_readlane(re, q);
-    e = sum;
-    js = wstar < 0 ? -1 : __builtin_amdgcn_readlane(rv, wstar);
-    return wstar;
-}
-
 // -------------------------------------------------------------- swap ----
 
 // Early stop of speculative scans: a wave that accepts a move raises the
@@ -556,6 +539,22 @@ __device__ __forceinline__ unsigned round_key(unsigned round, int w) { return (r
 __device__ __forceinline__ unsigned read_key(const unsigned *key)
 {
     return *(volatile const __attribute__((address_space(3))) unsigned *)(key);
+}
+
+// After a round's barrier: the first wave (in position order) whose scan
+// accepted a move and that move's j (js), or -1.  The highest key of the
+// round is that wave's; a key of an earlier round means no wave moved.  Each
+// wave counts its own evaluated moves when its position is at or before that
+// wave's (all positions when none moved): the moves the sequential scan
+// evaluates.
+template <int W>
+__device__ __forceinline__ int round_pick(const int *res, const unsigned *key, unsigned round, int lane, int &js)
+{
+    const int rv = lane < W ? res[lane] : -1;
+    const unsigned k = __builtin_amdgcn_readfirstlane(read_key(key));
+    const int wstar = (k >> 5) == round ? 31 - (int)(k & 31u) : -1;
+    js = wstar < 0 ? -1 : __builtin_amdgcn_readlane(rv, wstar);
+    return wstar;
 }
 
 // Customers at tour positions [1, x], x >= 1: ridx[x] counts the depots at
@@ -701,16 +700,16 @@ __device__ __forceinline__ void ls_swap(const DevProblem &p, LsLds &L, const Row
         if (i < S && __builtin_amdgcn_readfirstlane(L.tour[i]) != 0)
             j = swap_scan(p, L, S, i, rr.at(i - 1), rr.at(i), rr.at(i + 1), lane, stop, evc);
         if (j >= 0 && lane == 0) atomicMax(reinterpret_cast<unsigned *>(L.res + LsW<W>::res_first), stop.mine);
-        int *res = L.res + W * parity, *rese = L.res + LsW<W>::res_ev + W * parity;
+        int *res = L.res + W * parity;
         parity ^= 1;
-        if (lane == 0) { res[w] = j; rese[w] = evc; }
+        if (lane == 0) res[w] = j;
         prefetch_wait(win, pfw, W + 2);
         __syncthreads();
         // the sequential scan evaluates the waves' positions in order and
         // stops at the first accepted move
-        int js, e;
-        const int wstar = round_pick<W>(res, rese, lane, js, e);
-        if (tid == 0) ev += e;
+        int js;
+        const int wstar = round_pick<W>(res, stop.key, round, lane, js);
+        if (lane == 0 && (wstar < 0 || w <= wstar)) ev += evc;
         if (wstar < 0) {
             // no barrier: the next round writes the other result buffer, and
             // rows it loads belong to positions no wave still reads
@@ -887,16 +886,16 @@ __device__ __forceinline__ void ls_relocate(const DevProblem &p, LsLds &L, const
         if (scan)
             j = relocate_scan(p, L, S, i, rr.at(i - 1), rr.at(i), lane, stop, evc);
         if (j >= 0 && lane == 0) atomicMax(reinterpret_cast<unsigned *>(L.res + LsW<W>::res_first), stop.mine);
-        int *res = L.res + W * parity, *rese = L.res + LsW<W>::res_ev + W * parity;
+        int *res = L.res + W * parity;
         parity ^= 1;
-        if (lane == 0) { res[w] = j; rese[w] = evc; }
+        if (lane == 0) res[w] = j;
         prefetch_wait(win, pfw, W + 1);
         __syncthreads();
         // the sequential scan evaluates the waves' positions in order and
         // stops at the first accepted move
-        int js, e;
-        const int wstar = round_pick<W>(res, rese, lane, js, e);
-        if (tid == 0) ev += e;
+        int js;
+        const int wstar = round_pick<W>(res, stop.key, round, lane, js);
+        if (lane == 0 && (wstar < 0 || w <= wstar)) ev += evc;
         if (wstar < 0) {
             // no barrier: the next round writes the other result buffer, and
             // rows it loads belong to positions no wave still reads
