@@ -75,7 +75,7 @@ __host__ __device__ inline LsOffsets ls_offsets(int n, int ring, int rowbytes)
     o.res = at;  at = ls_align16(at + 4 * kResWords);
     o.pos = at;  at = ls_align16(at + 2 * N);
     o.rof = at;  at = ls_align16(at + 2 * N);
-    o.ridx = at; at = ls_align16(at + 2 * T);
+    o.ridx = at; at = ls_align16(at + 2 * (T + kScanPad));
     o.zpos = at; at = ls_align16(at + 2 * (N + 1));
     o.rc = at;   at = ls_align16(at + 2 * N);
     o.dlb = at;  at = ls_align16(at + N);
@@ -798,8 +798,15 @@ __device__ __forceinline__ bool relocate_feasible(const DevProblem &p, const LsL
 // i - 1 and i touch n1 itself (j = i).  j grows with q, so edges are taken
 // in the sequential scan's order, and every candidate's gain has the same
 // form: add = d(n1, tour[q]) + d(n1, tour[q+1]), rm = the edge's weight.
+//
+// Without a route-length limit a move of n1 is feasible exactly when its
+// target route is n1's own or has room for dn1 (R <= 64 routes: one bit per
+// route, from one ballot); the trip then tests each candidate's route bit
+// with the other masks, and only a chunk with an improving feasible
+// candidate leaves the common path.  Otherwise every route bit is set and
+// the full check runs for improving candidates.
 template <typename RT>
-__device__ __forceinline__ int relocate_scan(const DevProblem &p, const LsLds &L, int S, int i,
+__device__ __forceinline__ int relocate_scan(const DevProblem &p, const LsLds &L, int S, int R, int i,
                                              const RT *rowP, const RT *rowB, int lane,
                                              const ScanStop &stop, int &evc)
 {
@@ -809,9 +816,14 @@ __device__ __forceinline__ int relocate_scan(const DevProblem &p, const LsLds &L
     const int r1 = L.ridx[i], sn1 = L.tour[i + 1];
     const float a1 = L.dtw[i - 1], b1 = L.dtw[i], c1 = rget(rowP, sn1);
     const int dn1 = L.dem[n1];
+    u64 room = ~0ull;
+    if (!p.has_d && R <= kLanes) {
+        const int rl = lane < R ? L.rl[lane] : 0;
+        room = __ballot(lane < R && (lane == r1 || rl + dn1 <= p.Q));
+    }
     for (int qb = 0; qb < S - 1; qb += U * kLanes) {
         const unsigned key = read_key(stop.key);
-        int t0[U], t1[U];
+        int t0[U], t1[U], r2[U];
         float rm[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -819,6 +831,8 @@ __device__ __forceinline__ int relocate_scan(const DevProblem &p, const LsLds &L
             t0[u] = L.tour[q];
             t1[u] = L.tour[q + 1];
             rm[u] = L.dtw[q];
+            // route of the candidate position j (q + 1 before i, q after)
+            r2[u] = L.ridx[q < i ? q + 1 : q];
         }
         V v0[U], v1[U];
 #pragma unroll
@@ -834,7 +848,8 @@ __device__ __forceinline__ int relocate_scan(const DevProblem &p, const LsLds &L
             // the node at j; q >= S - 1 reads the closing depot or padding
             const int tj = q < i ? t1[u] : t0[u];
             add[u] = RowSum<RT>::f(v0[u] + v1[u]);
-            imp[u] = (tj != 0) & (tj != n1) & (relocate_gain(a1, b1, c1, rm[u], add[u]) < -kEps);
+            imp[u] = (tj != 0) & (tj != n1) & (relocate_gain(a1, b1, c1, rm[u], add[u]) < -kEps) &
+                     (((room >> (r2[u] & 63)) & 1ull) != 0);
             any = any | imp[u];
         }
         if (__ballot(any)) {
@@ -868,6 +883,8 @@ __device__ __forceinline__ void ls_relocate(const DevProblem &p, LsLds &L, const
 {
     const int lane = tid & 63, w = wave_id();
     const int cap = kMaxMovesFactor * p.n;
+    // relocation never empties a route, so the route count stays
+    const int R = __builtin_amdgcn_readfirstlane(L.res[LsW<W>::res_r]);
     int done = 0, i0 = 1, parity = 0;
     RowWindow win = {0, 0u};
     while (i0 < S - 1 && done < cap) {
@@ -884,7 +901,7 @@ __device__ __forceinline__ void ls_relocate(const DevProblem &p, LsLds &L, const
         if (scan) scan = __builtin_amdgcn_readfirstlane(L.tour[i]) != 0;
         if (scan) scan = __builtin_amdgcn_readfirstlane(L.rc[L.ridx[i]]) >= 2;
         if (scan)
-            j = relocate_scan(p, L, S, i, rr.at(i - 1), rr.at(i), lane, stop, evc);
+            j = relocate_scan(p, L, S, R, i, rr.at(i - 1), rr.at(i), lane, stop, evc);
         if (j >= 0 && lane == 0) atomicMax(reinterpret_cast<unsigned *>(L.res + LsW<W>::res_first), stop.mine);
         int *res = L.res + W * parity;
         parity ^= 1;
