@@ -131,10 +131,10 @@ __device__ __forceinline__ bool feasible(const DevProblem &p, const AntLds &L, i
 // The same mask without branches (every condition evaluated), so the loads
 // feeding it are issued together.
 template <bool HAS_D>
-__device__ __forceinline__ bool feasible_mask(const DevProblem &p, const AntLds &L, int i, float dci,
+__device__ __forceinline__ bool feasible_mask(const DevProblem &p, const AntLds &L, int i, int demi, float dci,
                                               int load, float rdist)
 {
-    bool f = (L.vis[i] == 0) & (load + L.dem[i] <= p.Q);
+    bool f = (L.vis[i] == 0) & (load + demi <= p.Q);
     if (HAS_D) {
         float t = rdist + (dci + p.svc);
         t = t + L.d0[i];
@@ -323,11 +323,12 @@ __device__ void construct_knn_ant(const DevProblem &p, const DevState &s, int a,
         const int j = p.nnk[e];
         const float tj = s.tiknn[e];
         const float dcj = HAS_D ? p.nnkd[e] : 0.0f;
-        const bool ok = (lane < k) & feasible_mask<HAS_D>(p, L, j, dcj, load, rdist);
+        const int dj = L.dem[j];   // the chosen lane's value is read out below
+        const bool ok = (lane < k) & feasible_mask<HAS_D>(p, L, j, dj, dcj, load, rdist);
         const float pj = tj * (ok ? 1.0f : 0.0f);
         const float x = wave_scan_f32(pj, lane);
         const float total = lane_f32(x, k - 1);
-        int w;
+        int w, dw;
         float dcw;
         if (total > 0.0f) {
             const float u = RNG == kRngPhilox ? lane_f32(ublk, (step - 1) & 63) : pm_ran01(z);
@@ -341,19 +342,24 @@ __device__ void construct_knn_ant(const DevProblem &p, const DevState &s, int a,
                 wc = 63 - __clzll((long long)m2);
             }
             w = lane_i32(j, wc);
+            dw = lane_i32(dj, wc);
             dcw = HAS_D ? lane_f32(dcj, wc) : 0.0f;
         } else {
             const float *drow = p.dist + (size_t)cur * p.P;
             w = greedy_pick<HAS_D>(p, L, s.tinfo + (size_t)cur * p.P, drow, load, rdist, lane);
             fb++;
-            dcw = HAS_D ? drow[uniform(w)] : 0.0f;
+            w = uniform(w);
+            // scalar reads keep their waits on this path (see the staged
+            // kernel)
+            dw = uniform(L.dem[w]);
+            dcw = HAS_D ? drow[w] : 0.0f;
         }
         w = uniform(w);
         if (w == 0) {
             load = 0;
             rdist = 0.0f;
         } else {
-            load += L.dem[w];
+            load += dw;
             rdist = rdist + (dcw + p.svc);
             vcnt++;
         }
@@ -466,7 +472,10 @@ __global__ __launch_bounds__(64 * A) void k_construct_knn_lds(DevBatch bt)
             w = greedy_pick<false>(p, L, s.tinfo + (size_t)cur * p.P, nullptr, load, 0.0f, lane);
             fb++;
             w = uniform(w);
-            dw = dem[w];
+            // scalar, so the wait for this read stays on this path: a vector
+            // result would place it where the paths join, where every step
+            // would also wait for the visited-flag store
+            dw = uniform(dem[w]);
         }
         w = uniform(w);
         if (w == 0) {
