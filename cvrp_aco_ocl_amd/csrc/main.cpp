@@ -31,7 +31,8 @@ static void usage(const char *prog)
             "  --nn-ls D                         2-opt neighbour depth (nodes-2)\n"
             "  --tau0 first|nn                   initial trail rule (first)\n"
             "  --dist float|nint                 EUC_2D distances (float, as the reference)\n"
-            "  --seed S  --no-graph  --report DIR\n",
+            "  --seed S  --no-graph  --report DIR\n"
+            "  --sol DIR                         best tour of all tries as DIR/<name>.sol\n",
             prog);
     exit(1);
 }
@@ -42,11 +43,48 @@ static double now_s()
     return duration<double>(steady_clock::now().time_since_epoch()).count();
 }
 
+// The tour in the .sol layout of the reference's dataset (what the dataset's
+// .sol files hold): total cost, route count, two solver-metadata lines (0:
+// not recorded), then per route "0 1 1 cost load stops 0 c1 .. ck 0", costs
+// the route's distances summed in order, 6 significant digits.
+static bool write_sol(const Problem &inst, const std::string &file)
+{
+    const std::vector<int> &t = inst.best_tour;
+    const int n = inst.num_node;
+    std::vector<std::vector<int>> routes;
+    std::vector<int> cur{0};
+    for (size_t q = 1; q < t.size(); ++q) {
+        cur.push_back(t[q]);
+        if (t[q] == 0) {
+            if (cur.size() > 2) routes.push_back(cur);
+            cur.assign(1, 0);
+        }
+    }
+    std::vector<double> cost(routes.size());
+    double total = 0.0;
+    for (size_t r = 0; r < routes.size(); ++r) {
+        for (size_t q = 0; q + 1 < routes[r].size(); ++q)
+            cost[r] += inst.distance[(size_t)routes[r][q] * n + routes[r][q + 1]];
+        total += cost[r];
+    }
+    FILE *fp = fopen(file.c_str(), "w");
+    if (!fp) return false;
+    fprintf(fp, "%.6g\n%zu\n0\n0\n", total, routes.size());
+    for (size_t r = 0; r < routes.size(); ++r) {
+        int load = 0;
+        for (int v : routes[r]) load += inst.demand[v];
+        fprintf(fp, " 0 1 1 %.6g %d  %zu", cost[r], load, routes[r].size());
+        for (int v : routes[r]) fprintf(fp, " %d", v);
+        fprintf(fp, "\n");
+    }
+    return fclose(fp) == 0;
+}
+
 int main(int argc, char **argv)
 {
     Problem defaults;
     int tries = 15, dist_mode = -1;
-    const char *path = nullptr, *report_dir = nullptr;
+    const char *path = nullptr, *report_dir = nullptr, *sol_dir = nullptr;
     for (int a = 1; a < argc; ++a) {
         const std::string s = argv[a];
         auto val = [&]() -> const char * {
@@ -66,6 +104,7 @@ int main(int argc, char **argv)
         else if (s == "--seed") defaults.seed = strtoull(val(), nullptr, 10);
         else if (s == "--no-graph") defaults.use_graph = false;
         else if (s == "--report") report_dir = val();
+        else if (s == "--sol") sol_dir = val();
         else if (s == "--selection") {
             const std::string v = val();
             defaults.selection = v == "knn" ? ACO_SEL_KNN : ACO_SEL_IROULETTE;
@@ -91,6 +130,7 @@ int main(int argc, char **argv)
     if (!path) usage(argv[0]);
     if (defaults.selection == ACO_SEL_KNN && defaults.k <= 0) defaults.k = 20;
 
+    Problem best_inst;   // the try with the shortest best tour
     for (int ntry = 0; ntry < tries; ++ntry) {
         Problem inst = defaults;
         const double t0 = now_s();
@@ -136,6 +176,13 @@ int main(int argc, char **argv)
             fprintf(stderr, "invalid solution\n");
             exit(1);
         }
+        if (best_inst.best_tour.empty() || inst.best_length < best_inst.best_length) best_inst = inst;
+    }
+    std::string base = best_inst.name;   // the instance file's name without ".vrp"
+    if (base.size() > 4 && base.compare(base.size() - 4, 4, ".vrp") == 0) base.resize(base.size() - 4);
+    if (sol_dir && !write_sol(best_inst, std::string(sol_dir) + "/" + base + ".sol")) {
+        fprintf(stderr, "cannot write the .sol file in %s\n", sol_dir);
+        exit(1);
     }
     return 0;
 }

@@ -162,6 +162,37 @@ def read_solution(path: str):
     return total, routes
 
 
+def write_solution(path: str, inst: Instance, tour) -> float:
+    """Write ``tour`` (the reference's layout, depot 0 between routes) in the
+    ``.sol`` layout of the reference's dataset that ``read_solution`` reads:
+    total cost, route count, two solver-metadata lines (0 here: no solver
+    time or iteration is recorded), then per route
+    ``0 1 1 cost load stops 0 c1 ... ck 0``.  A route's cost is its distances
+    summed in order (no service time), numbers printed with 6 significant
+    digits as in the dataset's files.  Returns the total."""
+    t = [int(v) for v in np.asarray(tour).ravel()]
+    if not t or t[0] != 0 or t[-1] != 0:
+        raise ValueError("a tour starts and ends at the depot")
+    d = np.asarray(inst.dist)
+    dem = np.asarray(inst.demand)
+    routes, cur = [], [0]
+    for v in t[1:]:
+        cur.append(v)
+        if v == 0:
+            if len(cur) > 2:
+                routes.append(cur)
+            cur = [0]
+    lines, total = [], 0.0
+    for r in routes:
+        cost = float(sum(float(d[a, b]) for a, b in zip(r[:-1], r[1:])))
+        total += cost
+        load = int(sum(int(dem[v]) for v in r))
+        lines.append(f" 0 1 1 {cost:.6g} {load}  {len(r)} " + " ".join(str(v) for v in r))
+    with open(path, "w") as f:
+        f.write(f"{total:.6g}\n{len(routes)}\n0\n0\n" + "\n".join(lines) + "\n")
+    return total
+
+
 def solution_tour(routes) -> np.ndarray:
     """Concatenate routes into the reference's tour layout [0, .., 0, .., 0]."""
     t = [0]

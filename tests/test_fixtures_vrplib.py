@@ -14,7 +14,7 @@ import pytest
 
 import oracle
 import cvrp_aco_ocl_amd as ca
-from cvrp_aco_ocl_amd.instances import read_solution, solution_tour
+from cvrp_aco_ocl_amd.instances import read_solution, solution_tour, write_solution
 
 from helpers import VRPLIB
 
@@ -90,3 +90,30 @@ def test_oracle_distances_equal_product_distances():
         np.testing.assert_array_equal(oracle.distances(inst.x, inst.y, mode), inst.dist)
         inst1 = ca.Instance.from_vrplib(vrp, ca.DIST_NINT)
         np.testing.assert_array_equal(oracle.distances(inst.x, inst.y, 1), inst1.dist)
+
+
+@pytest.mark.parametrize("vrp", X_FILES[:3] + FLOAT_FILES[:2], ids=os.path.basename)
+def test_write_solution_round_trip(vrp, tmp_path):
+    """A fixture's solution written and read back: the same routes, loads
+    and (to the layout's 6 significant digits) costs."""
+    mode = ca.DIST_NINT if os.path.basename(vrp).startswith("X-") else ca.DIST_FLOAT
+    inst = ca.Instance.from_vrplib(vrp, mode)
+    total, routes = read_solution(vrp[:-4] + ".sol")
+    tour = solution_tour(routes)
+    out = tmp_path / "out.sol"
+    written = write_solution(str(out), inst, tour)
+    total2, routes2 = read_solution(str(out))
+    assert [r["nodes"] for r in routes2] == [r["nodes"] for r in routes]
+    assert [r["load"] for r in routes2] == [r["load"] for r in routes]
+    assert total2 == pytest.approx(written, rel=1e-5)
+    for r, r2 in zip(routes, routes2):
+        assert r2["cost"] == pytest.approx(r["cost"], rel=1e-5)
+    np.testing.assert_array_equal(solution_tour(routes2), tour)
+    if mode == ca.DIST_NINT:
+        assert total2 == total
+
+
+def test_write_solution_rejects_open_tour(tmp_path):
+    inst = ca.Instance.from_vrplib(X_FILES[0], ca.DIST_NINT)
+    with pytest.raises(ValueError):
+        write_solution(str(tmp_path / "x.sol"), inst, [1, 2, 0])

@@ -6,6 +6,8 @@ import subprocess
 
 import pytest
 
+import cvrp_aco_ocl_amd as ca
+from cvrp_aco_ocl_amd.instances import read_solution, solution_tour
 from helpers import VRPLIB
 
 pytestmark = pytest.mark.gpu
@@ -19,14 +21,22 @@ def _run(args, timeout=120):
 
 def test_cli_trials_on_cmt1(tmp_path):
     r = _run(["--tries", "2", "--iterations", "30", "--ls", "full", "--selection", "knn", "--k", "20",
-              "--report", str(tmp_path), os.path.join(VRPLIB, "CMT1.vrp")])
+              "--report", str(tmp_path), "--sol", str(tmp_path), os.path.join(VRPLIB, "CMT1.vrp")])
     assert r.returncode == 0, r.stderr
     lines = [l for l in r.stdout.splitlines() if l.startswith("try ")]
     assert len(lines) == 2 and all("valid yes" in l for l in lines)
-    best = float(lines[0].split("Best Length ")[1].split()[0])
-    assert 524.6 <= best < 1.1 * 524.61
+    lengths = [float(l.split("Best Length ")[1].split()[0]) for l in lines]
+    assert 524.6 <= lengths[0] < 1.1 * 524.61
     assert (tmp_path / "best.CMT1.vrp").exists()
     assert (tmp_path / "best_so_far.CMT1.vrp").read_text().count("iter") >= 1
+    # the best try's tour in the dataset's .sol layout
+    total, routes = read_solution(str(tmp_path / "CMT1.sol"))
+    inst = ca.Instance.from_vrplib(os.path.join(VRPLIB, "CMT1.vrp"))
+    tour = solution_tour(routes)
+    assert inst.check_solution(tour)
+    assert total == pytest.approx(min(lengths), rel=1e-4)
+    for rt in routes:
+        assert rt["load"] == int(inst.demand[rt["nodes"]].sum()) <= inst.capacity
 
 
 def test_cli_reference_defaults_with_distance_limit():
