@@ -81,3 +81,75 @@ def test_random_colony_matches_oracle(seed):
         np.testing.assert_array_equal(gtau, otau)
         np.testing.assert_array_equal(gti, oti)
         assert [(r[0], r[1]) for r in g.bsf_records()] == [(r[0], r[1]) for r in col.bsf()]
+
+
+def _batch(seed, fused):
+    """3-6 random instances of one rule set (context-wide parameters)."""
+    r = np.random.default_rng(9000 + seed)
+    insts = []
+    for b in range(int(r.integers(3, 7))):
+        inst, _ = _case(100 * seed + b)
+        if fused and not math.isinf(inst.max_dist):
+            # the persistent kernel: no route-length limit
+            inst = Instance.from_coords(inst.x, inst.y, inst.demand, inst.capacity, inst.dist_mode,
+                                        service_time=inst.service_time)
+        insts.append(inst)
+    int16 = all(bool(np.all(x.dist == np.round(x.dist)) and x.dist.max() <= 65535) for x in insts)
+    if fused:
+        sel, rng = (ca.SEL_KNN, ca.RNG_PHILOX) if r.random() < 0.5 else (ca.SEL_KNN, ca.RNG_PARK_MILLER)
+        w, ring, pf = [(4, 16, 1), (4, 16, 0), (8, 32, 1), (8, 16, 0)][int(r.integers(0, 4))]
+        ls = int(r.choice([1, 3, 7]))
+    else:
+        sel, rng = RULES[int(r.integers(0, len(RULES)))]
+        w, ring, pf = LS_SHAPES[int(r.integers(0, len(LS_SHAPES)))]
+        ls = int(r.choice([0, 1, 7]))
+    k = int(r.integers(5, min(30, min(x.n for x in insts) - 2) + 1)) if sel == ca.SEL_KNN else 0
+    m = 16 if fused else int(r.choice([16, 32]))
+    tau0 = int(r.choice([ca.TAU0_FIRST_COLONY, ca.TAU0_NN_TOUR]))
+    row16 = int(int16 and r.random() < 0.7)
+    return insts, dict(sel=sel, rng=rng, k=k, m=m, ls=ls, tau0=tau0,
+                       shape=(w, ring, pf, 0, row16, 1 if fused else 0))
+
+
+def _check_batch(insts, c, seed, iters, graph):
+    with ca.GpuACO(insts, n_ants=c["m"], k=c["k"], seed=seed, selection=c["sel"], rng=c["rng"],
+                   ls_flags=c["ls"], tau0_rule=c["tau0"], use_graph=graph) as g:
+        g.set_launch_shape(*c["shape"])
+        g.init_aco()
+        g.run(iters)
+        for b, inst in enumerate(insts):
+            P = oracle_problem(inst, k=c["k"])
+            col = oracle.Colony(P, c["m"], selection=c["sel"], rng=c["rng"], ls_flags=c["ls"],
+                                tau0_rule=c["tau0"], seed=seed + b)
+            col.init()
+            for _ in range(iters):
+                col.iteration()
+            gt, gs, gl = g.ants(b)
+            ot, osz, ol = col.ants()
+            np.testing.assert_array_equal(gs, osz, err_msg=f"instance {b}")
+            for a in range(c["m"] + 1):
+                np.testing.assert_array_equal(gt[a, : gs[a]], ot[a, : osz[a]], err_msg=f"instance {b} ant {a}")
+            np.testing.assert_array_equal(gl, ol)
+            gtau, gti = g.pheromone(b)
+            otau, oti = col.tau()
+            np.testing.assert_array_equal(gtau, otau)
+            np.testing.assert_array_equal(gti, oti)
+            assert [r[:2] for r in g.bsf_records(b)] == [r[:2] for r in col.bsf()]
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_batch_matches_oracle(seed):
+    """Instances of uneven size in one context, each equal to its own oracle
+    colony seeded with seed + b."""
+    insts, c = _batch(seed, fused=False)
+    log(f"random batch seed={seed} n={[x.n for x in insts]} {c}")
+    _check_batch(insts, c, 21 + seed, 3, graph=True)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_random_persistent_kernel_matches_oracle(seed):
+    """The one-launch iteration (persistent kernel) on random batches, each
+    instance equal to its own oracle colony."""
+    insts, c = _batch(seed, fused=True)
+    log(f"random persistent batch seed={seed} n={[x.n for x in insts]} {c}")
+    _check_batch(insts, c, 31 + seed, 3, graph=bool(seed % 2))
