@@ -380,6 +380,126 @@ __device__ __forceinline__ void two_opt_route(const DevProblem &p, LsLds &L, int
     if (gl == 0) moves_total += moves;
 }
 
+// The same search with two positions at a time (16 lanes per route): the
+// halves of the group scan the next two positions whose don't-look bits are
+// clear.  The second result stands only when the first position has no
+// move: a position without a move changes nothing but its own don't-look
+// bit, which the scans do not read, so the second scan saw what the
+// sequential search would see.  After a move at the first position the
+// search continues right after it and scans the second again.  Halves
+// exchange their results with one DPP swap (16-lane groups are DPP rows).
+__device__ __forceinline__ bool two_opt_skip(const LsLds &L, int pos, bool dlb0)
+{
+    const int v = L.tour[pos];
+    return v == 0 ? dlb0 : (L.dlb[v] != 0);
+}
+
+// kind (2 bits) and the move's four nodes (13 bits each; n <= 4096)
+__device__ __forceinline__ u64 two_opt_pack(int kind, int h1, int h2, int h3, int h4)
+{
+    return ((u64)kind << 52) | ((u64)h1 << 39) | ((u64)h2 << 26) | ((u64)h3 << 13) | (u64)h4;
+}
+
+__device__ __forceinline__ void two_opt_route_pair(const DevProblem &p, LsLds &L, int r, int rbeg, int rend,
+                                                   int gl16, long long &evals, int &moves_total)
+{
+    constexpr int G = 2 * kOptGroup;
+    const int half = gl16 / kOptGroup, hl = gl16 % kOptGroup, lane = (int)(threadIdx.x & 63);
+    bool dlb0 = r > 0;
+    const int num = rend - rbeg + 1;
+    int moves = 0;
+    bool imp = true;
+    while (imp) {
+        imp = false;
+        int l = 0;
+        for (;;) {
+            int la = l;
+            while (la < num && two_opt_skip(L, rbeg + la, dlb0)) ++la;
+            if (la >= num) break;
+            int lb = la + 1;
+            while (lb < num && two_opt_skip(L, rbeg + lb, dlb0)) ++lb;
+            const int na = L.tour[rbeg + la], nb = lb < num ? L.tour[rbeg + lb] : -1;
+            const int lm = half == 0 ? la : lb;
+            int kind = 0, h1 = 0, h2 = 0, h3 = 0, h4 = 0;
+            long long ev = 0;
+            if (lm < num) {
+                const int pos1 = rbeg + lm;
+                const int n1 = L.tour[pos1];
+                const int s1 = pos1 == rend ? 0 : L.tour[pos1 + 1];
+                const int p1 = pos1 == rbeg ? L.tour[rend] : L.tour[pos1 - 1];
+                const float rads = L.dtw[pos1], radp = pos1 == rbeg ? L.dtw[rend] : L.dtw[pos1 - 1];
+                kind = two_opt_scans(p, L, rbeg, rend, pos1, n1, s1, p1, rads, radp, hl, ev, h1, h2, h3, h4);
+            }
+            const u64 mine = two_opt_pack(kind, h1, h2, h3, h4);
+            const u64 other = dpp_xor8_u64(mine, lane);
+            const u64 first = half == 0 ? mine : other, second = half == 0 ? other : mine;
+            const bool k0 = (first >> 52) != 0, k1 = (second >> 52) != 0;
+            // the second scan's evaluations count only when it stands
+            if (half == 0 || !k0) evals += ev;
+            if (!k0) {
+                if (na == 0) dlb0 = true;
+                else if (gl16 == 0) L.dlb[na] = 1;
+                if (!k1 && nb >= 0) {
+                    if (nb == 0) dlb0 = true;
+                    else if (gl16 == 0) L.dlb[nb] = 1;
+                }
+            }
+            if (!k0 && !k1) {
+                group_sync();
+                l = lb + 1;
+                continue;
+            }
+            imp = true;
+            const u64 win = k0 ? first : second;
+            const int msk = (1 << 13) - 1;
+            h1 = (int)(win >> 39) & msk;
+            h2 = (int)(win >> 26) & msk;
+            h3 = (int)(win >> 13) & msk;
+            h4 = (int)win & msk;
+            const int hs[4] = {h1, h2, h3, h4};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (hs[q] == 0) dlb0 = false;
+                else if (gl16 == 0) L.dlb[hs[q]] = 0;
+            }
+            int ph1 = h1 == 0 ? rbeg : L.pos[h1];
+            int ph3 = h3 == 0 ? rbeg : L.pos[h3];
+            if (ph3 < ph1) {
+                int t = h1; h1 = h3; h3 = t;
+                t = h2; h2 = h4; h4 = t;
+                t = ph1; ph1 = ph3; ph3 = t;
+            }
+            const int tb = h2 == 0 ? rbeg : L.pos[h2];
+            group_sync();
+            if (gl16 == 0) {
+                L.dtw[ph1] = Dd(p, h1, h3);
+                L.dtw[ph3] = Dd(p, h2, h4);
+            }
+            const int ed = (ph3 - ph1 - 1) / 2;
+            for (int t = gl16; t < ed; t += G) {
+                const int i = ph1 + 1 + t, j = ph3 - 1 - t;
+                const float a = L.dtw[i];
+                L.dtw[i] = L.dtw[j];
+                L.dtw[j] = a;
+            }
+            const int tp = (ph3 - tb + 1) / 2;
+            for (int t = gl16; t < tp; t += G) {
+                const int i = tb + t, j = ph3 - t;
+                const int a = L.tour[i], b = L.tour[j];
+                L.tour[i] = b;
+                L.tour[j] = a;
+                L.pos[a] = (short)j;
+                L.pos[b] = (short)i;
+            }
+            group_sync();
+            l = (k0 ? la : lb) + 1;
+            if (++moves >= kMaxMovesPerRoute) break;
+        }
+        if (moves >= kMaxMovesPerRoute) break;
+    }
+    if (gl16 == 0) moves_total += moves;
+}
+
 // ------------------------------------------------- distance row cache ----
 
 // Rows d(tour[q], .) live in slot q mod R of an LDS ring of R rows (R = 8,
@@ -1027,9 +1147,15 @@ __device__ void local_search_ant(const DevProblem &p, const DevState &s, int a, 
     if (flags & kLs2Opt) {
         ls_edges<W>(p, L, 0, S - 2, tid);
         const int R = ls_routes<W>(p, L, S, tid, false);
-        const int grp = tid / kOptGroup, gl = tid % kOptGroup;
-        for (int r = grp; r < R; r += NT / kOptGroup)
-            two_opt_route(p, L, r, L.zpos[r], L.zpos[r + 1] - 1, gl, ev, moves);
+        if (R <= NT / (2 * kOptGroup)) {
+            // a group of 16 lanes per route: two positions at a time
+            const int grp = tid / (2 * kOptGroup), gl = tid % (2 * kOptGroup);
+            if (grp < R) two_opt_route_pair(p, L, grp, L.zpos[grp], L.zpos[grp + 1] - 1, gl, ev, moves);
+        } else {
+            const int grp = tid / kOptGroup, gl = tid % kOptGroup;
+            for (int r = grp; r < R; r += NT / kOptGroup)
+                two_opt_route(p, L, r, L.zpos[r], L.zpos[r + 1] - 1, gl, ev, moves);
+        }
         __syncthreads();
     }
     if (flags & kLsSwap) {
