@@ -304,39 +304,21 @@ __device__ __forceinline__ int two_opt_scans(const DevProblem &p, const LsLds &L
     return kind;
 }
 
-// Don't-look bit of the node at pos (the depot's is the route's own flag).
-__device__ __forceinline__ bool two_opt_skip(const LsLds &L, int pos, bool dlb0)
-{
-    const int v = L.tour[pos];
-    return v == 0 ? dlb0 : (L.dlb[v] != 0);
-}
-
 // kernel.cl:126-277 for route r at positions [rbeg, rend] (rbeg holds the
 // depot), run by the kOptGroup lanes gl = 0.. of one group.
 __device__ __forceinline__ void two_opt_route(const DevProblem &p, LsLds &L, int r, int rbeg,
                                               int rend, int gl, long long &evals, int &moves_total)
 {
-    const int lane = (int)(threadIdx.x & 63);
     bool dlb0 = r > 0;
     const int num = rend - rbeg + 1;
     int moves = 0;
     bool imp = true;
     while (imp) {
         imp = false;
-        for (int l = 0;; ++l) {
-            // the next position from l whose don't-look bit is clear, tested
-            // 8 at a time by the group's lanes
-            int la = num;
-            for (int base = l; base < num && la >= num; base += kOptGroup) {
-                const int x = base + gl;
-                const bool clear = x < num && !two_opt_skip(L, rbeg + x, dlb0);
-                const unsigned bits = (unsigned)(__ballot(clear) >> (lane & ~(kOptGroup - 1))) & 0xFFu;
-                if (bits) la = base + __builtin_ctz(bits);
-            }
-            if (la >= num) break;
-            l = la;
+        for (int l = 0; l < num; ++l) {
             const int pos1 = rbeg + l;
             const int n1 = L.tour[pos1];
+            if (n1 == 0 ? dlb0 : (L.dlb[n1] != 0)) continue;
             int h1 = 0, h2 = 0, h3 = 0, h4 = 0;
             const int s1 = pos1 == rend ? 0 : L.tour[pos1 + 1];
             const int p1 = pos1 == rbeg ? L.tour[rend] : L.tour[pos1 - 1];
@@ -406,6 +388,12 @@ __device__ __forceinline__ void two_opt_route(const DevProblem &p, LsLds &L, int
 // sequential search would see.  After a move at the first position the
 // search continues right after it and scans the second again.  Halves
 // exchange their results with one DPP swap (16-lane groups are DPP rows).
+__device__ __forceinline__ bool two_opt_skip(const LsLds &L, int pos, bool dlb0)
+{
+    const int v = L.tour[pos];
+    return v == 0 ? dlb0 : (L.dlb[v] != 0);
+}
+
 // kind (2 bits) and the move's four nodes (13 bits each; n <= 4096)
 __device__ __forceinline__ u64 two_opt_pack(int kind, int h1, int h2, int h3, int h4)
 {
