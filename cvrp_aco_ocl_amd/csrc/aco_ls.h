@@ -413,20 +413,11 @@ __device__ __forceinline__ void two_opt_route_pair(const DevProblem &p, LsLds &L
         imp = false;
         int l = 0;
         for (;;) {
-            // the next two positions from l whose don't-look bits are clear,
-            // tested 16 at a time by the group's lanes
-            int la = num, lb = num;
-            for (int base = l; base < num && lb >= num; base += G) {
-                const int x = base + gl16;
-                const bool clear = x < num && !two_opt_skip(L, rbeg + x, dlb0);
-                unsigned bits = (unsigned)(__ballot(clear) >> (lane & ~(G - 1))) & 0xFFFFu;
-                if (la >= num && bits) {
-                    la = base + __builtin_ctz(bits);
-                    bits &= bits - 1u;
-                }
-                if (bits) lb = base + __builtin_ctz(bits);
-            }
+            int la = l;
+            while (la < num && two_opt_skip(L, rbeg + la, dlb0)) ++la;
             if (la >= num) break;
+            int lb = la + 1;
+            while (lb < num && two_opt_skip(L, rbeg + lb, dlb0)) ++lb;
             const int na = L.tour[rbeg + la], nb = lb < num ? L.tour[rbeg + lb] : -1;
             const int lm = half == 0 ? la : lb;
             int kind = 0, h1 = 0, h2 = 0, h3 = 0, h4 = 0;
