@@ -75,6 +75,7 @@ struct aco_ctx {
     DevBatch bt{};
     std::vector<void *> allocs;
     hipStream_t stream = nullptr;
+    LsStreams lss;   // batches: local-search size classes side by side
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     bool inited = false;
@@ -265,7 +266,7 @@ static int run_local_search(aco_ctx *c)
 {
     if (!c->prm.ls_flags) return ACO_OK;
     HIP_TRY(timed(c, "local_search",
-                  [&] { return launch_local_search(c->bt, c->prm.ls_flags, c->ls, c->stream); }));
+                  [&] { return launch_local_search(c->bt, c->prm.ls_flags, c->ls, c->stream, &c->lss); }));
     return ACO_OK;
 }
 
@@ -650,6 +651,15 @@ int aco_create_batch(const aco_problem *pbs, int32_t count, const aco_params *pr
         ++bt.ls_classes;
     }
     bt.ls_first[bt.ls_classes] = (int)ls_order.size();
+    // streams for the size classes beyond the first (default priority)
+    if (bt.ls_classes > 1) {
+        CHK(hipEventCreateWithFlags(&c->lss.fork, hipEventDisableTiming));
+        for (int i = 0; i < bt.ls_classes - 1; ++i) {
+            CHK(hipStreamCreateWithFlags(&c->lss.aux[i], hipStreamNonBlocking));
+            c->lss.count = i + 1;
+            CHK(hipEventCreateWithFlags(&c->lss.join[i], hipEventDisableTiming));
+        }
+    }
     void *q = nullptr;
     const size_t tb = sizeof(DevProblem) * count + sizeof(DevState) * count + 4 * ant_inst.size() +
                       4 * ant_base.size() + 4 * ls_order.size() + 1536;
@@ -692,6 +702,11 @@ int aco_destroy(aco_ctx *c)
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
     for (void *p : c->allocs) (void)hipFree(p);
+    for (int i = 0; i < c->lss.count; ++i) {
+        (void)hipStreamDestroy(c->lss.aux[i]);
+        (void)hipEventDestroy(c->lss.join[i]);
+    }
+    if (c->lss.fork) (void)hipEventDestroy(c->lss.fork);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return ACO_OK;
@@ -1136,7 +1151,7 @@ int aco_autotune(aco_ctx *c, int32_t reps, aco_tune_result *results, int32_t cap
         LsShape sh = kLs[q];
         sh.construct_apb = 0;
         if (!shape_fits(c, sh)) continue;
-        time_it(built, [&] { return launch_local_search(c->bt, c->prm.ls_flags, kLs[q], c->stream); }, tl[q]);
+        time_it(built, [&] { return launch_local_search(c->bt, c->prm.ls_flags, kLs[q], c->stream, &c->lss); }, tl[q]);
         if (err == hipSuccess && (best_l < 0 || tl[q] < tl[best_l])) best_l = q;
     }
     // stage 2: whole iterations, per-phase launches against the fused launch
@@ -1168,7 +1183,7 @@ int aco_autotune(aco_ctx *c, int32_t reps, aco_tune_result *results, int32_t cap
         time_it(orig, [&] {
             if (sh.fused) return launch_iteration_fused(c->bt, c->prm.rng, c->prm.ls_flags, sh, c->tick_to_s, c->stream);
             hipError_t e = launch_construct(c->bt, c->prm.selection, c->prm.rng, sh.construct_apb, c->prm.k, c->stream);
-            if (e == hipSuccess) e = launch_local_search(c->bt, c->prm.ls_flags, sh, c->stream);
+            if (e == hipSuccess) e = launch_local_search(c->bt, c->prm.ls_flags, sh, c->stream, &c->lss);
             if (e == hipSuccess) e = launch_statistics(c->bt, c->tick_to_s, c->stream);
             if (e == hipSuccess) e = launch_pheromone_update(c->bt, c->stream);
             if (e == hipSuccess) e = launch_gather(c->bt, c->stream);

@@ -203,7 +203,18 @@ struct LsShape {
 hipError_t launch_construct(const DevBatch &b, int selection, int rng, int apb, int k, hipStream_t st);
 size_t construct_lds_staged_bytes(int n, int k, int apb);
 bool construct_staged_ok(const DevBatch &b, int selection, int k, int ants_per_instance, int apb);
-hipError_t launch_local_search(const DevBatch &b, int flags, const LsShape &sh, hipStream_t st);
+// Extra streams on which the size classes of a batch run beside the first
+// (forked from and joined back into the launch stream; under capture they
+// become parallel graph branches).  count = 0: every class on the launch
+// stream.
+struct LsStreams {
+    hipStream_t aux[kLsClasses - 1] = {};
+    hipEvent_t fork = nullptr;
+    hipEvent_t join[kLsClasses - 1] = {};
+    int count = 0;
+};
+hipError_t launch_local_search(const DevBatch &b, int flags, const LsShape &sh, hipStream_t st,
+                               const LsStreams *par = nullptr);
 // compute_nn_lists for `count` instances on the device (host arrays in and
 // out, one thread per row, the reference's sort steps); ACO_OK or an error.
 int gpu_nn_lists(hipStream_t st, int count, const int *n, const float *const *dist, const int *width,

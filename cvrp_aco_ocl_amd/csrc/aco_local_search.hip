@@ -31,16 +31,29 @@ bool ls_shape_valid(const LsShape &sh)
     return true;
 }
 
-hipError_t launch_local_search(const DevBatch &b, int flags, const LsShape &sh, hipStream_t st)
+hipError_t launch_local_search(const DevBatch &b, int flags, const LsShape &sh, hipStream_t st,
+                               const LsStreams *par)
 {
     if (!flags) return hipSuccess;
     if (!ls_shape_valid(sh)) return hipErrorInvalidValue;
     // one launch per size class, largest instances first, each with LDS for
-    // its own largest n
+    // its own largest n.  The classes search disjoint ants, so with extra
+    // streams they run side by side (one launch's tail overlaps the next
+    // class) and join the launch stream afterwards.
+    const bool fork = par && par->count >= b.ls_classes - 1 && b.ls_classes > 1;
+    if (fork) {
+        const hipError_t e = hipEventRecord(par->fork, st);
+        if (e != hipSuccess) return e;
+    }
     for (int cl = 0; cl < b.ls_classes; ++cl) {
         const int first = b.ls_first[cl], count = b.ls_first[cl + 1] - first;
         const size_t lds = local_search_lds_bytes(b.ls_maxn[cl], sh.ring, sh.row16);
-#define KLS(W, RT) hipLaunchKernelGGL((k_local_search<W, RT>), dim3(count), dim3(W * kLanes), lds, st, b, first, \
+        const hipStream_t s = fork && cl > 0 ? par->aux[cl - 1] : st;
+        if (s != st) {
+            const hipError_t e = hipStreamWaitEvent(s, par->fork, 0);
+            if (e != hipSuccess) return e;
+        }
+#define KLS(W, RT) hipLaunchKernelGGL((k_local_search<W, RT>), dim3(count), dim3(W * kLanes), lds, s, b, first, \
                                       flags, sh.ring, sh.prefetch)
         switch (sh.waves) {
         case 16: if (sh.row16) KLS(16, unsigned short); else KLS(16, float); break;
@@ -49,8 +62,13 @@ hipError_t launch_local_search(const DevBatch &b, int flags, const LsShape &sh, 
         default: if (sh.row16) KLS(4, unsigned short); else KLS(4, float); break;
         }
 #undef KLS
-        const hipError_t e = hipGetLastError();
+        hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
+        if (s != st) {
+            e = hipEventRecord(par->join[cl - 1], s);
+            if (e == hipSuccess) e = hipStreamWaitEvent(st, par->join[cl - 1], 0);
+            if (e != hipSuccess) return e;
+        }
     }
     return hipSuccess;
 }
