@@ -143,7 +143,8 @@ def test_random_batch_matches_oracle(seed):
     colony seeded with seed + b."""
     insts, c = _batch(seed, fused=False)
     log(f"random batch seed={seed} n={[x.n for x in insts]} {c}")
-    _check_batch(insts, c, 21 + seed, 3, graph=True)
+    # eager launches and graph replay (size classes on forked streams in both)
+    _check_batch(insts, c, 21 + seed, 3, graph=bool(seed % 2))
 
 
 @pytest.mark.parametrize("seed", range(4))
