@@ -128,6 +128,8 @@ _SIGNATURES = {
                                   C.POINTER(C.c_int32)]),
     "aco_check_solution": (C.c_int, [C.c_int32, _P, _P, C.c_int32, C.c_float, C.c_float, _P,
                                      C.c_int32]),
+    "aco_nn_tour": (C.c_int, [C.c_int32, _P, _P, C.c_int32, C.c_float, C.c_float, _P, C.c_int32,
+                              C.POINTER(C.c_int32), C.POINTER(C.c_float)]),
     # learned prior (cvrp_prior.h)
     "aco_prior_default_train_params": (None, [C.POINTER(AcoPriorTrainParams)]),
     "aco_prior_create": (C.c_int, [C.c_uint64, C.POINTER(C.c_void_p)]),

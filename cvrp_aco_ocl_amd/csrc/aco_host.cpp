@@ -150,13 +150,20 @@ int check_solution(int n, const float *dist, const int *dem, int Q, float D, flo
     return 1;
 }
 
-// Greedy nearest feasible customer, ties to the lowest index (tau0 rule).
-float nn_tour_length(int n, const float *dist, const int *dem, int Q, float D, float svc)
+// Greedy nearest feasible customer, ties to the lowest index (tau0 rule):
+// from the current node take the unvisited customer with the smallest
+// d(cur, i) among those that keep load <= Q and rdist + (d(cur,i) + svc) +
+// d(i,0) <= D; when none fits, return to the depot.  The length adds the
+// edges in step order.  When tour != nullptr the node sequence (depot
+// first and last) is written there, 2n entries at most.  Returns the tour
+// size, or -1 when some customer can never be served.
+int nn_tour(int n, const float *dist, const int *dem, int Q, float D, float svc, int *tour, float *length)
 {
     std::vector<char> vis(n, 0);
     vis[0] = 1;
-    int cur = 0, load = 0, left = n - 1;
+    int cur = 0, load = 0, left = n - 1, size = 1;
     float rdist = 0.0f, len = 0.0f;
+    if (tour) tour[0] = 0;
     while (left > 0) {
         int best = -1;
         float bd = INFINITY;
@@ -169,16 +176,27 @@ float nn_tour_length(int n, const float *dist, const int *dem, int Q, float D, f
         }
         int w;
         if (best < 0) {
-            if (cur == 0) return -1.0f;
+            if (cur == 0) return -1;
             w = 0; load = 0; rdist = 0.0f;
         } else {
             w = best; load += dem[w]; rdist = rdist + (dist[(size_t)cur * n + w] + svc); left--;
         }
         vis[w] = 1;
         len = len + dist[(size_t)cur * n + w];
+        if (tour) tour[size] = w;
+        ++size;
         cur = w;
     }
-    return len + dist[(size_t)cur * n];
+    len = len + dist[(size_t)cur * n];
+    if (tour) tour[size] = 0;
+    *length = len;
+    return size + 1;
+}
+
+float nn_tour_length(int n, const float *dist, const int *dem, int Q, float D, float svc)
+{
+    float len = -1.0f;
+    return nn_tour(n, dist, dem, Q, D, svc, nullptr, &len) < 0 ? -1.0f : len;
 }
 
 // Tournament order of the reference's argmax loop (kernel.cl:340-366):
@@ -321,6 +339,19 @@ int aco_read_vrplib(const char *path, int32_t capacity, int32_t *n, double *x, d
     if (service_time) *service_time = inst.service;
     if (optimum) *optimum = inst.optimum;
     if (edge_weight_type) *edge_weight_type = inst.edge_type;
+    return ACO_OK;
+}
+
+int aco_nn_tour(int32_t n, const float *dist, const int32_t *demand, int32_t capacity, float max_dist,
+                float service_time, int32_t *tour, int32_t tour_capacity, int32_t *size, float *length)
+{
+    if (n < 2 || !dist || !demand || !length || capacity <= 0 || (tour && tour_capacity < 2 * n))
+        return set_error(ACO_ERR_ARG, "aco_nn_tour: bad argument (tour needs 2n entries)");
+    float len = 0.0f;
+    const int sz = nn_tour(n, dist, demand, capacity, max_dist, service_time, tour, &len);
+    if (sz < 0) return set_error(ACO_ERR_INFEASIBLE, "aco_nn_tour: a customer cannot be served");
+    if (size) *size = sz;
+    *length = len;
     return ACO_OK;
 }
 

@@ -30,6 +30,7 @@ float distance_of(int mode, double xi, double yi, double xj, double yj);
 void nn_lists(int n, const float *dist, int nn, int tie, int *out);
 int check_solution(int n, const float *dist, const int *dem, int Q, float D, float svc,
                    const int *tour, int size);
+int nn_tour(int n, const float *dist, const int *dem, int Q, float D, float svc, int *tour, float *length);
 float nn_tour_length(int n, const float *dist, const int *dem, int Q, float D, float svc);
 void tree_rank(int N, int *rank);
 int read_vrplib(const char *path, VrpInstance &inst, std::string &err);

@@ -110,6 +110,27 @@ class Instance:
             check(rc, "aco_check_solution")
         return bool(rc)
 
+    def subset(self, count: int) -> "Instance":
+        """The depot and the first count-1 customers (same capacity and
+        limits)."""
+        return Instance(n=count, x=self.x[:count].copy(), y=self.y[:count].copy(),
+                        demand=self.demand[:count].copy(), capacity=self.capacity,
+                        dist=np.ascontiguousarray(self.dist[:count, :count]),
+                        max_dist=self.max_dist, service_time=self.service_time,
+                        name=f"{self.name}[:{count}]", dist_mode=self.dist_mode)
+
+    def nn_tour(self):
+        """The tau0 = 1/(n * L_nn) rule's greedy tour (aco_nn_tour): returns
+        (tour, L_nn)."""
+        d = np.ascontiguousarray(self.dist, np.float32)
+        t = np.empty(2 * self.n, np.int32)
+        size = C.c_int32()
+        length = C.c_float()
+        check(lib.aco_nn_tour(self.n, _ptr(d), _ptr(self.demand), self.capacity, self.max_dist,
+                              self.service_time, _ptr(t), len(t), C.byref(size), C.byref(length)),
+              "aco_nn_tour")
+        return t[: size.value].copy(), float(length.value)
+
     def tour_length(self, tour) -> float:
         t = np.asarray(tour, np.int64)
         length = np.float32(0.0)

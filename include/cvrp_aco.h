@@ -242,6 +242,18 @@ int aco_compute_nn_lists_batch(int32_t count, const int32_t *n, const float *con
 int aco_read_vrplib(const char *path, int32_t capacity, int32_t *n, double *x, double *y,
                     int32_t *demand, int32_t *vehicle_capacity, float *max_dist,
                     float *service_time, float *optimum, int32_t *edge_weight_type);
+/* Initial trail of the ACO_TAU0_NN_TOUR rule, tau0 = 1/(n * L_nn).  No
+ * reference counterpart: the reference derives tau0 from a first colony
+ * (g_aco.cpp:79-90); this is the nearest-neighbour rule of the ACO
+ * literature, restated for capacity and route length: from the current
+ * node go to the nearest unvisited customer i (ties to the lowest index)
+ * with load + demand[i] <= capacity and rdist + (d(cur,i) + service_time) +
+ * d(i,0) <= max_dist; when none fits, back to the depot.  *length adds the
+ * edges in step order.  tour (optional, tour_capacity >= 2n) receives the
+ * node sequence, depot first and last, *size its length.
+ * ACO_ERR_INFEASIBLE when some customer cannot be served. */
+int aco_nn_tour(int32_t n, const float *dist, const int32_t *demand, int32_t capacity, float max_dist,
+                float service_time, int32_t *tour, int32_t tour_capacity, int32_t *size, float *length);
 /* check_solution (problem.cpp:226-319); returns 1 valid, 0 invalid */
 int aco_check_solution(int32_t n, const float *dist, const int32_t *demand, int32_t capacity,
                        float max_dist, float service_time, const int32_t *tour, int32_t size);
