@@ -84,6 +84,8 @@ struct aco_ctx {
     std::map<std::string, ProfSlot> prof;
     LsShape ls{4, 16, 1, 0, 0};
     size_t lds_limit = 0;
+    // static LDS of the kernel families, added to each launch's dynamic LDS
+    size_t lds_static_cons = 0, lds_static_ls = 0, lds_static_fused = 0;
     int max_n = 0;
     bool int16_rows = true;   // every distance an integer in [0, 65535]
 };
@@ -463,10 +465,15 @@ int aco_create_batch(const aco_problem *pbs, int32_t count, const aco_params *pr
     // fits, else an 8-row ring
     c->lds_limit = prop.sharedMemPerBlock;
     c->max_n = max_n;
+    CHK(construct_static_lds(c->lds_static_cons));
+    CHK(local_search_static_lds(c->lds_static_ls));
+    CHK(fused_static_lds(c->lds_static_fused));
     c->ls = LsShape{4, 16, 1, 0, 0, 0};
-    if (local_search_lds_bytes(max_n, 16, 0) > prop.sharedMemPerBlock) c->ls = LsShape{4, 8, 0, 0, 0, 0};
-    const size_t lds_need = std::max(construct_lds_bytes(max_n, prm.selection),
-                                     prm.ls_flags ? local_search_lds_bytes(max_n, c->ls.ring, 0) : 0);
+    if (local_search_lds_bytes(max_n, 16, 0) + c->lds_static_ls > prop.sharedMemPerBlock)
+        c->ls = LsShape{4, 8, 0, 0, 0, 0};
+    const size_t lds_need =
+        std::max(construct_lds_bytes(max_n, prm.selection) + c->lds_static_cons,
+                 prm.ls_flags ? local_search_lds_bytes(max_n, c->ls.ring, 0) + c->lds_static_ls : 0);
     if (lds_need > prop.sharedMemPerBlock)
         return fail(set_error(ACO_ERR_ARG, "instance too large for LDS: needs " + std::to_string(lds_need) +
                                                " bytes, device has " + std::to_string(prop.sharedMemPerBlock)));
@@ -1019,14 +1026,15 @@ static void drop_graph(aco_ctx *c)
 
 static bool shape_fits(const aco_ctx *c, const LsShape &sh)
 {
-    if (!ls_shape_valid(sh) || local_search_lds_bytes(c->max_n, sh.ring, sh.row16) > c->lds_limit) return false;
+    if (!ls_shape_valid(sh) || local_search_lds_bytes(c->max_n, sh.ring, sh.row16) + c->lds_static_ls > c->lds_limit)
+        return false;
     if (sh.row16 && !c->int16_rows) return false;
     if (sh.fused && (!fused_iteration_ok(c->bt, c->prm.selection, c->prm.ls_flags, c->m, sh) ||
-                     fused_lds_bytes(c->max_n, sh) > c->lds_limit))
+                     fused_lds_bytes(c->max_n, sh) + c->lds_static_fused > c->lds_limit))
         return false;
     if (!construct_staged_ok(c->bt, c->prm.selection, c->prm.k, c->m, sh.construct_apb)) return false;
     return sh.construct_apb == 0 ||
-           construct_lds_staged_bytes(c->max_n, c->prm.k, sh.construct_apb) <= c->lds_limit;
+           construct_lds_staged_bytes(c->max_n, c->prm.k, sh.construct_apb) + c->lds_static_cons <= c->lds_limit;
 }
 
 static aco_launch_shape to_public(const LsShape &sh)

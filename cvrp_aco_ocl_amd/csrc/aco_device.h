@@ -233,5 +233,11 @@ size_t fused_lds_bytes(int n, const LsShape &sh);
 hipError_t launch_iteration_fused(const DevBatch &b, int rng, int ls_flags, const LsShape &sh, float tick_to_s,
                                   hipStream_t st);
 bool ls_shape_valid(const LsShape &sh);
+// Largest static LDS (__shared__ arrays of the kernel itself) over every
+// instantiation of the construction, local-search and fused kernels; the
+// dynamic LDS of a launch must fit beside it.
+hipError_t construct_static_lds(size_t &bytes);
+hipError_t local_search_static_lds(size_t &bytes);
+hipError_t fused_static_lds(size_t &bytes);
 
 }  // namespace cvrp

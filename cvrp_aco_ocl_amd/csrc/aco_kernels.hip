@@ -511,6 +511,32 @@ size_t construct_lds_staged_bytes(int n, int k, int apb)
     return nk * 8 + (size_t)n * 8 + (size_t)apb * ((n + 15) & ~15) + 16;
 }
 
+static hipError_t max_static_lds(const void *const *fns, int count, size_t &bytes)
+{
+    bytes = 0;
+    for (int i = 0; i < count; ++i) {
+        hipFuncAttributes a;
+        const hipError_t e = hipFuncGetAttributes(&a, fns[i]);
+        if (e != hipSuccess) return e;
+        bytes = a.sharedSizeBytes > bytes ? a.sharedSizeBytes : bytes;
+    }
+    return hipSuccess;
+}
+
+hipError_t construct_static_lds(size_t &bytes)
+{
+    const void *fns[] = {(const void *)k_construct_iroulette<false>, (const void *)k_construct_iroulette<true>,
+                         (const void *)k_construct_knn<false, kRngPhilox>, (const void *)k_construct_knn<true, kRngPhilox>,
+                         (const void *)k_construct_knn<false, kRngParkMiller>,
+                         (const void *)k_construct_knn<true, kRngParkMiller>,
+                         (const void *)k_construct_knn_lds<kRngPhilox, 1>, (const void *)k_construct_knn_lds<kRngPhilox, 2>,
+                         (const void *)k_construct_knn_lds<kRngPhilox, 4>,
+                         (const void *)k_construct_knn_lds<kRngParkMiller, 1>,
+                         (const void *)k_construct_knn_lds<kRngParkMiller, 2>,
+                         (const void *)k_construct_knn_lds<kRngParkMiller, 4>};
+    return max_static_lds(fns, (int)(sizeof(fns) / sizeof(fns[0])), bytes);
+}
+
 bool construct_staged_ok(const DevBatch &b, int selection, int k, int ants_per_instance, int apb)
 {
     if (apb == 0) return true;
@@ -936,6 +962,19 @@ size_t fused_lds_bytes(int n, const LsShape &sh)
     const size_t cons = (size_t)n * 8 + (size_t)sh.waves * ((n + 15) & ~15) + 16;
     const size_t ls = local_search_lds_bytes(n, sh.ring, sh.row16);
     return cons > ls ? cons : ls;
+}
+
+hipError_t fused_static_lds(size_t &bytes)
+{
+    const void *fns[] = {(const void *)k_iteration_fused<4, float, kRngPhilox>,
+                         (const void *)k_iteration_fused<4, unsigned short, kRngPhilox>,
+                         (const void *)k_iteration_fused<8, float, kRngPhilox>,
+                         (const void *)k_iteration_fused<8, unsigned short, kRngPhilox>,
+                         (const void *)k_iteration_fused<4, float, kRngParkMiller>,
+                         (const void *)k_iteration_fused<4, unsigned short, kRngParkMiller>,
+                         (const void *)k_iteration_fused<8, float, kRngParkMiller>,
+                         (const void *)k_iteration_fused<8, unsigned short, kRngParkMiller>};
+    return max_static_lds(fns, (int)(sizeof(fns) / sizeof(fns[0])), bytes);
 }
 
 hipError_t launch_iteration_fused(const DevBatch &b, int rng, int ls_flags, const LsShape &sh, float tick_to_s,

@@ -22,6 +22,24 @@ __global__ __launch_bounds__(W * 64) void k_local_search(DevBatch bt, int first,
     local_search_ant<W, RT>(p, s, g - bt.ant_base[inst], flags, ring, prefetch, smem);
 }
 
+// Largest static LDS (the kernel's own __shared__ arrays, beside the
+// dynamic allocation) of the local-search kernels.
+hipError_t local_search_static_lds(size_t &bytes)
+{
+    const void *fns[] = {(const void *)k_local_search<4, float>,  (const void *)k_local_search<4, unsigned short>,
+                         (const void *)k_local_search<8, float>,  (const void *)k_local_search<8, unsigned short>,
+                         (const void *)k_local_search<12, float>, (const void *)k_local_search<12, unsigned short>,
+                         (const void *)k_local_search<16, float>, (const void *)k_local_search<16, unsigned short>};
+    bytes = 0;
+    for (const void *f : fns) {
+        hipFuncAttributes a;
+        const hipError_t e = hipFuncGetAttributes(&a, f);
+        if (e != hipSuccess) return e;
+        bytes = a.sharedSizeBytes > bytes ? a.sharedSizeBytes : bytes;
+    }
+    return hipSuccess;
+}
+
 bool ls_shape_valid(const LsShape &sh)
 {
     if (sh.waves != 4 && sh.waves != 8 && sh.waves != 12 && sh.waves != 16) return false;
