@@ -122,6 +122,7 @@ _SIGNATURES = {
     "aco_compute_distances": (C.c_int, [C.c_int32, _P, _P, C.c_int32, _P]),
     "aco_compute_nn_lists": (C.c_int, [C.c_int32, _P, C.c_int32, C.c_int32, _P]),
     "aco_compute_nn_lists_batch": (C.c_int, [C.c_int32, _P, _P, _P, C.c_int32, _P]),
+    "aco_compute_nn_lists_batch_noalloc": (C.c_int, [C.c_int32, _P, _P, _P, C.c_int32, _P, _P]),
     "aco_read_vrplib": (C.c_int, [C.c_char_p, C.c_int32, C.POINTER(C.c_int32), _P, _P, _P,
                                   C.POINTER(C.c_int32), C.POINTER(C.c_float),
                                   C.POINTER(C.c_float), C.POINTER(C.c_float),

@@ -11,6 +11,8 @@
 // entries.  Rows with equal distances therefore come out in the
 // reference's order (tie 0), or in index order when the index breaks ties
 // (tie 1), exactly as the host builder aco_compute_nn_lists.
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "aco_device.h"
@@ -106,6 +108,80 @@ __global__ __launch_bounds__(256) void k_nn_lists(const NnJob *jobs, int njobs, 
     for (int h = 0; h < J.width; ++h) J.out[(size_t)node * J.width + h] = ix[h];
 }
 
+// ---- the same lists without device allocation ----
+//
+// aco_compute_nn_lists_batch_noalloc sorts in `scratch`, a file-scope device
+// array of kNnScratchBytes that every call on a GPU shares (each GPU holds
+// its own copy until the process ends).  A launch takes up to kNnRowJobs
+// row ranges, passed by value, so nothing is copied to the device first;
+// each row of an n-node instance uses 8n bytes of scratch (sort keys and
+// node ids).  Every scratch word a row reads is written by that row first,
+// so nothing has to start at zero.  Calls that overlap on one GPU race on
+// the array.
+constexpr size_t kNnScratchBytes = size_t(64) << 20;
+constexpr int kNnRowJobs = 32;
+
+static __device__ __align__(256) unsigned char scratch[kNnScratchBytes];
+
+struct NnRowJob {
+    const float *dist;   // n x n device array
+    int *out;            // n x width device array
+    float *v;            // rows x n scratch keys
+    int *ix;             // rows x n scratch ids
+    int n, width, row0, first;   // first: first thread of this job in the launch
+};
+
+struct NnRowJobs {
+    NnRowJob j[kNnRowJobs];
+    int count, threads;
+};
+
+__global__ __launch_bounds__(256) void k_nn_rows(NnRowJobs J, int tie)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= J.threads) return;
+    int q = 0;
+    while (q + 1 < J.count && J.j[q + 1].first <= t) ++q;
+    const NnRowJob R = J.j[q];
+    const int r = t - R.first, node = R.row0 + r, n = R.n;
+    const float *d = R.dist + (size_t)node * n;
+    float *v = R.v + (size_t)r * n;
+    int *ix = R.ix + (size_t)r * n;
+    for (int i = 0; i < n; ++i) {
+        v[i] = d[i];
+        ix[i] = i;
+    }
+    v[node] = INFINITY;
+    v[0] = INFINITY;
+    nn_sort_row(v, ix, n, tie);
+    for (int h = 0; h < R.width; ++h) R.out[(size_t)node * R.width + h] = ix[h];
+}
+
+// Per-device address of `scratch` (the array's name is a host-side stand-in,
+// not a device address), looked up once per device with that device current.
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[64];
+
+static int scratch_address(void **ptr)
+{
+    int dev = 0;
+    const hipError_t ed = hipGetDevice(&dev);
+    if (ed != hipSuccess || dev < 0 || dev >= 64)
+        return set_error(ACO_ERR_NODEV, std::string("aco_compute_nn_lists_batch_noalloc: no current device: ") +
+                                            hipGetErrorString(ed));
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return set_error(ACO_ERR_HIP, std::string("aco_compute_nn_lists_batch_noalloc: hipGetSymbolAddress: ") +
+                                              hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *ptr = g_scratch_ptr[dev];
+    return ACO_OK;
+}
+
 // Instances are processed in chunks of at most this many matrix elements.
 constexpr size_t kNnChunkElems = size_t(1) << 26;
 
@@ -192,4 +268,68 @@ extern "C" int aco_compute_nn_lists_batch(int32_t count, const int32_t *n, const
     const int rc = gpu_nn_lists(st, count, n, dist, nn, tie, out);
     (void)hipStreamDestroy(st);
     return rc;
+}
+
+extern "C" int aco_compute_nn_lists_batch_noalloc(int32_t count, const int32_t *n, const float *const *dist,
+                                                  const int32_t *nn, int32_t tie, int32_t *const *out, void *stream)
+{
+    using namespace cvrp;
+    if (count < 0 || (count > 0 && (!n || !dist || !nn || !out)) || (tie != 0 && tie != 1))
+        return set_error(ACO_ERR_ARG, "aco_compute_nn_lists_batch_noalloc: bad argument");
+    for (int b = 0; b < count; ++b) {
+        if (n[b] < 3 || n[b] > kMaxNodes || nn[b] < 1 || nn[b] > n[b] - 2 || !dist[b] || !out[b])
+            return set_error(ACO_ERR_ARG, "aco_compute_nn_lists_batch_noalloc: instance " + std::to_string(b) +
+                                              " needs 3 <= n <= 4096 and 1 <= nn <= n-2");
+        if ((size_t)n[b] * 8 > kNnScratchBytes)
+            return set_error(ACO_ERR_ARG, "aco_compute_nn_lists_batch_noalloc: a row of instance " +
+                                              std::to_string(b) + " exceeds the " +
+                                              std::to_string(kNnScratchBytes) + "-byte scratch");
+    }
+    if (count == 0) return ACO_OK;
+    void *base = nullptr;
+    int rc = scratch_address(&base);
+    if (rc) return rc;
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    unsigned char *const sbase = static_cast<unsigned char *>(base);
+    // fill launches: row ranges of consecutive instances while scratch and
+    // the job table last
+    NnRowJobs J;
+    size_t used = 0;
+    J.count = 0;
+    J.threads = 0;
+    auto flush = [&]() -> int {
+        if (J.count == 0) return ACO_OK;
+        hipLaunchKernelGGL(k_nn_rows, dim3((J.threads + 255) / 256), dim3(256), 0, st, J, (int)tie);
+        const hipError_t e = hipGetLastError();
+        J.count = 0;
+        J.threads = 0;
+        used = 0;
+        if (e != hipSuccess)
+            return set_error(ACO_ERR_HIP, std::string("aco_compute_nn_lists_batch_noalloc: ") + hipGetErrorString(e));
+        return ACO_OK;
+    };
+    for (int b = 0; b < count; ++b) {
+        const size_t row_bytes = (size_t)n[b] * 8;
+        int row = 0;
+        while (row < n[b]) {
+            if (J.count == kNnRowJobs || used + row_bytes > kNnScratchBytes) {
+                if ((rc = flush())) return rc;
+            }
+            const int rows = (int)std::min<size_t>((size_t)(n[b] - row), (kNnScratchBytes - used) / row_bytes);
+            NnRowJob &R = J.j[J.count++];
+            R.dist = dist[b];
+            R.out = out[b];
+            R.v = reinterpret_cast<float *>(sbase + used);
+            R.ix = reinterpret_cast<int *>(sbase + used + (size_t)rows * n[b] * 4);
+            R.n = n[b];
+            R.width = nn[b];
+            R.row0 = row;
+            R.first = J.threads;
+            J.threads += rows;
+            used += (size_t)rows * row_bytes;
+            used = (used + 255) & ~size_t(255);
+            row += rows;
+        }
+    }
+    return flush();
 }

@@ -235,6 +235,21 @@ int aco_compute_nn_lists(int32_t n, const float *dist, int32_t nn, int32_t tie, 
  * it for instances given without nn_list. */
 int aco_compute_nn_lists_batch(int32_t count, const int32_t *n, const float *const *dist,
                                const int32_t *nn, int32_t tie, int32_t *const *out);
+/* The same lists without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * dist[b] (n[b] x n[b]) and out[b] (n[b] x nn[b]) are DEVICE arrays;
+ * `stream` is a hipStream_t (NULL: the null stream) and the call only
+ * enqueues work on it (nothing is waited for or copied).  Sorting uses a
+ * library-owned 64 MiB device array per GPU, held until the process ends;
+ * a row of an n-node instance takes 8n bytes of it, so one launch sorts
+ * the rows of several instances while they fit (n <= 4096 always fits).
+ * Calls that overlap on one GPU share that array and race: serialise them
+ * (one stream, or the caller's own ordering).  Results equal
+ * aco_compute_nn_lists_batch. */
+int aco_compute_nn_lists_batch_noalloc(int32_t count, const int32_t *n, const float *const *dist,
+                                       const int32_t *nn, int32_t tie, int32_t *const *out,
+                                       void *stream);
 /* read_instance_file (io.cpp:334-517).  First call with capacity 0 to get
  * *n; then call with capacity >= *n to fill the arrays.  max_dist is
  * INFINITY and service_time 0 when the file has none; edge_weight_type is
