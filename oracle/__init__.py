@@ -98,6 +98,7 @@ def _load():
         "orc_colony_set_tau": (None, [C.c_void_p, _f32p, _f32p]),
         "orc_colony_set_ants": (None, [C.c_void_p, _i32p, C.c_int, _i32p, _f32p]),
         "orc_colony_ls_evals": (C.c_longlong, [C.c_void_p]),
+        "orc_colony_fallbacks": (C.c_longlong, [C.c_void_p]),
         "orc_colony_pm_state": (None, [C.c_void_p, _i32p]),
     }
     for name, (res, args) in sig.items():
@@ -386,6 +387,9 @@ class Colony:
 
     def ls_evals(self):
         return int(lib.orc_colony_ls_evals(self.h))
+
+    def fallbacks(self):
+        return int(lib.orc_colony_fallbacks(self.h))
 
     def pm_state(self):
         out = np.empty(self.m * 64, np.int32)

@@ -846,6 +846,7 @@ struct orc_colony {
     orc_bsf_record *bsf;
     int nbsf, bsf_cap;
     long long ls_evals;
+    long long fallbacks;
     double t0;
 };
 
@@ -900,16 +901,20 @@ void orc_colony_construct(orc_colony *c)
     orc_draw_cfg cfg = {c->prm.selection, c->prm.rng, c->prm.seed, (uint32_t)c->iteration};
     int nthreads = c->prm.threads > 0 ? c->prm.threads : 1;
     (void)nthreads;
+    long long fbs = 0;
 #ifdef _OPENMP
-#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads) reduction(+ : fbs)
 #endif
     for (int a = 0; a < c->m; ++a) {
         float len;
+        int fb = 0;
         int sz = orc_construct_ant(&c->p, c->tinfo, c->tiknn, &cfg, a, c->pm + (size_t)a * ORC_LANES,
-                                   c->tours + (size_t)a * c->T, &len, NULL);
+                                   c->tours + (size_t)a * c->T, &len, &fb);
         c->sizes[a] = sz;
         c->lens[a] = len;
+        fbs += fb;
     }
+    c->fallbacks += fbs;
 }
 
 void orc_colony_local_search(orc_colony *c)
@@ -1047,6 +1052,8 @@ void orc_colony_set_ants(orc_colony *c, const int *tours, int stride, const int 
         c->lens[a] = lens[a];
     }
 }
+
+long long orc_colony_fallbacks(const orc_colony *c) { return c->fallbacks; }
 
 long long orc_colony_ls_evals(const orc_colony *c) { return c->ls_evals; }
 

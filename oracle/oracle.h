@@ -148,6 +148,8 @@ void  orc_colony_set_tau(orc_colony *c, const float *tau, const float *tinfo);
 void  orc_colony_set_ants(orc_colony *c, const int *tours, int stride, const int *sizes,
                           const float *lens);
 long long orc_colony_ls_evals(const orc_colony *c);
+/* full-row greedy picks of the construction steps so far */
+long long orc_colony_fallbacks(const orc_colony *c);
 void  orc_colony_pm_state(const orc_colony *c, int32_t *out);
 
 #ifdef __cplusplus
