@@ -1,24 +1,32 @@
-"""Benchmark: ACO iterations per second on one MI355X, as ant-solutions/s.
+"""Benchmark: ACO iterations on one MI355X, as ant-solutions/s.
 
-Workload (the reference's metric configuration that fits one GPU): the
-481-node clustered instance -- 480 customers in 8 Gaussian clusters
-(sigma 6) in [0,1000]^2, random depot, demand U{1..100}, capacity for 30
-routes, nint int32 distances -- with 480 ants, alpha 1, beta 2, rho 0.1,
-6 ranks, roulette over the k=30 nearest candidates (Philox draws), and
-2-opt + swap + relocate local search.  One step = one whole iteration
+Workload (the largest single-GPU configuration of the metric): the
+1001-node mixed instance -- 500 uniform and 500 clustered customers (10
+Gaussian clusters, sigma 25) in [0,1000]^2, depot at the centre, demand
+U{1..100} with 5% heavy outliers in [80,100], capacity for ~12 customers per
+route, nint int32 distances -- with 256 ants, alpha 1, beta 2, rho 0.1, 6
+ranks, roulette over the k=32 nearest candidates (Philox draws) and full
+local search (2-opt + swap + relocate).  One step = one whole iteration
 (construct -> local search -> statistics -> fused pheromone pass), replayed
 as one hipGraph, inputs resident in HBM.
 
-Prints ONE JSON line.  Extra legs: per-kernel device time from HIP event
-pairs on the library's stream (roofline of the dominant kernel), the
-best-so-far trajectory of a full run (time to a 2% gap), and the CPU
-oracle timed on this host (cpu_baseline).
+Prints ONE JSON line.  Besides the timed steps: per-kernel device time from
+HIP event pairs on the library's stream (roofline of the dominant kernel
+against the LDS read bandwidth measured here by libcvrpprobe.so), the
+best-so-far trajectory of a 2000-iteration run (time to a 2% gap), the same
+workload under the reference's rules (I-Roulette over the full row), the
+CPU oracle on this host (reference rules on the CPU share, on one thread,
+and the candidate-roulette comparator), and the other configurations as
+extra keys: the 481-node clustered instance, 256 batched instances, and the
+learned prior against plain eta on 64 batched instances.
 """
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -30,8 +38,9 @@ sys.path.insert(0, ROOT)
 METRIC = "ant-solutions/s + time-to-2%-gap, learned-prior vs plain ACO, 1 MI355X"
 STAGES = {0: "construct", 1: "local_search", 2: "iteration"}
 HBM_PEAK_GBS = 8000.0
-WORKLOAD = dict(customers=480, clusters=8, sigma=6.0, routes=30, m=480, k=30, ls=7,
-                iterations_full=1000)
+WORKLOAD = dict(m=256, k=32, ls=7, iterations_full=2000)
+CLUSTERED = dict(customers=480, clusters=8, sigma=6.0, routes=30, m=480, k=30, ls=7)
+KERNELS = ("construct", "local_search", "statistics", "pheromone", "gather")
 
 
 def parse():
@@ -43,24 +52,35 @@ def parse():
     ap.add_argument("--full-iterations", type=int, default=WORKLOAD["iterations_full"],
                     help="length of the best-so-far run for time-to-gap (0 = skip)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
-                    help="CPU baseline sample budget (0 = skip)")
-    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cores)")
+                    help="CPU baseline sample budget per leg (0 = skip)")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="0 = this process's CPU share (OMP_NUM_THREADS if set, else the "
+                         "CPUs it may run on)")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--shape", default="",
+                    help="launch shape waves,ring,prefetch,apb,row16,fused instead of autotuning "
+                         "(profiling runs, so the trace holds only the measured shape)")
+    ap.add_argument("--reference-rules-steps", type=int, default=5,
+                    help="iterations of the workload under I-Roulette on the GPU (0 = skip)")
+    ap.add_argument("--clustered-steps", type=int, default=50,
+                    help="481-node clustered leg: timed iterations (0 = skip)")
     ap.add_argument("--prior-instances", type=int, default=64,
                     help="learned-prior vs plain leg: evaluation instances (0 = skip)")
     ap.add_argument("--prior-iterations", type=int, default=2000,
                     help="plain iterations that fix the reference cost C*")
-    ap.add_argument("--mlp-probe", type=int, default=0, metavar="INSTANCES",
-                    help="only run the learned prior's MLP on that many evaluation instances")
     ap.add_argument("--batch-instances", type=int, default=256,
                     help="instances of the batched leg (0 = skip)")
-    ap.add_argument("--n1000-seconds", type=float, default=10.0,
-                    help="GPU vs CPU oracle on the 1001-node instance (0 = skip)")
+    ap.add_argument("--mlp-probe", type=int, default=0, metavar="INSTANCES",
+                    help="only run the learned prior's MLP on that many evaluation instances")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 10 device-to-device copies of MIB MiB (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes) and exit")
+    ap.add_argument("--lds-probe", action="store_true",
+                    help="only run the LDS bandwidth microbenchmark and exit")
     return ap.parse_args()
 
+
+# ---------------------------------------------------------------- probes ----
 
 def copy_probe(mib: int) -> None:
     import torch
@@ -72,6 +92,22 @@ def copy_probe(mib: int) -> None:
     torch.cuda.synchronize()
     print(json.dumps({"copy_probe_bytes_read": nbytes, "copy_probe_bytes_written": nbytes,
                       "launches": 10}))
+
+
+def lds_peak():
+    """Aggregate LDS read bandwidth of this device (GB/s), measured by
+    libcvrpprobe.so: 2 blocks of 256 threads per CU, 2048 trips of 16
+    sixteen-byte reads per thread (about 0.6 ms per launch), best of 5."""
+    path = os.path.join(ROOT, "cvrp_aco_ocl_amd", "libcvrpprobe.so")
+    lib = ctypes.CDLL(path)
+    lib.probe_lds_bandwidth.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_double),
+                                        ctypes.POINTER(ctypes.c_double)]
+    gbps, ms = ctypes.c_double(), ctypes.c_double()
+    rc = lib.probe_lds_bandwidth(5, 2048, 2, ctypes.byref(gbps), ctypes.byref(ms))
+    if rc != 0:
+        raise RuntimeError(f"probe_lds_bandwidth failed ({rc})")
+    return gbps.value, ms.value
 
 
 def mlp_probe(instances_count: int) -> None:
@@ -100,17 +136,17 @@ def mlp_probe(instances_count: int) -> None:
                       "forward_flop_per_edge": 2 * (12 * 64 + 64 * 64 + 64)}))
 
 
-def make_solver(ca, inst, seed):
-    return ca.GpuACO(inst, n_ants=WORKLOAD["m"], k=WORKLOAD["k"], seed=seed + 1,
-                     selection=ca.SEL_KNN, rng=ca.RNG_PHILOX, ls_flags=WORKLOAD["ls"],
-                     use_graph=True)
+# --------------------------------------------------------------- helpers ----
+
+def log(msg):
+    print(f"# {msg}", file=sys.stderr, flush=True)
 
 
-def measured_traffic(kernel: str):
-    """HBM bytes per launch of ``kernel`` on this workload from the committed
-    FETCH_SIZE / WRITE_SIZE profile (calibrated against a plain copy), or
+def measured_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of ``kernel`` from the committed FETCH_SIZE /
+    WRITE_SIZE profile of ``workload`` (calibrated against a plain copy), or
     None when there is no measurement for it."""
-    path = os.path.join(ROOT, "profiles", "traffic_clustered481.md")
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.md")
     key = f"bench: {kernel}_traffic_bytes="
     try:
         with open(path) as fh:
@@ -123,19 +159,275 @@ def measured_traffic(kernel: str):
 
 
 def construct_bytes(inst, sizes, m, k, fallbacks):
-    """Algorithmic bytes of one construction launch: per step k candidate
-    ids + k total_info + the chosen distance + the tour write (8k+8 B), per
-    ant the demand and depot-distance rows (8n B), per full-row fallback
-    the total_info row (4n B)."""
+    """Algorithmic bytes of one candidate-roulette construction launch: per
+    step k candidate ids + k total_info + the chosen distance + the tour
+    write (8k+8 B), per ant the demand and depot-distance rows (8n B), per
+    full-row fallback the total_info row (4n B)."""
     steps = int(np.sum(sizes[:m] - 1))
     n = inst.n
     return steps * (8 * k + 8) + m * 8 * n + fallbacks * 4 * n
 
 
-def pheromone_bytes(inst):
+def pheromone_bytes(n):
     """Fused pass: read tau and heur, write tau and total_info (16 n^2 B)."""
-    return 16 * inst.n * inst.n
+    return 16 * n * n
 
+
+def cpu_share():
+    """Threads this process may use: OMP_NUM_THREADS when the host sets a
+    share, else the CPUs in its affinity mask."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return (int(env) if env.isdigit() and int(env) > 0 else aff), aff
+
+
+def cpu_model():
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return None
+
+
+def profile_iterations(g, inst, m, k, steps):
+    """Per-kernel device time (HIP event pairs on the library's stream) over
+    `steps` per-phase iterations, with the algorithmic bytes of each."""
+    g.profile_enable(True)
+    ev_before = g.counters()["ls_evals"]
+    cons_bytes = 0
+    for _ in range(steps):
+        fb0 = g.counters()["fallbacks"]
+        g.run_aco_iteration()
+        _, sizes, _ = g.ants()
+        # local search does not change tour sizes, so the construction's
+        # bytes follow from the sizes after the iteration
+        cons_bytes += construct_bytes(inst, sizes, m, k, g.counters()["fallbacks"] - fb0)
+    kern = {}
+    for name in KERNELS:
+        ms, launches = g.profile(name)
+        if launches:
+            kern[name] = ms / launches
+    ls_evals = (g.counters()["ls_evals"] - ev_before) / steps
+    g.profile_enable(False)
+    per = {}
+    for name, ms in kern.items():
+        e = {"avg_ms": round(ms, 4)}
+        if name == "construct":
+            e["algorithmic_bytes"] = int(cons_bytes / steps)
+            e["GBps"] = round(cons_bytes / steps / (ms * 1e-3) / 1e9, 2)
+        elif name == "pheromone":
+            e["algorithmic_bytes"] = pheromone_bytes(inst.n)
+            e["GBps"] = round(pheromone_bytes(inst.n) / (ms * 1e-3) / 1e9, 2)
+        elif name == "local_search":
+            e["move_evals_per_launch"] = int(ls_evals)
+            e["move_evals_per_s"] = round(ls_evals / (ms * 1e-3), 1)
+            e["algorithmic_lds_bytes"] = int(16 * ls_evals)
+        per[name] = e
+    return kern, per, ls_evals
+
+
+def time_to_gap(records, cstar, frac=0.02):
+    for it, length, t in records:
+        if length <= (1.0 + frac) * cstar:
+            return it, t
+    return None, None
+
+
+# ------------------------------------------------------------------- legs ----
+
+def leg_quality(ca, inst, shape, iterations, seed):
+    """2000-iteration plain run: C* = its final best; time and iteration at
+    which the best-so-far first came within 2% of C*."""
+    g = ca.GpuACO(inst, n_ants=WORKLOAD["m"], k=WORKLOAD["k"], seed=seed + 1, selection=ca.SEL_KNN,
+                  rng=ca.RNG_PHILOX, ls_flags=WORKLOAD["ls"], use_graph=True)
+    g.set_launch_shape(*shape)
+    t0 = time.perf_counter()
+    g.init_aco()
+    g.run(iterations)
+    g.synchronize()
+    wall = time.perf_counter() - t0
+    rec = g.bsf_records()
+    g.close()
+    cstar = rec[-1][1]
+    it, t = time_to_gap(rec, cstar)
+    log(f"{iterations} iterations: C* {cstar} in {wall:.2f}s, 2% gap at iteration {it} ({t:.3f}s)")
+    return {"iterations": iterations, "cstar": cstar, "wall_s": round(wall, 3),
+            "time_to_2pct_gap_s": round(t, 4), "iter_to_2pct_gap": it, "records": len(rec)}
+
+
+def leg_reference_rules_gpu(ca, inst, steps, seed):
+    """The workload under the reference's rules on the GPU: I-Roulette over
+    the full row with Park-Miller lane streams (kernel.cl:325-373), same
+    local search."""
+    g = ca.GpuACO(inst, n_ants=WORKLOAD["m"], seed=seed + 1, selection=ca.SEL_IROULETTE,
+                  rng=ca.RNG_PARK_MILLER, ls_flags=WORKLOAD["ls"], use_graph=True)
+    g.init_aco()
+    g.autotune(reps=2)
+    g.run(2)
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.run(steps)
+    g.synchronize()
+    el = time.perf_counter() - t0
+    g.profile_enable(True)
+    g.run_aco_iteration()
+    cons_ms = g.profile("construct")[0]
+    ls_ms = g.profile("local_search")[0]
+    g.close()
+    rate = WORKLOAD["m"] * steps / el
+    log(f"reference rules on the GPU: {rate:.0f} ant-sol/s ({1e3 * el / steps:.2f} ms/iteration)")
+    return {"ant_solutions_per_s": round(rate, 1), "ms_per_iteration": round(1e3 * el / steps, 3),
+            "construct_ms": round(cons_ms, 3), "local_search_ms": round(ls_ms, 3),
+            "rules": "I-Roulette over the full row, Park-Miller lane streams, 2-opt+swap+relocate"}
+
+
+def _cpu_rate(oracle, inst, m, selection, rng, k, threads, seconds, max_iters):
+    P = oracle.Problem(inst.dist, inst.demand, inst.capacity, inst.max_dist, inst.service_time,
+                       k=k, tie=0)
+    col = oracle.Colony(P, m, selection=selection, rng=rng, ls_flags=WORKLOAD["ls"], seed=1,
+                        tau0_rule=1, threads=threads)
+    col.init()
+    its, t0 = 0, time.perf_counter()
+    while True:
+        col.iteration()
+        its += 1
+        el = time.perf_counter() - t0
+        if el >= seconds or its >= max_iters:
+            break
+    return m * its / el, its, el
+
+
+def leg_cpu(inst, args):
+    """The CPU oracle on this host: the reference's rules (I-Roulette over
+    the full row, Park-Miller, the same local search) with OpenMP over ants
+    on this process's CPU share and on one thread, and the candidate
+    roulette (the GPU path's rule) as an algorithm-matched comparator."""
+    import oracle
+    share, affinity = cpu_share()
+    threads = args.cpu_threads or share
+    nproc = os.cpu_count()
+    ref_rate, ref_its, ref_el = _cpu_rate(oracle, inst, WORKLOAD["m"], oracle.SEL_IROULETTE,
+                                          oracle.RNG_PM, 0, threads, args.cpu_seconds, 100)
+    log(f"cpu reference rules, {threads} threads: {ref_rate:.1f} ant-sol/s")
+    one_m = 16
+    one_rate, one_its, one_el = _cpu_rate(oracle, inst, one_m, oracle.SEL_IROULETTE, oracle.RNG_PM,
+                                          0, 1, max(2.0, args.cpu_seconds / 2), 100)
+    log(f"cpu reference rules, 1 thread: {one_rate:.2f} ant-sol/s")
+    knn_rate, knn_its, knn_el = _cpu_rate(oracle, inst, WORKLOAD["m"], oracle.SEL_KNN,
+                                          oracle.RNG_PHILOX, WORKLOAD["k"], threads,
+                                          max(2.0, args.cpu_seconds / 2), 100)
+    log(f"cpu candidate roulette, {threads} threads: {knn_rate:.1f} ant-sol/s")
+    return {
+        "value": round(ref_rate, 2), "unit": "ant-solutions/s", "cores": threads, "kind": "port",
+        "sample": f"{ref_its} whole iterations of the same workload under the reference's rules "
+                  f"(I-Roulette over the full row, Park-Miller, 2-opt+swap+relocate; 256 ants, "
+                  f"OpenMP over ants) in {ref_el:.2f}s",
+        "nproc": nproc, "affinity_cpus": affinity,
+        "omp_num_threads": os.environ.get("OMP_NUM_THREADS"), "cpu_model": cpu_model(),
+        "one_thread": {"value": round(one_rate, 3),
+                       "sample": f"{one_its} iterations of {one_m} ants in {one_el:.2f}s"},
+        "per_core_times_nproc_estimate": round(one_rate * (nproc or 1), 1),
+        "knn_comparator": {"value": round(knn_rate, 2), "cores": threads,
+                           "sample": f"{knn_its} iterations (k=32 candidate roulette, Philox) in "
+                                     f"{knn_el:.2f}s"},
+    }
+
+
+def leg_clustered(ca, instances, args):
+    inst = instances.clustered(args.seed, customers=CLUSTERED["customers"],
+                               nclust=CLUSTERED["clusters"], sigma=CLUSTERED["sigma"],
+                               routes=CLUSTERED["routes"])
+    m, k = CLUSTERED["m"], CLUSTERED["k"]
+    mk = lambda: ca.GpuACO(inst, n_ants=m, k=k, seed=args.seed + 1, selection=ca.SEL_KNN,
+                           rng=ca.RNG_PHILOX, ls_flags=CLUSTERED["ls"], use_graph=True)
+    g = mk()
+    g.init_aco()
+    g.autotune(reps=3)
+    shape = g.launch_shape()
+    g.run(args.warmup)
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.run(args.clustered_steps)
+    g.synchronize()
+    el = time.perf_counter() - t0
+    if shape[5]:
+        g.set_launch_shape(*shape[:5], 0)
+    kern, per, _ = profile_iterations(g, inst, m, k, 10)
+    g.close()
+    out = {"workload": "481 nodes: 480 customers in 8 Gaussian clusters (sigma 6), random depot, "
+                       "Q for 30 routes; 480 ants, k=30 Philox roulette, 2-opt+swap+relocate",
+           "ant_solutions_per_s": round(m * args.clustered_steps / el, 1),
+           "ms_per_iteration": round(1e3 * el / args.clustered_steps, 4),
+           "launch_shape": shape, "kernels": per}
+    if args.full_iterations > 0:
+        g = mk()
+        g.set_launch_shape(*shape)
+        t0 = time.perf_counter()
+        g.init_aco()
+        g.run(2000)
+        g.synchronize()
+        wall = time.perf_counter() - t0
+        rec = g.bsf_records()
+        g.close()
+        it, t = time_to_gap(rec, rec[-1][1])
+        out["quality"] = {"iterations": 2000, "cstar": rec[-1][1], "wall_s": round(wall, 3),
+                          "time_to_2pct_gap_s": round(t, 4), "iter_to_2pct_gap": it}
+    log(f"clustered 481: {out['ant_solutions_per_s']:.0f} ant-sol/s")
+    return out
+
+
+def leg_batch(ca, instances, args):
+    binst = instances.batch(0, count=args.batch_instances)
+    ns = [it.n for it in binst]
+    t0 = time.perf_counter()
+    gq = ca.GpuACO(binst, n_ants=64, k=16, seed=1, selection=ca.SEL_KNN, rng=ca.RNG_PHILOX,
+                   ls_flags=7)
+    gq.init_aco()
+    gq.synchronize()
+    create_s = time.perf_counter() - t0
+    tuned = gq.autotune(reps=2)
+    gq.run(2)
+    gq.synchronize()
+    K = 10
+    t0 = time.perf_counter()
+    gq.run(K)
+    gq.synchronize()
+    el = time.perf_counter() - t0
+    bshape = gq.launch_shape()
+    gq.close()
+    out = {"workload": f"{len(binst)} instances, n {min(ns)}..{max(ns)} (uniform or 4-12 "
+                       "clusters, demands 1..50, 5-40 routes), 64 ants each, k=16 Philox "
+                       "roulette, 2-opt+swap+relocate, one batched context",
+           "ant_solutions_per_s": round(len(binst) * 64 * K / el, 1),
+           "ms_per_iteration": round(1e3 * el / K, 3), "create_and_init_s": round(create_s, 3),
+           "launch_shape": bshape,
+           "iteration_ms": {f"{'fused' if sh[5] else 'per-phase'}{sh}": round(ms, 3)
+                            for sh, st, ms in tuned if st == 2 and ms > 0}}
+    log(f"batch: {out['ant_solutions_per_s']:.0f} ant-sol/s, {out['ms_per_iteration']:.2f} ms/iter")
+    return out
+
+
+def leg_prior(instances, args):
+    from cvrp_aco_ocl_amd import prior as PR, train_prior as TP
+    net = PR.PriorNet.load()
+    ev = instances.evaluation(args.prior_instances)
+    log(f"prior leg: {len(ev)} instances, plain {args.prior_iterations} iterations")
+    out = TP.compare(ev, args.prior_iterations, net, ants=64, k=16, exponent=TP.PRIOR_EXPONENT,
+                     budgets=(0.5, 1.0, 2.0, 5.0, 10.0))
+    out["workload"] = ("64 clustered instances (500 customers, 8 clusters, sigma 6, 30 routes; "
+                       "seeds 100000+), 64 ants each, k=16 Philox roulette, 2-opt+swap+relocate, "
+                       f"one batched context; C* = plain best after {args.prior_iterations} "
+                       "iterations; prior time includes aco_prior_apply")
+    net.close()
+    log(f"prior leg: plain median {out['plain_time_to_2pct_s']['median']:.3f}s, prior median "
+        f"{out['prior_time_to_2pct_s']['median']:.3f}s")
+    return out
+
+
+# ------------------------------------------------------------------- main ----
 
 def main():
     args = parse()
@@ -147,25 +439,31 @@ def main():
     if args.copy_probe:
         copy_probe(args.copy_probe)
         return
+    if args.lds_probe:
+        gbps, ms = lds_peak()
+        print(json.dumps({"lds_read_GBps": round(gbps, 1), "best_launch_ms": round(ms, 4)}))
+        return
     if args.mlp_probe:
         mlp_probe(args.mlp_probe)
         return
     import cvrp_aco_ocl_amd as ca
     from cvrp_aco_ocl_amd import instances
 
-    inst = instances.clustered(args.seed, customers=WORKLOAD["customers"],
-                               nclust=WORKLOAD["clusters"], sigma=WORKLOAD["sigma"],
-                               routes=WORKLOAD["routes"])
+    inst = instances.mixed_1000(args.seed)
     m, k = WORKLOAD["m"], WORKLOAD["k"]
-    print(f"# workload {inst.name}: n={inst.n} Q={inst.capacity} m={m} k={k}", file=sys.stderr,
-          flush=True)
+    log(f"workload {inst.name}: n={inst.n} Q={inst.capacity} m={m} k={k}")
 
     # ---- timed region: K iterations, graph replay ----
-    g = make_solver(ca, inst, args.seed)
+    g = ca.GpuACO(inst, n_ants=m, k=k, seed=args.seed + 1, selection=ca.SEL_KNN,
+                  rng=ca.RNG_PHILOX, ls_flags=WORKLOAD["ls"], use_graph=True)
     g.init_aco()
-    tune = g.autotune(reps=3)   # local-search launch shape; results are the same for every shape
+    if args.shape:
+        g.set_launch_shape(*[int(v) for v in args.shape.split(",")])
+        tune = []
+    else:
+        tune = g.autotune(reps=3)   # launch shapes; results are the same for every shape
     shape = g.launch_shape()
-    print(f"# autotune: {tune} -> {shape}", file=sys.stderr, flush=True)
+    log(f"autotune -> {shape}")
     g.run(args.warmup)
     g.synchronize()
     t0 = time.perf_counter()
@@ -174,208 +472,56 @@ def main():
     elapsed = time.perf_counter() - t0
     ms_per_step = 1e3 * elapsed / args.steps
     value = m * args.steps / elapsed
-    cnt0 = g.counters()
-    print(f"# timed {args.steps} iterations: {ms_per_step:.3f} ms/iter, {value:.0f} ant-sol/s",
-          file=sys.stderr, flush=True)
+    log(f"timed {args.steps} iterations: {ms_per_step:.3f} ms/iter, {value:.0f} ant-sol/s")
 
-    # ---- per-kernel device time (HIP events on the library stream) ----
-    # (per-phase launches, so each kernel is timed on its own)
+    # ---- per-kernel device time (per-phase launches, HIP events) ----
     if shape[5]:
         g.set_launch_shape(*shape[:5], 0)
-    g.profile_enable(True)
-    fb_before = g.counters()["fallbacks"]
-    ev_before = g.counters()["ls_evals"]
-    size_sum = 0
-    cons_bytes = 0
-    for _ in range(args.profile_steps):
-        fb0 = g.counters()["fallbacks"]
-        g.run_aco_iteration()
-        _, sizes, _ = g.ants()
-        # construct bytes need this iteration's tours before local search moved
-        # nodes around: sizes do not change under local search
-        cons_bytes += construct_bytes(inst, sizes, m, k, g.counters()["fallbacks"] - fb0)
-        size_sum += int(np.sum(sizes[:m]))
-    kern = {}
-    for name in ("construct", "local_search", "statistics", "pheromone", "gather"):
-        ms, launches = g.profile(name)
-        if launches:
-            kern[name] = {"avg_ms": ms / launches, "launches": launches}
-    cnt1 = g.counters()
-    ls_evals = cnt1["ls_evals"] - ev_before
-    g.profile_enable(False)
+    kern, per_kernel, ls_evals = profile_iterations(g, inst, m, k, args.profile_steps)
     g.close()
-
-    avg = {n: v["avg_ms"] for n, v in kern.items()}
-    dominant = max(avg, key=avg.get)
-    cons_bpl = cons_bytes / args.profile_steps
-    ph_bpl = pheromone_bytes(inst)
-    per_kernel = {}
-    for name, ms in avg.items():
-        entry = {"avg_ms": round(ms, 4)}
-        if name == "construct":
-            entry["algorithmic_bytes"] = int(cons_bpl)
-            entry["GBps"] = round(cons_bpl / (ms * 1e-3) / 1e9, 2)
-        if name == "pheromone":
-            entry["algorithmic_bytes"] = ph_bpl
-            entry["GBps"] = round(ph_bpl / (ms * 1e-3) / 1e9, 2)
-        if name == "local_search":
-            entry["move_evals_per_s"] = round(ls_evals / args.profile_steps / (ms * 1e-3), 1)
-        per_kernel[name] = entry
-    if dominant == "construct":
-        achieved = cons_bpl / (avg["construct"] * 1e-3) / 1e9
-        traffic_alg = int(cons_bpl)
-    elif dominant == "pheromone":
-        achieved = ph_bpl / (avg["pheromone"] * 1e-3) / 1e9
-        traffic_alg = ph_bpl
+    dominant = max(kern, key=kern.get)
+    peak_lds, _ = lds_peak()
+    log(f"LDS read peak {peak_lds:.0f} GB/s; dominant kernel {dominant} {kern[dominant]:.3f} ms")
+    if dominant == "local_search":
+        # 16 B per evaluated move (the four distance / edge words of a 2-opt,
+        # swap or relocate delta), all read from LDS
+        alg = int(16 * ls_evals)
+        achieved = alg / (kern[dominant] * 1e-3) / 1e9
+        roofline = {"bound": "lds", "kernel": dominant, "achieved": round(achieved, 2),
+                    "peak": round(peak_lds, 1), "unit": "GB/s", "frac": round(achieved / peak_lds, 5),
+                    "traffic": measured_traffic(dominant, "mixed1001") if args.seed == 0 else None,
+                    "algorithmic_bytes_per_launch": alg,
+                    "peak_source": "LDS read bandwidth measured in this run (libcvrpprobe.so)"}
     else:
-        # local search: 16 B per evaluated move (the four distance / edge
-        # words a 2-opt, swap or relocate delta reads)
-        traffic_alg = int(16 * ls_evals / max(args.profile_steps, 1))
-        achieved = traffic_alg / (avg[dominant] * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 2),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                "traffic": measured_traffic(dominant) if args.seed == 0 else None,
-                "algorithmic_bytes_per_launch": traffic_alg}
+        alg = per_kernel[dominant].get("algorithmic_bytes", 0)
+        achieved = alg / (kern[dominant] * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 2),
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                    "traffic": measured_traffic(dominant, "mixed1001") if args.seed == 0 else None,
+                    "algorithmic_bytes_per_launch": alg}
+    ph = per_kernel.get("pheromone")
+    hbm_pass = None
+    if ph:
+        hbm_pass = {"bound": "hbm", "kernel": "pheromone", "achieved": ph["GBps"],
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(ph["GBps"] / HBM_PEAK_GBS, 5),
+                    "note": "4 MB per matrix at this size: resident in the last-level cache"}
 
-    # ---- best-so-far trajectory: time to 2% of the run's final best ----
-    quality = None
-    if args.full_iterations > 0:
-        g = make_solver(ca, inst, args.seed)
-        g.set_launch_shape(*shape)
-        t0 = time.perf_counter()
-        g.init_aco()
-        g.run(args.full_iterations)
-        g.synchronize()
-        wall = time.perf_counter() - t0
-        rec = g.bsf_records()
-        g.close()
-        best = rec[-1][1]
-        target = 1.02 * best
-        hit = next(r for r in rec if r[1] <= target)
-        quality = {"iterations": args.full_iterations, "best": best, "wall_s": round(wall, 3),
-                   "time_to_2pct_gap_s": round(hit[2], 4), "iter_to_2pct_gap": hit[0],
-                   "records": len(rec)}
-        print(f"# full run {args.full_iterations} it: best {best} in {wall:.2f}s, 2% gap at "
-              f"it {hit[0]} ({hit[2]:.3f}s)", file=sys.stderr, flush=True)
-
-    # ---- CPU baseline: the oracle on this host's cores ----
-    cpu = None
-    if args.cpu_seconds > 0:
-        import oracle
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        P = oracle.Problem(inst.dist, inst.demand, inst.capacity, inst.max_dist,
-                           inst.service_time, k=k, tie=0)
-        col = oracle.Colony(P, m, selection=oracle.SEL_KNN, rng=oracle.RNG_PHILOX,
-                            ls_flags=WORKLOAD["ls"], seed=args.seed + 1, threads=threads)
-        col.init()
-        its = 0
-        t0 = time.perf_counter()
-        while True:
-            col.iteration()
-            its += 1
-            el = time.perf_counter() - t0
-            if el >= args.cpu_seconds or its >= 200:
-                break
-        cpu = {"value": round(m * its / el, 2), "unit": "ant-solutions/s", "cores": threads,
-               "kind": "port",
-               "sample": f"{its} whole iterations (480 ants each, same workload, OpenMP over ants) "
-                         f"in {el:.2f}s"}
-        print(f"# cpu baseline: {cpu['value']} ant-sol/s on {threads} threads", file=sys.stderr,
-              flush=True)
-
-    # ---- learned prior vs plain: 64 clustered n=501 instances, batched ----
-    prior_leg = None
-    if args.prior_instances > 0:
-        from cvrp_aco_ocl_amd import prior as PR, train_prior as TP
-        net = PR.PriorNet.load()
-        ev = instances.evaluation(args.prior_instances)
-        print(f"# prior leg: {len(ev)} instances, plain {args.prior_iterations} iterations",
-              file=sys.stderr, flush=True)
-        prior_leg = TP.compare(ev, args.prior_iterations, net, ants=64, k=16,
-                               exponent=TP.PRIOR_EXPONENT, budgets=(0.5, 1.0, 2.0, 5.0, 10.0))
-        prior_leg["workload"] = ("64 clustered instances (500 customers, 8 clusters, sigma 6, "
-                                 "30 routes; seeds 100000+), 64 ants each, k=16 Philox roulette, "
-                                 "2-opt+swap+relocate, one batched context; C* = plain best after "
-                                 f"{args.prior_iterations} iterations; prior time includes "
-                                 "aco_prior_apply")
-        net.close()
-        print(f"# prior leg: plain median {prior_leg['plain_time_to_2pct_s']['median']:.3f}s, "
-              f"prior median {prior_leg['prior_time_to_2pct_s']['median']:.3f}s",
-              file=sys.stderr, flush=True)
-
-    # ---- 1001-node instance: GPU vs the CPU oracle on all host cores ----
-    n1000 = None
-    if args.n1000_seconds > 0:
-        big = instances.mixed_1000(0)
-        gb = ca.GpuACO(big, n_ants=256, k=32, seed=1, selection=ca.SEL_KNN, rng=ca.RNG_PHILOX,
-                       ls_flags=7)
-        gb.init_aco()
-        tuned = gb.autotune(reps=2)
-        gb.run(3)
-        gb.synchronize()
-        t0 = time.perf_counter()
-        gb.run(20)
-        gb.synchronize()
-        g_rate = 256 * 20 / (time.perf_counter() - t0)
-        gb_shape = gb.launch_shape()
-        gb.close()
-        import oracle
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        Pb = oracle.Problem(big.dist, big.demand, big.capacity, big.max_dist, big.service_time,
-                            k=32, tie=0)
-        colb = oracle.Colony(Pb, 256, selection=oracle.SEL_KNN, rng=oracle.RNG_PHILOX, ls_flags=7,
-                             seed=1, threads=threads)
-        colb.init()
-        its, t0 = 0, time.perf_counter()
-        while True:
-            colb.iteration()
-            its += 1
-            el = time.perf_counter() - t0
-            if el >= args.n1000_seconds or its >= 50:
-                break
-        c_rate = 256 * its / el
-        n1000 = {"workload": "1001 nodes (500 uniform + 500 clustered, 5% heavy demands, ~12 "
-                             "customers/route), 256 ants, k=32, full local search",
-                 "gpu_ant_solutions_per_s": round(g_rate, 1),
-                 "cpu_ant_solutions_per_s": round(c_rate, 2), "cpu_threads": threads,
-                 "cpu_sample": f"{its} iterations in {el:.2f}s",
-                 "ratio": round(g_rate / c_rate, 2), "launch_shape": gb_shape,
-                 "autotune_ms": {f"{STAGES[st]}{sh}": round(ms, 3) for sh, st, ms in tuned}}
-        print(f"# n=1001: GPU {g_rate:.0f} vs CPU {c_rate:.1f} ant-sol/s ({g_rate / c_rate:.1f}x)",
-              file=sys.stderr, flush=True)
-
-    # ---- 256 uneven instances batched in one context (parity-test config,
-    # reported for its throughput; not the metric's workload) ----
-    batch = None
-    if args.batch_instances > 0:
-        binst = instances.batch(0, count=args.batch_instances)
-        ns = [it.n for it in binst]
-        t0 = time.perf_counter()
-        gq = ca.GpuACO(binst, n_ants=64, k=16, seed=1, selection=ca.SEL_KNN, rng=ca.RNG_PHILOX,
-                       ls_flags=7)
-        gq.init_aco()
-        gq.synchronize()
-        create_s = time.perf_counter() - t0
-        tuned = gq.autotune(reps=2)
-        gq.run(2)
-        gq.synchronize()
-        K = 10
-        t0 = time.perf_counter()
-        gq.run(K)
-        gq.synchronize()
-        el = time.perf_counter() - t0
-        bshape = gq.launch_shape()
-        gq.close()
-        batch = {"workload": f"{len(binst)} instances, n {min(ns)}..{max(ns)} (uniform or 4-12 "
-                             "clusters, demands 1..50, 5-40 routes), 64 ants each, k=16 Philox "
-                             "roulette, 2-opt+swap+relocate, one batched context",
-                 "ant_solutions_per_s": round(len(binst) * 64 * K / el, 1),
-                 "ms_per_iteration": round(1e3 * el / K, 3), "create_and_init_s": round(create_s, 3),
-                 "launch_shape": bshape,
-                 "iteration_ms": {f"{'fused' if sh[5] else 'per-phase'}{sh}": round(ms, 3)
-                                  for sh, st, ms in tuned if st == 2 and ms > 0}}
-        print(f"# batch: {batch['ant_solutions_per_s']:.0f} ant-sol/s, {batch['ms_per_iteration']:.2f} "
-              f"ms/iter, shape {bshape}", file=sys.stderr, flush=True)
+    quality = leg_quality(ca, inst, shape, args.full_iterations, args.seed) \
+        if args.full_iterations > 0 else None
+    ref_gpu = leg_reference_rules_gpu(ca, inst, args.reference_rules_steps, args.seed) \
+        if args.reference_rules_steps > 0 else None
+    cpu = leg_cpu(inst, args) if args.cpu_seconds > 0 else None
+    if cpu is not None:
+        cpu["gpu_over_cpu"] = {"candidate_roulette_gpu_vs_reference_rules_cpu": round(value / cpu["value"], 2),
+                               "candidate_roulette_gpu_vs_candidate_roulette_cpu":
+                                   round(value / cpu["knn_comparator"]["value"], 2)}
+        if ref_gpu is not None:
+            cpu["gpu_over_cpu"]["reference_rules_gpu_vs_reference_rules_cpu"] = round(
+                ref_gpu["ant_solutions_per_s"] / cpu["value"], 2)
+    clustered = leg_clustered(ca, instances, args) if args.clustered_steps > 0 else None
+    batch = leg_batch(ca, instances, args) if args.batch_instances > 0 else None
+    prior_leg = leg_prior(instances, args) if args.prior_instances > 0 else None
 
     out = {
         "metric": METRIC,
@@ -391,9 +537,11 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"clustered-481: {WORKLOAD['customers']} customers, 8 Gaussian clusters "
-                        f"sigma 6, Q for 30 routes, nint int32 distances; m=480, k=30 roulette_knn "
-                        f"(Philox), 2-opt+swap+relocate, alpha 1 beta 2 rho 0.1, 6 ranks",
+            "workload": "mixed-1001: 1000 customers (500 uniform, 500 in 10 Gaussian clusters "
+                        "sigma 25) in [0,1000]^2, central depot, demand U{1..100} with 5% in "
+                        "[80,100], Q for ~12 customers per route, nint int32 distances; m=256, "
+                        "k=32 roulette_knn (Philox), 2-opt+swap+relocate, alpha 1 beta 2 rho 0.1, "
+                        "6 ranks",
             "n": inst.n, "ants": m, "k": k, "capacity": inst.capacity, "seed": args.seed,
             "step": "one ACO iteration (hipGraph replay)",
             "launch_shape": {"ls_waves": shape[0], "ls_ring": shape[1], "ls_prefetch": shape[2],
@@ -401,13 +549,16 @@ def main():
                              "fused_iteration": shape[5]},
         },
         "roofline": roofline,
+        "roofline_pheromone_pass": hbm_pass,
         "kernels": per_kernel,
-        "ls_move_evals_per_iter": int(ls_evals / max(args.profile_steps, 1)),
+        "ls_move_evals_per_iter": int(ls_evals),
+        "autotune_ms": {f"{STAGES[st]}{sh}": round(ms, 3) for sh, st, ms in tune},
         "quality": quality,
-        "learned_prior": prior_leg,
-        "n1001": n1000,
-        "batch256": batch,
+        "reference_rules_gpu": ref_gpu,
         "cpu_baseline": cpu,
+        "clustered481": clustered,
+        "batch256": batch,
+        "learned_prior": prior_leg,
     }
     print(json.dumps(out))
 

@@ -685,6 +685,7 @@ int aco_create_batch(const aco_problem *pbs, int32_t count, const aco_params *pr
     CHK(hipMemcpyAsync(dinst, ant_inst.data(), 4 * ant_inst.size(), hipMemcpyHostToDevice, c->stream));
     CHK(hipMemcpyAsync(dbase, ant_base.data(), 4 * ant_base.size(), hipMemcpyHostToDevice, c->stream));
     CHK(hipStreamSynchronize(c->stream));
+    bt.cons_lds_budget = c->lds_limit - c->lds_static_cons;
     bt.probs = dprobs;
     bt.states = dstates;
     bt.ant_inst = dinst;
@@ -1034,7 +1035,7 @@ static bool shape_fits(const aco_ctx *c, const LsShape &sh)
         return false;
     if (!construct_staged_ok(c->bt, c->prm.selection, c->prm.k, c->m, sh.construct_apb)) return false;
     return sh.construct_apb == 0 ||
-           construct_lds_staged_bytes(c->max_n, c->prm.k, sh.construct_apb) + c->lds_static_cons <= c->lds_limit;
+           construct_staged_rows(c->max_n, c->prm.k, sh.construct_apb, c->bt.cons_lds_budget) >= 0;
 }
 
 static aco_launch_shape to_public(const LsShape &sh)

@@ -17,6 +17,16 @@ namespace cvrp {
 
 constexpr int kLanes = 64;             // wavefront = one ant
 
+// Load through a global-address-space pointer: the device arrays reach the
+// kernels through descriptor structs, so the compiler sees generic pointers
+// and would emit flat loads, which also occupy the LDS counter (an LDS wait
+// then waits for them too) and cost more to issue.
+template <typename T>
+__device__ __forceinline__ T gload(const T *p)
+{
+    return *(const __attribute__((address_space(1))) T *)(p);
+}
+
 // Index of the calling wave in its block, as a wave-uniform value (the
 // compiler cannot prove threadIdx.x >> 6 uniform by itself; without this,
 // loops and branches on it are compiled as divergent).
@@ -183,6 +193,7 @@ struct DevBatch {
     int ls_first[kLsClasses + 1];     // class c: ls_order[ls_first[c] .. ls_first[c+1])
     int ls_maxn[kLsClasses];          // largest n in class c
     int *fq_next;                     // fused iteration: next work item (reset before each launch)
+    size_t cons_lds_budget;           // dynamic LDS the staged construction may use (host side)
 };
 
 enum { kTau0Arg = 0, kTau0Bsf = 1, kTau0Nn = 2 };
@@ -201,7 +212,11 @@ struct LsShape {
 // apb = ants per block of the LDS-staged candidate roulette (0: rows from
 // L2/HBM, one ant per block); k = candidate list width
 hipError_t launch_construct(const DevBatch &b, int selection, int rng, int apb, int k, hipStream_t st);
-size_t construct_lds_staged_bytes(int n, int k, int apb);
+// Staged construction for instances of up to n nodes within `budget` bytes
+// of dynamic LDS: candidate rows staged (n when the whole table fits, -1
+// when not even the demand and visited arrays fit) and the LDS launched.
+int construct_staged_rows(int n, int k, int apb, size_t budget);
+size_t construct_lds_staged_bytes(int n, int k, int apb, size_t budget);
 bool construct_staged_ok(const DevBatch &b, int selection, int k, int ants_per_instance, int apb);
 // Extra streams on which the size classes of a batch run beside the first
 // (forked from and joined back into the launch stream; under capture they

@@ -145,9 +145,13 @@ __device__ __forceinline__ bool feasible_mask(const DevProblem &p, const AntLds 
 
 // Full-row greedy pick when the roulette has no positive mass: largest
 // total_info among feasible nodes, ties to the lowest index; 0 = none.
-// The row's values are loaded kGreedyChunk per lane at a time, all issued
-// before any is used, so a chunk costs one memory latency.
-constexpr int kGreedyChunk = 8;
+// Nodes are taken kGreedyChunk per lane at a time (kGreedyChunk * 64 per
+// pass).  The visited flags and capacity test come from LDS first; a pass
+// whose nodes are all infeasible reads nothing from global memory (a route
+// that is full returns to the depot without touching the row), otherwise
+// the row values (and, with a length limit, distances) of the feasible
+// nodes are loaded together, one memory latency per pass.
+constexpr int kGreedyChunk = 16;
 
 template <bool HAS_D>
 __device__ int greedy_pick(const DevProblem &p, const AntLds &L, const float *tirow,
@@ -155,18 +159,35 @@ __device__ int greedy_pick(const DevProblem &p, const AntLds &L, const float *ti
 {
     u64 best = 0;
     for (int base = 0; base < p.n; base += kGreedyChunk * kLanes) {
-        float tv[kGreedyChunk], dv[kGreedyChunk];
+        unsigned fm = 0;   // bit u: node base + u*64 + lane passes the LDS tests
 #pragma unroll
         for (int u = 0; u < kGreedyChunk; ++u) {
             const int i = base + u * kLanes + lane;
             const int ic = i < p.n ? i : p.n - 1;   // in-row index, masked below
-            tv[u] = tirow[ic];
-            dv[u] = HAS_D ? drow[ic] : 0.0f;
+            const bool f = (i < p.n) & (L.vis[ic] == 0) & (load + L.dem[ic] <= p.Q);
+            fm |= f ? 1u << u : 0u;
+        }
+        if (!__ballot(fm != 0)) continue;
+        // every load of the pass unconditional (clamped index, masked by fm
+        // below), so they are all in flight together
+        float tv[kGreedyChunk], dv[kGreedyChunk];
+#pragma unroll
+        for (int u = 0; u < kGreedyChunk; ++u) {
+            const int i = base + u * kLanes + lane;
+            const int ic = i < p.n ? i : p.n - 1;
+            tv[u] = gload(tirow + ic);
+            dv[u] = HAS_D ? gload(drow + ic) : 0.0f;
         }
 #pragma unroll
         for (int u = 0; u < kGreedyChunk; ++u) {
             const int i = base + u * kLanes + lane;
-            if (i < p.n && feasible<HAS_D>(p, L, i, dv[u], load, rdist)) {
+            bool f = (fm >> u) & 1u;
+            if (HAS_D) {
+                float t = rdist + (dv[u] + p.svc);
+                t = t + L.d0[i < p.n ? i : p.n - 1];
+                f = f & (t <= p.D);
+            }
+            if (f) {
                 const u64 key = ((u64)(f2u(tv[u]) + 1u) << 32) | (u64)(0xFFFFFFFFu - (u32)i);
                 best = key > best ? key : best;
             }
@@ -320,9 +341,9 @@ __device__ void construct_knn_ant(const DevProblem &p, const DevState &s, int a,
         // last entry and are masked.  total_info is finite and >= 0, so the
         // product by 1 or 0 is exact and keeps its load unconditional.
         const int e = cur * k + (lane < k ? lane : k - 1);
-        const int j = p.nnk[e];
-        const float tj = s.tiknn[e];
-        const float dcj = HAS_D ? p.nnkd[e] : 0.0f;
+        const int j = gload(p.nnk + e);
+        const float tj = gload(s.tiknn + e);
+        const float dcj = HAS_D ? gload(p.nnkd + e) : 0.0f;
         const int dj = L.dem[j];   // the chosen lane's value is read out below
         const bool ok = (lane < k) & feasible_mask<HAS_D>(p, L, j, dj, dcj, load, rdist);
         const float pj = tj * (ok ? 1.0f : 0.0f);
@@ -406,13 +427,19 @@ __global__ __launch_bounds__(64) void k_construct_knn(DevBatch bt)
 // The same roulette with the instance's candidate table staged in LDS, for
 // instances without a route-length limit and with 16-bit demands: A ants
 // per block (one wave each, no block-wide barrier after staging) share the
-// n*k candidate entries {total_info, id | demand << 16}, 8 bytes each, so a
+// candidate entries {total_info, id | demand << 16}, 8 bytes each, so a
 // step reads its candidates' values, ids and demands with one LDS load and
-// then their visited flags with a second.  The chosen edge's length is
-// fetched after the pick and added one step later (the same additions in
-// the same order).
+// then their visited flags with a second.  When the whole n*k table does not
+// fit, the first `rows_cap` rows are staged and a step from a later node
+// reads its row from the candidate arrays in global memory (ids and values
+// together, then the demands from LDS), as the unstaged kernel does; the
+// branch is wave-uniform.  The chosen edge's length is fetched after the
+// pick and added one step later (the same additions in the same order).
+// LDS: dem[n] | vis[A][pitch] | ent[rows * k].
+__device__ __forceinline__ int construct_vis_pitch(int n) { return (n + 15) & ~15; }
+
 template <int RNG, int A>
-__global__ __launch_bounds__(64 * A) void k_construct_knn_lds(DevBatch bt)
+__global__ __launch_bounds__(64 * A) void k_construct_knn_lds(DevBatch bt, int rows_cap)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const int wv = wave_id(), lane = threadIdx.x & 63;
@@ -420,38 +447,65 @@ __global__ __launch_bounds__(64 * A) void k_construct_knn_lds(DevBatch bt)
     const DevProblem p = bt.probs[inst];
     const DevState s = bt.states[inst];
     const int a = blockIdx.x * A + wv - bt.ant_base[inst];
-    const int n = p.n, k = p.k, nk = n * k;
-    uint2 *ent = reinterpret_cast<uint2 *>(smem);
-    int *dem = reinterpret_cast<int *>(ent + nk);
-    float *d0 = reinterpret_cast<float *>(dem + n);
-    unsigned char *vis = reinterpret_cast<unsigned char *>(d0 + n) + wv * ((n + 15) & ~15);
-    for (int e = threadIdx.x; e < nk; e += 64 * A) {
-        const int id = p.nnk[e];
-        ent[e] = make_uint2(__float_as_uint(s.tiknn[e]), (unsigned)id | ((unsigned)p.dem[id] << 16));
-    }
-    for (int i = threadIdx.x; i < n; i += 64 * A) {
-        dem[i] = p.dem[i];
-        d0[i] = p.d0[i];
-    }
+    const int n = p.n, k = p.k;
+    const int rows = n < rows_cap ? n : rows_cap;       // staged rows 0 .. rows-1
+    const int nmax = bt.max_n, pitch = construct_vis_pitch(nmax);
+    int *dem = reinterpret_cast<int *>(smem);
+    unsigned char *vis_all = reinterpret_cast<unsigned char *>(dem + ((nmax + 3) & ~3));
+    unsigned char *vis = vis_all + wv * pitch;
+    uint2 *ent = reinterpret_cast<uint2 *>(vis_all + A * pitch);
+    for (int i = threadIdx.x; i < n; i += 64 * A) dem[i] = p.dem[i];
     for (int i = lane; i < n; i += kLanes) vis[i] = i == 0;
     __syncthreads();
-    const AntLds L{dem, d0, nullptr, vis};
+    // entries kStageUnroll per thread at a time: ids and values loaded
+    // together, then the demands of those ids from LDS
+    constexpr int kStageUnroll = 8;
+    const int nent = rows * k;
+    for (int e0 = threadIdx.x; e0 < nent; e0 += 64 * A * kStageUnroll) {
+        int id[kStageUnroll];
+        unsigned tv[kStageUnroll];
+#pragma unroll
+        for (int u = 0; u < kStageUnroll; ++u) {
+            const int e = e0 + u * 64 * A;
+            const int ec = e < nent ? e : nent - 1;   // in-range index, store masked below
+            id[u] = gload(p.nnk + ec);
+            tv[u] = __float_as_uint(gload(s.tiknn + ec));
+        }
+#pragma unroll
+        for (int u = 0; u < kStageUnroll; ++u) {
+            const int e = e0 + u * 64 * A;
+            if (e < nent) ent[e] = make_uint2(tv[u], (unsigned)id[u] | ((unsigned)dem[id[u]] << 16));
+        }
+    }
+    __syncthreads();
+    const AntLds L{dem, nullptr, nullptr, vis};
     const u64 seed = p.seed;
     const u32 iter = (u32)s.ctrl[kCtrlIter];
     int z = RNG == kRngParkMiller ? s.pm[a * kLanes] : 0;
     int *tour = s.tours + (size_t)a * p.T;
     int cur = 0, load = 0, vcnt = 1, step = 0, fb = 0, tbuf = 0;
     float ublk = 0.0f;   // Philox draws of steps step+1 .. step+64, refreshed every 64 steps
+    const int c = lane < k ? lane : k - 1;   // lanes >= k read the row's last entry and are masked
     while (vcnt < n) {
         step++;
         if (RNG == kRngPhilox && ((step - 1) & 63) == 0) ublk = philox_block(seed, iter, (u32)a, (u32)step, lane);
-        // lanes >= k read the row's last entry and are masked
-        const uint2 en = ent[cur * k + (lane < k ? lane : k - 1)];
-        const int j = (int)(en.y & 0xFFFFu), dj = (int)(en.y >> 16);
+        int j, dj;
+        float tj;
+        if (cur < rows) {
+            const uint2 en = ent[cur * k + c];
+            j = (int)(en.y & 0xFFFFu);
+            dj = (int)(en.y >> 16);
+            tj = __uint_as_float(en.x);
+        } else {
+            const int e = cur * k + c;
+            j = gload(p.nnk + e);
+            tj = gload(s.tiknn + e);
+            dj = dem[j];
+        }
         const bool ok = (lane < k) & (vis[j] == 0) & (load + dj <= p.Q);   // the mask of feasible<false>
         // total_info is finite and >= 0, so x * 1 = x and x * 0 = 0 exactly; the
         // product keeps its load unconditional (one 8-byte LDS read with the id)
-        const float pj = __uint_as_float(en.x) * (ok ? 1.0f : 0.0f);
+        const float pj = tj * (ok ? 1.0f : 0.0f);
         const float x = wave_scan_f32(pj, lane);
         const float total = lane_f32(x, k - 1);
         int w, dw;
@@ -505,10 +559,33 @@ __global__ __launch_bounds__(64 * A) void k_construct_knn_lds(DevBatch bt)
     }
 }
 
-size_t construct_lds_staged_bytes(int n, int k, int apb)
+// LDS of the staged kernel for instances of up to n nodes: the demand and
+// visited arrays, then as many candidate rows as fit in `budget` bytes (all
+// n when they do).  Returns 0 when not even the fixed part fits.
+static size_t construct_fixed_bytes(int n, int apb)
 {
-    const size_t nk = (size_t)n * k;
-    return nk * 8 + (size_t)n * 8 + (size_t)apb * ((n + 15) & ~15) + 16;
+    return (size_t)((n + 3) & ~3) * 4 + (size_t)apb * ((n + 15) & ~15) + 16;
+}
+
+int construct_staged_rows(int n, int k, int apb, size_t budget)
+{
+    const size_t fixed = construct_fixed_bytes(n, apb);
+    if (k <= 0 || fixed > budget) return -1;
+    const size_t rows = (budget - fixed) / ((size_t)k * 8);
+    return rows < (size_t)n ? (int)rows : n;
+}
+
+size_t construct_lds_staged_bytes(int n, int k, int apb, size_t budget)
+{
+    const int rows = construct_staged_rows(n, k, apb, budget);
+    return rows < 0 ? (size_t)-1 : construct_fixed_bytes(n, apb) + (size_t)rows * k * 8;
+}
+
+bool construct_staged_ok(const DevBatch &b, int selection, int k, int ants_per_instance, int apb)
+{
+    if (apb == 0) return true;
+    if (apb != 1 && apb != 2 && apb != 4) return false;
+    return selection == kSelKnn && !b.has_d && b.dem16 && k > 0 && ants_per_instance % apb == 0;
 }
 
 static hipError_t max_static_lds(const void *const *fns, int count, size_t &bytes)
@@ -537,19 +614,14 @@ hipError_t construct_static_lds(size_t &bytes)
     return max_static_lds(fns, (int)(sizeof(fns) / sizeof(fns[0])), bytes);
 }
 
-bool construct_staged_ok(const DevBatch &b, int selection, int k, int ants_per_instance, int apb)
-{
-    if (apb == 0) return true;
-    if (apb != 1 && apb != 2 && apb != 4) return false;
-    return selection == kSelKnn && !b.has_d && b.dem16 && k > 0 && ants_per_instance % apb == 0;
-}
-
 hipError_t launch_construct(const DevBatch &b, int selection, int rng, int apb, int k, hipStream_t st)
 {
     if (apb > 0) {
-        const size_t lds = construct_lds_staged_bytes(b.max_n, k, apb);
+        const int rows = construct_staged_rows(b.max_n, k, apb, b.cons_lds_budget);
+        if (rows < 0) return hipErrorInvalidValue;
+        const size_t lds = construct_lds_staged_bytes(b.max_n, k, apb, b.cons_lds_budget);
         const dim3 grid(b.ants / apb), block(kLanes * apb);
-#define CKL(R, A) hipLaunchKernelGGL((k_construct_knn_lds<R, A>), grid, block, lds, st, b)
+#define CKL(R, A) hipLaunchKernelGGL((k_construct_knn_lds<R, A>), grid, block, lds, st, b, rows)
         if (rng == kRngPhilox) {
             if (apb == 1) CKL(kRngPhilox, 1); else if (apb == 2) CKL(kRngPhilox, 2); else CKL(kRngPhilox, 4);
         } else {

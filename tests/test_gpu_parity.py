@@ -90,6 +90,44 @@ def test_construct_knn_matches_oracle(name, rng):
     assert fb > 0  # the full-row fallback was exercised
 
 
+@pytest.mark.parametrize("apb", [1, 2, 4])
+@pytest.mark.parametrize("rng", [ca.RNG_PHILOX, ca.RNG_PARK_MILLER])
+def test_partially_staged_construction_matches_oracle(apb, rng):
+    """1001 nodes with k = 32: the candidate table (256 KB) does not fit in
+    LDS, so the staged kernel holds its first rows there and reads the rest
+    from the global candidate arrays; batched with a 101-node instance whose
+    whole table fits.  Every ant equals the oracle's."""
+    insts = [I.mixed_1000(0), I.uniform_100(6)]
+    k, m = 32, 8
+    log(f"partially staged construction apb={apb} rng={rng}")
+    with ca.GpuACO(insts, n_ants=m, k=k, seed=9, selection=ca.SEL_KNN, rng=rng,
+                   tau0_rule=ca.TAU0_NN_TOUR, use_graph=False) as g:
+        g.init_aco()     # nearest-neighbour tau0: no colony, lane states still the seeds
+        g.set_launch_shape(4, 16, 1, apb, 0)
+        Ps, tis = [], []
+        for b, inst in enumerate(insts):
+            P = oracle_problem(inst, k=k)
+            tau = random_tau(P.n, 40 + b, lo=1e-6, hi=1e-3)
+            ti = (tau * P.heur).astype(np.float32)
+            g.set_pheromone(tau, ti, b)
+            Ps.append(P)
+            tis.append(ti)
+        it = g.iteration
+        g.construct_solutions()
+        got = [g.ants(b) for b in range(len(insts))]
+    fbs = 0
+    for b, inst in enumerate(insts):
+        tours, sizes, lens = got[b]
+        pm0 = np.array([[oracle.pm_seed(9 + b, a, l) for l in range(64)] for a in range(m)],
+                       np.int32)
+        otours, olens, _, fb = oracle_construct_all(Ps[b], tis[b], 1, rng, 9 + b, it, pm0)
+        fbs += fb
+        for a, t in enumerate(otours):
+            assert sizes[a] == len(t), (b, a, sizes[a], len(t))
+            np.testing.assert_array_equal(tours[a, : sizes[a]], t, err_msg=f"instance {b} ant {a}")
+            assert lens[a] == olens[a]
+
+
 def test_construct_knn_widest_row_feasible():
     """k = 64 (the widest candidate row one wavefront holds)."""
     inst = I.uniform_100(2)
