@@ -153,9 +153,26 @@ __device__ __forceinline__ bool feasible_mask(const DevProblem &p, const AntLds 
 // nodes are loaded together, one memory latency per pass.
 constexpr int kGreedyChunk = 16;
 
-template <bool HAS_D>
+// The first pass's row values, loaded by the caller ahead of time (at the
+// start of every step of the unstaged kernel, so a step that falls back
+// finds them already in flight).
+struct GreedyPre {
+    float tv[kGreedyChunk];
+};
+
+__device__ __forceinline__ void greedy_preload(const DevProblem &p, const float *tirow, int lane, GreedyPre &g)
+{
+#pragma unroll
+    for (int u = 0; u < kGreedyChunk; ++u) {
+        const int i = u * kLanes + lane;
+        g.tv[u] = gload(tirow + (i < p.n ? i : p.n - 1));
+    }
+}
+
+template <bool HAS_D, bool PRE = false>
 __device__ int greedy_pick(const DevProblem &p, const AntLds &L, const float *tirow,
-                           const float *drow, int load, float rdist, int lane)
+                           const float *drow, int load, float rdist, int lane,
+                           const GreedyPre *pre = nullptr)
 {
     u64 best = 0;
     for (int base = 0; base < p.n; base += kGreedyChunk * kLanes) {
@@ -175,7 +192,7 @@ __device__ int greedy_pick(const DevProblem &p, const AntLds &L, const float *ti
         for (int u = 0; u < kGreedyChunk; ++u) {
             const int i = base + u * kLanes + lane;
             const int ic = i < p.n ? i : p.n - 1;
-            tv[u] = gload(tirow + ic);
+            tv[u] = PRE && base == 0 ? pre->tv[u] : gload(tirow + ic);
             dv[u] = HAS_D ? gload(drow + ic) : 0.0f;
         }
 #pragma unroll
@@ -322,7 +339,7 @@ __global__ __launch_bounds__(64) void k_construct_iroulette(DevBatch bt)
 // demands and depot distances and this ant's visited flags (depot set).
 // The step loop synchronises the wave only, so the waves of a block may
 // build different ants.
-template <bool HAS_D, int RNG>
+template <bool HAS_D, int RNG, bool SPEC = false>
 __device__ void construct_knn_ant(const DevProblem &p, const DevState &s, int a, const AntLds &L, int lane)
 {
     const int n = p.n, k = p.k;
@@ -344,6 +361,9 @@ __device__ void construct_knn_ant(const DevProblem &p, const DevState &s, int a,
         const int j = gload(p.nnk + e);
         const float tj = gload(s.tiknn + e);
         const float dcj = HAS_D ? gload(p.nnkd + e) : 0.0f;
+        // the row's values for a fallback, requested with the candidates
+        GreedyPre pre;
+        if (SPEC) greedy_preload(p, s.tinfo + (size_t)cur * p.P, lane, pre);
         const int dj = L.dem[j];   // the chosen lane's value is read out below
         const bool ok = (lane < k) & feasible_mask<HAS_D>(p, L, j, dj, dcj, load, rdist);
         const float pj = tj * (ok ? 1.0f : 0.0f);
@@ -367,7 +387,7 @@ __device__ void construct_knn_ant(const DevProblem &p, const DevState &s, int a,
             dcw = HAS_D ? lane_f32(dcj, wc) : 0.0f;
         } else {
             const float *drow = p.dist + (size_t)cur * p.P;
-            w = greedy_pick<HAS_D>(p, L, s.tinfo + (size_t)cur * p.P, drow, load, rdist, lane);
+            w = greedy_pick<HAS_D, SPEC>(p, L, s.tinfo + (size_t)cur * p.P, drow, load, rdist, lane, &pre);
             fb++;
             w = uniform(w);
             // scalar reads keep their waits on this path (see the staged
@@ -421,7 +441,7 @@ __global__ __launch_bounds__(64) void k_construct_knn(DevBatch bt)
         L.vis[i] = i == 0;
     }
     __syncthreads();
-    construct_knn_ant<HAS_D, RNG>(p, s, a, L, lane);
+    construct_knn_ant<HAS_D, RNG, true>(p, s, a, L, lane);
 }
 
 // The same roulette with the instance's candidate table staged in LDS, for
@@ -491,6 +511,8 @@ __global__ __launch_bounds__(64 * A) void k_construct_knn_lds(DevBatch bt, int r
         if (RNG == kRngPhilox && ((step - 1) & 63) == 0) ublk = philox_block(seed, iter, (u32)a, (u32)step, lane);
         int j, dj;
         float tj;
+        GreedyPre pre;
+        greedy_preload(p, s.tinfo + (size_t)cur * p.P, lane, pre);
         if (cur < rows) {
             const uint2 en = ent[cur * k + c];
             j = (int)(en.y & 0xFFFFu);
@@ -523,7 +545,7 @@ __global__ __launch_bounds__(64 * A) void k_construct_knn_lds(DevBatch bt, int r
             w = lane_i32(j, wc);
             dw = lane_i32(dj, wc);
         } else {
-            w = greedy_pick<false>(p, L, s.tinfo + (size_t)cur * p.P, nullptr, load, 0.0f, lane);
+            w = greedy_pick<false, true>(p, L, s.tinfo + (size_t)cur * p.P, nullptr, load, 0.0f, lane, &pre);
             fb++;
             w = uniform(w);
             // scalar, so the wait for this read stays on this path: a vector
