@@ -345,7 +345,7 @@ def leg_clustered(ca, instances, args):
                            rng=ca.RNG_PHILOX, ls_flags=CLUSTERED["ls"], use_graph=True)
     g = mk()
     g.init_aco()
-    g.autotune(reps=3)
+    tune = g.autotune(reps=3)
     shape = g.launch_shape()
     g.run(args.warmup)
     g.synchronize()
@@ -361,7 +361,8 @@ def leg_clustered(ca, instances, args):
                        "Q for 30 routes; 480 ants, k=30 Philox roulette, 2-opt+swap+relocate",
            "ant_solutions_per_s": round(m * args.clustered_steps / el, 1),
            "ms_per_iteration": round(1e3 * el / args.clustered_steps, 4),
-           "launch_shape": shape, "kernels": per}
+           "launch_shape": shape, "kernels": per,
+           "autotune_ms": {f"{STAGES[st]}{sh}": round(ms, 3) for sh, st, ms in tune}}
     if args.full_iterations > 0:
         g = mk()
         g.set_launch_shape(*shape)

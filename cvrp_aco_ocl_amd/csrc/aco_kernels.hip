@@ -511,8 +511,6 @@ __global__ __launch_bounds__(64 * A) void k_construct_knn_lds(DevBatch bt, int r
         if (RNG == kRngPhilox && ((step - 1) & 63) == 0) ublk = philox_block(seed, iter, (u32)a, (u32)step, lane);
         int j, dj;
         float tj;
-        GreedyPre pre;
-        greedy_preload(p, s.tinfo + (size_t)cur * p.P, lane, pre);
         if (cur < rows) {
             const uint2 en = ent[cur * k + c];
             j = (int)(en.y & 0xFFFFu);
@@ -545,7 +543,7 @@ __global__ __launch_bounds__(64 * A) void k_construct_knn_lds(DevBatch bt, int r
             w = lane_i32(j, wc);
             dw = lane_i32(dj, wc);
         } else {
-            w = greedy_pick<false, true>(p, L, s.tinfo + (size_t)cur * p.P, nullptr, load, 0.0f, lane, &pre);
+            w = greedy_pick<false>(p, L, s.tinfo + (size_t)cur * p.P, nullptr, load, 0.0f, lane);
             fb++;
             w = uniform(w);
             // scalar, so the wait for this read stays on this path: a vector
