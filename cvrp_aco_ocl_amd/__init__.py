@@ -8,7 +8,7 @@ is the Python side of that boundary: ``GpuACO`` mirrors ``g_ACO``,
 ``Instance`` the reference's ``Problem``.
 """
 from ._lib import (  # noqa: F401
-    DIST_ATT, DIST_CEIL, DIST_FLOAT, DIST_GEO, DIST_NINT, LS_2OPT, LS_RELOCATE, LS_SWAP,
+    DIST_ATT, DIST_CEIL, DIST_FLOAT, DIST_GEO, DIST_NINT, LS_2OPT, LS_2OPT_RANDOM, LS_RELOCATE, LS_SWAP,
     RNG_PARK_MILLER, RNG_PHILOX, SEL_IROULETTE, SEL_KNN, TAU0_FIRST_COLONY, TAU0_NN_TOUR,
     AcoError, LIB_PATH,
 )

@@ -20,7 +20,7 @@ ERRORS = {-1: "ACO_ERR_ARG", -2: "ACO_ERR_HIP", -3: "ACO_ERR_STATE", -4: "ACO_ER
 
 SEL_IROULETTE, SEL_KNN = 0, 1
 RNG_PARK_MILLER, RNG_PHILOX = 0, 1
-LS_2OPT, LS_SWAP, LS_RELOCATE = 1, 2, 4
+LS_2OPT, LS_SWAP, LS_RELOCATE, LS_2OPT_RANDOM = 1, 2, 4, 8
 TAU0_FIRST_COLONY, TAU0_NN_TOUR = 0, 1
 DIST_FLOAT, DIST_NINT, DIST_CEIL, DIST_GEO, DIST_ATT = 0, 1, 2, 3, 4
 
@@ -112,6 +112,9 @@ _SIGNATURES = {
     "aco_get_pheromone": (C.c_int, [_P, _P, _P]),
     "aco_set_pheromone": (C.c_int, [_P, _P, _P]),
     "aco_get_rng_state": (C.c_int, [_P, _P]),
+    "aco_set_rng_state": (C.c_int, [_P, _P]),
+    "aco_get_ls_seed": (C.c_int, [_P, C.POINTER(C.c_int32)]),
+    "aco_set_ls_seed": (C.c_int, [_P, C.c_int32]),
     "aco_get_counters": (C.c_int, [_P, C.POINTER(AcoCounters)]),
     "aco_profile_enable": (C.c_int, [_P, C.c_int32]),
     "aco_profile_get": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),

@@ -37,6 +37,14 @@ static uint64_t splitmix64(uint64_t x)
 
 // Lane seed: 1 + splitmix64(seed + golden * (1 + 64*ant + lane)) mod (2^31-2).
 // (The reference derives lane seeds from time(NULL), g_aco.cpp:715-719.)
+// Seed of the random-order 2-opt's scan-order stream (the reference draws
+// it from instance->rnd_seed, problem.cpp:213; here it follows the colony
+// seed).
+static int32_t ls_seed_of(uint64_t seed)
+{
+    return (int32_t)(1 + splitmix64(seed ^ 0xC2B2AE3D27D4EB4Full) % 2147483646ull);
+}
+
 static int32_t pm_seed(uint64_t seed, int ant, int lane)
 {
     const uint64_t z = splitmix64(seed + 0x9E3779B97F4A7C15ull *
@@ -63,6 +71,7 @@ struct InstanceData {
     int Q = 0;
     float D = INFINITY, svc = 0.0f;
     uint64_t seed = 0;
+    int32_t ls_seed0 = 1;     // initial state of the scan-order stream
     DevProblem dp{};
     DevState ds{};
     void *arena = nullptr;
@@ -338,7 +347,9 @@ static int check_params(aco_params &prm, int min_n)
         return set_error(ACO_ERR_ARG, "k must be in [1, min(64, n-2)] for roulette_knn");
     if (prm.selection != ACO_SEL_KNN) prm.k = 0;
     if (prm.nn_ls < 0 || prm.nn_ls > min_n - 2) return set_error(ACO_ERR_ARG, "nn_ls must be <= n-2");
-    if (prm.ls_flags & ~7) return set_error(ACO_ERR_ARG, "unknown local search flag");
+    if (prm.ls_flags & ~15) return set_error(ACO_ERR_ARG, "unknown local search flag");
+    if ((prm.ls_flags & ACO_LS_2OPT) && (prm.ls_flags & ACO_LS_2OPT_RANDOM))
+        return set_error(ACO_ERR_ARG, "ACO_LS_2OPT and ACO_LS_2OPT_RANDOM are two orders of one 2-opt: choose one");
     if (prm.tau0_rule != ACO_TAU0_FIRST_COLONY && prm.tau0_rule != ACO_TAU0_NN_TOUR)
         return set_error(ACO_ERR_ARG, "unknown tau0 rule");
     return ACO_OK;
@@ -542,6 +553,7 @@ int aco_create_batch(const aco_problem *pbs, int32_t count, const aco_params *pr
         I.D = pb->max_dist;
         I.svc = pb->service_time;
         I.seed = prm.seed + (uint64_t)b;
+        I.ls_seed0 = ls_seed_of(I.seed);
         const int P = I.P;
         const size_t bytes = carve_instance(I, m, k, nnP, ras, prm.bsf_capacity, nullptr);
         CHK(hipMalloc(&I.arena, bytes));
@@ -739,6 +751,7 @@ int aco_init(aco_ctx *c)
             for (int l = 0; l < kLanes; ++l) pm[(size_t)a * kLanes + l] = pm_seed(I.seed, a, l);
         HIP_TRY(hipMemcpyAsync(I.ds.pm, pm.data(), pm.size() * 4, hipMemcpyHostToDevice, c->stream));
         HIP_TRY(hipMemcpyAsync(I.ds.ctrl, zeros, sizeof(zeros), hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(hipMemcpyAsync(I.ds.ctrl + kCtrlLsSeed, &c->inst[b].ls_seed0, 4, hipMemcpyHostToDevice, c->stream));
         HIP_TRY(hipMemcpyAsync(I.ds.totals, zeros64, sizeof(zeros64), hipMemcpyHostToDevice, c->stream));
         HIP_TRY(hipMemcpyAsync(I.ds.lens + m, &inf, 4, hipMemcpyHostToDevice, c->stream));
         HIP_TRY(hipMemsetAsync(I.ds.sizes + m, 0, 4, c->stream));
@@ -984,6 +997,40 @@ int aco_get_rng_state(aco_ctx *c, int32_t *state)
     if (rc) return rc;
     if (!state) return set_error(ACO_ERR_ARG, "null output");
     HIP_TRY(hipMemcpyAsync(state, c->inst[0].ds.pm, (size_t)c->m * kLanes * 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return ACO_OK;
+}
+
+int aco_set_rng_state(aco_ctx *c, const int32_t *state)
+{
+    int rc = need(c, false);
+    if (rc) return rc;
+    if (!state) return set_error(ACO_ERR_ARG, "null state");
+    const size_t cnt = (size_t)c->m * kLanes;
+    for (size_t q = 0; q < cnt; ++q)
+        if (state[q] < 1 || state[q] > 2147483646)
+            return set_error(ACO_ERR_ARG, "Park-Miller states must lie in [1, 2^31 - 2]");
+    HIP_TRY(hipMemcpyAsync(c->inst[0].ds.pm, state, cnt * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return ACO_OK;
+}
+
+int aco_get_ls_seed(aco_ctx *c, int32_t *seed)
+{
+    int rc = need(c, true);
+    if (rc) return rc;
+    if (!seed) return set_error(ACO_ERR_ARG, "null output");
+    HIP_TRY(hipMemcpyAsync(seed, c->inst[0].ds.ctrl + kCtrlLsSeed, 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return ACO_OK;
+}
+
+int aco_set_ls_seed(aco_ctx *c, int32_t seed)
+{
+    int rc = need(c, true);
+    if (rc) return rc;
+    if (seed < 1 || seed > 2147483646) return set_error(ACO_ERR_ARG, "the seed must lie in [1, 2^31 - 2]");
+    HIP_TRY(hipMemcpyAsync(c->inst[0].ds.ctrl + kCtrlLsSeed, &seed, 4, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return ACO_OK;
 }

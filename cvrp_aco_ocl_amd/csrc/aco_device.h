@@ -115,11 +115,12 @@ constexpr int kMaxMovesFactor = 64;    // swap/relocate cap = 64*n per call
 
 enum { kSelIRoulette = 0, kSelKnn = 1 };
 enum { kRngParkMiller = 0, kRngPhilox = 1 };
-enum { kLs2Opt = 1, kLsSwap = 2, kLsRelocate = 4 };
+enum { kLs2Opt = 1, kLsSwap = 2, kLsRelocate = 4, kLs2OptRandom = 8 };
 
 // control words (DevState::ctrl)
 enum { kCtrlIter = 0, kCtrlNbsf = 1, kCtrlIterBest = 2, kCtrlFlag = 3, kCtrlOverflow = 4,
        kCtrlDone = 5,   // fused iteration: ants of the instance finished this launch
+       kCtrlLsSeed = 6, // Park-Miller state of the random-order 2-opt's scan-order stream
        kCtrlWords = 8 };
 // running totals (DevState::totals)
 enum { kTotEvals = 0, kTotMoves = 1, kTotFallbacks = 2, kTotIters = 3, kTotWords = 4 };
@@ -230,6 +231,18 @@ struct LsStreams {
 };
 hipError_t launch_local_search(const DevBatch &b, int flags, const LsShape &sh, hipStream_t st,
                                const LsStreams *par = nullptr);
+// Park-Miller state after t draws from z (16807^t z mod 2^31-1).
+__host__ __device__ inline int pm_jump(int z, unsigned long long t)
+{
+    const unsigned long long M = 2147483647ull;
+    unsigned long long a = 16807ull, r = 1ull;
+    while (t) {
+        if (t & 1ull) r = r * a % M;
+        a = a * a % M;
+        t >>= 1;
+    }
+    return (int)(r * (unsigned long long)(unsigned)z % M);
+}
 // compute_nn_lists for `count` instances on the device (host arrays in and
 // out, one thread per row, the reference's sort steps); ACO_OK or an error.
 int gpu_nn_lists(hipStream_t st, int count, const int *n, const float *const *dist, const int *width,

@@ -1045,7 +1045,8 @@ __global__ __launch_bounds__(W * 64, W == 4 ? 3 : 4) void k_iteration_fused(DevB
 
 bool fused_iteration_ok(const DevBatch &b, int selection, int ls_flags, int ants_per_instance, const LsShape &sh)
 {
-    return selection == kSelKnn && !b.has_d && ls_flags != 0 && (sh.waves == 4 || sh.waves == 8) &&
+    return selection == kSelKnn && !b.has_d && ls_flags != 0 && !(ls_flags & kLs2OptRandom) &&
+           (sh.waves == 4 || sh.waves == 8) &&
            ants_per_instance % sh.waves == 0 && ls_shape_valid(sh);
 }
 

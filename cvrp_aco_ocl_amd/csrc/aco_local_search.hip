@@ -40,6 +40,23 @@ hipError_t local_search_static_lds(size_t &bytes)
     return hipSuccess;
 }
 
+// After a local search with the random-order 2-opt: each instance's
+// scan-order stream advances past the draws its ants made (size - 1 each).
+__global__ __launch_bounds__(256) void k_ls_seed_advance(DevBatch bt)
+{
+    __shared__ long long red[4];
+    const DevProblem p = bt.probs[blockIdx.x];
+    const DevState s = bt.states[blockIdx.x];
+    long long t = 0;
+    for (int a = threadIdx.x; a < p.m; a += 256) t += s.sizes[a] - 1;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = t;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        s.ctrl[kCtrlLsSeed] = pm_jump(s.ctrl[kCtrlLsSeed], (unsigned long long)(red[0] + red[1] + red[2] + red[3]));
+}
+
 bool ls_shape_valid(const LsShape &sh)
 {
     if (sh.waves != 4 && sh.waves != 8 && sh.waves != 12 && sh.waves != 16) return false;
@@ -87,6 +104,10 @@ hipError_t launch_local_search(const DevBatch &b, int flags, const LsShape &sh, 
             if (e == hipSuccess) e = hipStreamWaitEvent(st, par->join[cl - 1], 0);
             if (e != hipSuccess) return e;
         }
+    }
+    if (flags & kLs2OptRandom) {
+        hipLaunchKernelGGL(k_ls_seed_advance, dim3(b.B), dim3(256), 0, st, b);
+        return hipGetLastError();
     }
     return hipSuccess;
 }

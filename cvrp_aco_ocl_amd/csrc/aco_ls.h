@@ -50,14 +50,14 @@ template <int W> struct LsW {
 struct LsLds {
     int *tour, *rl, *dem, *res;
     float *dtw, *rd, *rows;
-    short *pos, *rof, *ridx, *zpos, *rc;
+    short *pos, *rof, *ridx, *zpos, *rc, *perm;
     unsigned char *dlb;
 };
 
 __host__ __device__ constexpr size_t ls_align16(size_t x) { return (x + 15) & ~size_t(15); }
 
 struct LsOffsets {
-    size_t tour, dtw, rl, rd, dem, rows, res, pos, rof, ridx, zpos, rc, dlb, total;
+    size_t tour, dtw, rl, rd, dem, rows, res, pos, rof, ridx, zpos, rc, perm, dlb, total;
 };
 
 __host__ __device__ inline LsOffsets ls_offsets(int n, int ring, int rowbytes)
@@ -78,6 +78,7 @@ __host__ __device__ inline LsOffsets ls_offsets(int n, int ring, int rowbytes)
     o.ridx = at; at = ls_align16(at + 2 * (T + kScanPad));
     o.zpos = at; at = ls_align16(at + 2 * (N + 1));
     o.rc = at;   at = ls_align16(at + 2 * N);
+    o.perm = at; at = ls_align16(at + 2 * T);
     o.dlb = at;  at = ls_align16(at + N);
     o.total = at;
     return o;
@@ -91,7 +92,8 @@ __device__ __forceinline__ LsLds ls_bind(int n, int ring, int rowbytes, unsigned
     L.rl = (int *)(b + o.rl); L.rd = (float *)(b + o.rd); L.dem = (int *)(b + o.dem);
     L.rows = (float *)(b + o.rows); L.res = (int *)(b + o.res);
     L.pos = (short *)(b + o.pos); L.rof = (short *)(b + o.rof); L.ridx = (short *)(b + o.ridx);
-    L.zpos = (short *)(b + o.zpos); L.rc = (short *)(b + o.rc); L.dlb = b + o.dlb;
+    L.zpos = (short *)(b + o.zpos); L.rc = (short *)(b + o.rc); L.perm = (short *)(b + o.perm);
+    L.dlb = b + o.dlb;
     return L;
 }
 
@@ -304,8 +306,40 @@ __device__ __forceinline__ int two_opt_scans(const DevProblem &p, const LsLds &L
     return kind;
 }
 
+// Scan order of the CPU local search (generate_random_permutation,
+// localSearch.cpp:97-119): offsets 0..num-1 of the route's positions,
+// shuffled with one draw each from the Park-Miller state z; written by one
+// lane to perm[rbeg .. rbeg+num).  A draw of exactly 1.0 (state >=
+// 2147483584) would index past the end, where the reference asserts; it
+// takes the last offset instead.
+__device__ __forceinline__ void route_permutation(LsLds &L, int rbeg, int num, int z)
+{
+    short *r = L.perm + rbeg;
+    for (int i = 0; i < num; ++i) r[i] = (short)i;
+    for (int i = 0; i < num; ++i) {
+        int k = z / 127773;
+        z = 16807 * (z - k * 127773) - 2836 * k;
+        if (z < 0) z += 2147483647;
+        const float rnd = (float)z * 0x1p-31f;
+        int node = (int)(rnd * (float)(num - i));
+        if (i + node >= num) node = num - 1 - i;
+        const short h = r[i];
+        r[i] = r[i + node];
+        r[i + node] = h;
+    }
+}
+
+// Position of the l-th scan step: in sequence, or the random order.
+template <bool RANDOM>
+__device__ __forceinline__ int scan_pos(const LsLds &L, int rbeg, int l)
+{
+    return RANDOM ? rbeg + L.perm[rbeg + l] : rbeg + l;
+}
+
 // kernel.cl:126-277 for route r at positions [rbeg, rend] (rbeg holds the
-// depot), run by the kOptGroup lanes gl = 0.. of one group.
+// depot), run by the kOptGroup lanes gl = 0.. of one group; RANDOM scans
+// the positions in the order of perm (localSearch.cpp:180-294).
+template <bool RANDOM>
 __device__ __forceinline__ void two_opt_route(const DevProblem &p, LsLds &L, int r, int rbeg,
                                               int rend, int gl, long long &evals, int &moves_total)
 {
@@ -316,7 +350,7 @@ __device__ __forceinline__ void two_opt_route(const DevProblem &p, LsLds &L, int
     while (imp) {
         imp = false;
         for (int l = 0; l < num; ++l) {
-            const int pos1 = rbeg + l;
+            const int pos1 = scan_pos<RANDOM>(L, rbeg, l);
             const int n1 = L.tour[pos1];
             if (n1 == 0 ? dlb0 : (L.dlb[n1] != 0)) continue;
             int h1 = 0, h2 = 0, h3 = 0, h4 = 0;
@@ -388,9 +422,10 @@ __device__ __forceinline__ void two_opt_route(const DevProblem &p, LsLds &L, int
 // sequential search would see.  After a move at the first position the
 // search continues right after it and scans the second again.  Halves
 // exchange their results with one DPP swap (16-lane groups are DPP rows).
-__device__ __forceinline__ bool two_opt_skip(const LsLds &L, int pos, bool dlb0)
+template <bool RANDOM>
+__device__ __forceinline__ bool two_opt_skip(const LsLds &L, int rbeg, int l, bool dlb0)
 {
-    const int v = L.tour[pos];
+    const int v = L.tour[scan_pos<RANDOM>(L, rbeg, l)];
     return v == 0 ? dlb0 : (L.dlb[v] != 0);
 }
 
@@ -400,6 +435,7 @@ __device__ __forceinline__ u64 two_opt_pack(int kind, int h1, int h2, int h3, in
     return ((u64)kind << 52) | ((u64)h1 << 39) | ((u64)h2 << 26) | ((u64)h3 << 13) | (u64)h4;
 }
 
+template <bool RANDOM>
 __device__ __forceinline__ void two_opt_route_pair(const DevProblem &p, LsLds &L, int r, int rbeg, int rend,
                                                    int gl16, long long &evals, int &moves_total)
 {
@@ -414,16 +450,17 @@ __device__ __forceinline__ void two_opt_route_pair(const DevProblem &p, LsLds &L
         int l = 0;
         for (;;) {
             int la = l;
-            while (la < num && two_opt_skip(L, rbeg + la, dlb0)) ++la;
+            while (la < num && two_opt_skip<RANDOM>(L, rbeg, la, dlb0)) ++la;
             if (la >= num) break;
             int lb = la + 1;
-            while (lb < num && two_opt_skip(L, rbeg + lb, dlb0)) ++lb;
-            const int na = L.tour[rbeg + la], nb = lb < num ? L.tour[rbeg + lb] : -1;
+            while (lb < num && two_opt_skip<RANDOM>(L, rbeg, lb, dlb0)) ++lb;
+            const int na = L.tour[scan_pos<RANDOM>(L, rbeg, la)];
+            const int nb = lb < num ? L.tour[scan_pos<RANDOM>(L, rbeg, lb)] : -1;
             const int lm = half == 0 ? la : lb;
             int kind = 0, h1 = 0, h2 = 0, h3 = 0, h4 = 0;
             long long ev = 0;
             if (lm < num) {
-                const int pos1 = rbeg + lm;
+                const int pos1 = scan_pos<RANDOM>(L, rbeg, lm);
                 const int n1 = L.tour[pos1];
                 const int s1 = pos1 == rend ? 0 : L.tour[pos1 + 1];
                 const int p1 = pos1 == rbeg ? L.tour[rend] : L.tour[pos1 - 1];
@@ -1150,11 +1187,47 @@ __device__ void local_search_ant(const DevProblem &p, const DevState &s, int a, 
         if (R <= NT / (2 * kOptGroup)) {
             // a group of 16 lanes per route: two positions at a time
             const int grp = tid / (2 * kOptGroup), gl = tid % (2 * kOptGroup);
-            if (grp < R) two_opt_route_pair(p, L, grp, L.zpos[grp], L.zpos[grp + 1] - 1, gl, ev, moves);
+            if (grp < R) two_opt_route_pair<false>(p, L, grp, L.zpos[grp], L.zpos[grp + 1] - 1, gl, ev, moves);
         } else {
             const int grp = tid / kOptGroup, gl = tid % kOptGroup;
             for (int r = grp; r < R; r += NT / kOptGroup)
-                two_opt_route(p, L, r, L.zpos[r], L.zpos[r + 1] - 1, gl, ev, moves);
+                two_opt_route<false>(p, L, r, L.zpos[r], L.zpos[r + 1] - 1, gl, ev, moves);
+        }
+        __syncthreads();
+    }
+    if (flags & kLs2OptRandom) {
+        // CPU variant's scan order: one Park-Miller stream per instance,
+        // consumed ant after ant and route after route (one draw per route
+        // position, size - 1 per ant) as sequential calls would; ant a
+        // starts after the draws of ants 0..a-1, route r after those of the
+        // routes before it (zpos[r] positions), reached by jump-ahead
+        __shared__ long long red_off[W];
+        long long off = 0;
+        for (int b = tid; b < a; b += NT) off += s.sizes[b] - 1;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) off += __shfl_xor(off, o);
+        if ((tid & 63) == 0) red_off[tid >> 6] = off;
+        ls_edges<W>(p, L, 0, S - 2, tid);
+        const int R = ls_routes<W>(p, L, S, tid, false);   // its barriers publish red_off
+        off = 0;
+        for (int q = 0; q < W; ++q) off += red_off[q];
+        const int zant = pm_jump(s.ctrl[kCtrlLsSeed], (unsigned long long)off);
+        if (R <= NT / (2 * kOptGroup)) {
+            const int grp = tid / (2 * kOptGroup), gl = tid % (2 * kOptGroup);
+            if (grp < R) {
+                const int rbeg = L.zpos[grp], rend = L.zpos[grp + 1] - 1;
+                if (gl == 0) route_permutation(L, rbeg, rend - rbeg + 1, pm_jump(zant, (unsigned long long)rbeg));
+                group_sync();
+                two_opt_route_pair<true>(p, L, grp, rbeg, rend, gl, ev, moves);
+            }
+        } else {
+            const int grp = tid / kOptGroup, gl = tid % kOptGroup;
+            for (int r = grp; r < R; r += NT / kOptGroup) {
+                const int rbeg = L.zpos[r], rend = L.zpos[r + 1] - 1;
+                if (gl == 0) route_permutation(L, rbeg, rend - rbeg + 1, pm_jump(zant, (unsigned long long)rbeg));
+                group_sync();
+                two_opt_route<true>(p, L, r, rbeg, rend, gl, ev, moves);
+            }
         }
         __syncthreads();
     }

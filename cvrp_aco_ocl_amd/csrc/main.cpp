@@ -27,7 +27,9 @@ static void usage(const char *prog)
             "  --ras R                           ranked ants incl. best-so-far (6)\n"
             "  --selection iroulette|knn --k K   next-node rule and candidate width\n"
             "  --rng pm|philox                   random numbers\n"
-            "  --ls none|2opt|2opt+swap|full     local search (none)\n"
+            "  --ls none|2opt|2opt-random|2opt+swap|full|full-random\n"
+            "                                    local search (none); -random: 2-opt in the CPU\n"
+            "                                    variant's random scan order\n"
             "  --nn-ls D                         2-opt neighbour depth (nodes-2)\n"
             "  --tau0 first|nn                   initial trail rule (first)\n"
             "  --dist float|nint                 EUC_2D distances (float, as the reference)\n"
@@ -114,8 +116,10 @@ int main(int argc, char **argv)
         } else if (s == "--ls") {
             const std::string v = val();
             defaults.ls_flags = v == "none" ? 0 : v == "2opt" ? ACO_LS_2OPT
+                              : v == "2opt-random" ? ACO_LS_2OPT_RANDOM
                               : v == "2opt+swap" ? (ACO_LS_2OPT | ACO_LS_SWAP)
-                              : v == "full" ? (ACO_LS_2OPT | ACO_LS_SWAP | ACO_LS_RELOCATE) : -1;
+                              : v == "full" ? (ACO_LS_2OPT | ACO_LS_SWAP | ACO_LS_RELOCATE)
+                              : v == "full-random" ? (ACO_LS_2OPT_RANDOM | ACO_LS_SWAP | ACO_LS_RELOCATE) : -1;
             if (defaults.ls_flags < 0) usage(argv[0]);
         } else if (s == "--tau0") {
             defaults.tau0_rule = std::string(val()) == "nn" ? ACO_TAU0_NN_TOUR : ACO_TAU0_FIRST_COLONY;

@@ -289,6 +289,18 @@ class GpuACO:
         check(lib.aco_get_rng_state(self.h, _ptr(out)), "aco_get_rng_state")
         return out
 
+    def set_rng_state(self, state):
+        st = np.ascontiguousarray(state, np.int32)
+        check(lib.aco_set_rng_state(self.h, _ptr(st)), "aco_set_rng_state")
+
+    def ls_seed(self) -> int:
+        z = C.c_int32()
+        check(lib.aco_get_ls_seed(self.h, C.byref(z)), "aco_get_ls_seed")
+        return z.value
+
+    def set_ls_seed(self, seed: int):
+        check(lib.aco_set_ls_seed(self.h, int(seed)), "aco_set_ls_seed")
+
     def counters(self) -> dict:
         c = _lib.AcoCounters()
         check(lib.aco_get_counters(self.h, C.byref(c)), "aco_get_counters")

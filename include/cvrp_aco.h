@@ -40,6 +40,11 @@ extern "C" {
 #define ACO_LS_2OPT          1   /* intra-route 2-opt (kernel.cl:126-319)            */
 #define ACO_LS_SWAP          2   /* swap (localSearch.cpp:301-410)                   */
 #define ACO_LS_RELOCATE      4   /* relocate (neighbourSearch.cpp:227-365)           */
+/* 2-opt in the CPU variant's random scan order (LocalSearch::two_opt_single_route
+ * with generate_random_permutation, localSearch.cpp:97-119,180-294): one
+ * Park-Miller stream per instance, consumed ant after ant and route after
+ * route as sequential calls would; replaces ACO_LS_2OPT (not both). */
+#define ACO_LS_2OPT_RANDOM   8
 /* initial trail */
 #define ACO_TAU0_FIRST_COLONY 0  /* 0.5, one colony, then 1/(rho*L) (g_aco.cpp:79-90) */
 #define ACO_TAU0_NN_TOUR      1  /* 1/(n*L_nn) of a nearest-neighbour tour           */
@@ -157,8 +162,15 @@ int aco_get_bsf(aco_ctx *ctx, aco_bsf_record *records, int32_t capacity, int32_t
 /* n*n pheromone and total_info, row-major */
 int aco_get_pheromone(aco_ctx *ctx, float *tau, float *total_info);
 int aco_set_pheromone(aco_ctx *ctx, const float *tau, const float *total_info);
-/* m*64 Park-Miller lane states */
+/* m*64 Park-Miller lane states of instance 0 (the per-ant draw streams,
+ * kernel.cl:42-52); set replaces them (each in [1, 2^31-2]), e.g. to replay
+ * a recorded state or force a particular draw */
 int aco_get_rng_state(aco_ctx *ctx, int32_t *state);
+int aco_set_rng_state(aco_ctx *ctx, const int32_t *state);
+/* state of instance 0's scan-order stream for ACO_LS_2OPT_RANDOM (the
+ * reference's instance->rnd_seed, utilities.cpp:271-288); after aco_init */
+int aco_get_ls_seed(aco_ctx *ctx, int32_t *seed);
+int aco_set_ls_seed(aco_ctx *ctx, int32_t seed);
 int aco_get_counters(aco_ctx *ctx, aco_counters *counters);
 /* per-step device time: enable = 1 records a HIP event pair around each
  * launch (iterations then run without the graph) */

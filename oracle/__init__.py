@@ -55,6 +55,8 @@ def _load():
         "orc_ran01": (C.c_float, [C.POINTER(C.c_int32)]),
         "orc_splitmix64": (C.c_uint64, [C.c_uint64]),
         "orc_pm_seed": (C.c_int32, [C.c_uint64, C.c_int, C.c_int]),
+        "orc_ls_seed": (C.c_int32, [C.c_uint64]),
+        "orc_pm_jump": (C.c_int32, [C.c_int32, C.c_uint64]),
         "orc_philox4x32_10": (None, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                      C.POINTER(C.c_uint32)]),
         "orc_philox_u01": (C.c_float, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32]),
@@ -79,6 +81,8 @@ def _load():
                                    C.POINTER(C.c_longlong)]),
         "orc_local_search": (C.c_float, [C.POINTER(_Problem), C.c_int, _i32p, C.c_int,
                                          C.POINTER(C.c_longlong)]),
+        "orc_local_search_seeded": (C.c_float, [C.POINTER(_Problem), C.c_int, _i32p, C.c_int,
+                                                C.POINTER(C.c_longlong), C.POINTER(C.c_int32)]),
         "orc_tour_length": (C.c_float, [C.c_int, _f32p, _i32p, C.c_int]),
         "orc_check_solution": (C.c_int, [C.POINTER(_Problem), _i32p, C.c_int]),
         "orc_nn_tour": (C.c_int, [C.POINTER(_Problem), _i32p, C.POINTER(C.c_float)]),
@@ -99,6 +103,9 @@ def _load():
         "orc_colony_set_ants": (None, [C.c_void_p, _i32p, C.c_int, _i32p, _f32p]),
         "orc_colony_ls_evals": (C.c_longlong, [C.c_void_p]),
         "orc_colony_fallbacks": (C.c_longlong, [C.c_void_p]),
+        "orc_colony_ls_seed": (C.c_int32, [C.c_void_p]),
+        "orc_colony_set_ls_seed": (None, [C.c_void_p, C.c_int32]),
+        "orc_colony_set_pm_state": (None, [C.c_void_p, _i32p]),
         "orc_colony_pm_state": (None, [C.c_void_p, _i32p]),
     }
     for name, (res, args) in sig.items():
@@ -127,6 +134,16 @@ def ran01_stream(seed: int, count: int):
     for i in range(count):
         out[i] = lib.orc_ran01(C.byref(s))
     return out, s.value
+
+
+def ls_seed(seed: int) -> int:
+    """Seed of the random-order 2-opt's scan-order stream."""
+    return int(lib.orc_ls_seed(seed))
+
+
+def pm_jump(z: int, t: int) -> int:
+    """Park-Miller state after t draws from state z."""
+    return int(lib.orc_pm_jump(z, t))
 
 
 def pm_seed(seed: int, ant: int, lane: int) -> int:
@@ -256,6 +273,15 @@ class Problem:
         ev = C.c_longlong(0)
         mv = lib.orc_relocate(C.byref(self.c), t, len(t), C.byref(ev))
         return t, mv, ev.value
+
+    def local_search_seeded(self, tour, flags, seed):
+        """(tour, length, evaluated moves, seed after): flags may hold 8,
+        the random-order 2-opt drawing from the stream at `seed`."""
+        t = np.ascontiguousarray(tour, np.int32).copy()
+        ev = C.c_longlong(0)
+        s = C.c_int32(seed)
+        L = lib.orc_local_search_seeded(C.byref(self.c), flags, t, len(t), C.byref(ev), C.byref(s))
+        return t, L, ev.value, s.value
 
     def local_search(self, tour, flags):
         t = np.ascontiguousarray(tour, np.int32).copy()
@@ -390,6 +416,15 @@ class Colony:
 
     def fallbacks(self):
         return int(lib.orc_colony_fallbacks(self.h))
+
+    def ls_seed(self):
+        return int(lib.orc_colony_ls_seed(self.h))
+
+    def set_ls_seed(self, seed):
+        lib.orc_colony_set_ls_seed(self.h, int(seed))
+
+    def set_pm_state(self, state):
+        lib.orc_colony_set_pm_state(self.h, np.ascontiguousarray(state, np.int32).ravel())
 
     def pm_state(self):
         out = np.empty(self.m * 64, np.int32)

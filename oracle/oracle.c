@@ -72,6 +72,27 @@ int32_t orc_pm_seed(uint64_t seed, int ant, int lane)
     return (int32_t)(1 + z % 2147483646ull);
 }
 
+/* Seed of the local search's scan-order stream (the reference draws it
+ * from instance->rnd_seed, problem.cpp:213; here it is derived from the
+ * colony seed). */
+int32_t orc_ls_seed(uint64_t seed)
+{
+    return (int32_t)(1 + orc_splitmix64(seed ^ 0xC2B2AE3D27D4EB4Full) % 2147483646ull);
+}
+
+/* 16807^t * z mod (2^31 - 1): the Park-Miller state after t draws. */
+int32_t orc_pm_jump(int32_t z, uint64_t t)
+{
+    const uint64_t M = 2147483647ull;
+    uint64_t a = 16807, r = 1;
+    while (t) {
+        if (t & 1) r = r * a % M;
+        a = a * a % M;
+        t >>= 1;
+    }
+    return (int32_t)(r * (uint64_t)z % M);
+}
+
 /* Philox-4x32-10 (Salmon, Moraes, Dror & Shaw, SC'11). */
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
 {
@@ -746,7 +767,16 @@ int orc_relocate(const orc_problem *p, int *tour, int size, long long *evals)
 
 float orc_local_search(const orc_problem *p, int flags, int *tour, int size, long long *evals)
 {
+    return orc_local_search_seeded(p, flags, tour, size, evals, NULL);
+}
+
+/* flags 8: 2-opt in the CPU variant's random scan order, drawing from *seed
+ * (one draw per route position, localSearch.cpp:97-119,127-169) */
+float orc_local_search_seeded(const orc_problem *p, int flags, int *tour, int size, long long *evals,
+                              int32_t *seed)
+{
     if (flags & 1) orc_two_opt(p, tour, size, 0, NULL, evals);
+    if (flags & 8) orc_two_opt(p, tour, size, 1, seed, evals);
     if (flags & 2) orc_swap(p, tour, size, evals);
     if (flags & 4) orc_relocate(p, tour, size, evals);
     return orc_tour_length(p->n, p->dist, tour, size);
@@ -847,6 +877,7 @@ struct orc_colony {
     int nbsf, bsf_cap;
     long long ls_evals;
     long long fallbacks;
+    int32_t ls_seed;        /* scan-order stream of the random-order 2-opt */
     double t0;
 };
 
@@ -873,6 +904,7 @@ orc_colony *orc_colony_create(const orc_problem *p, const orc_params *prm)
     for (int a = 0; a < c->m; ++a)
         for (int l = 0; l < ORC_LANES; ++l) c->pm[(size_t)a * ORC_LANES + l] = orc_pm_seed(prm->seed, a, l);
     c->bsf_cap = 4096;
+    c->ls_seed = orc_ls_seed(prm->seed);
     c->bsf = (orc_bsf_record *)calloc(c->bsf_cap, sizeof(orc_bsf_record));
     c->lens[c->m] = INFINITY;
     return c;
@@ -920,6 +952,16 @@ void orc_colony_construct(orc_colony *c)
 void orc_colony_local_search(orc_colony *c)
 {
     if (!c->prm.ls_flags) return;
+    if (c->prm.ls_flags & 8) {
+        /* one scan-order stream, consumed ant after ant as sequential calls
+         * of the CPU local search would */
+        long long ev = 0;
+        for (int a = 0; a < c->m; ++a)
+            c->lens[a] = orc_local_search_seeded(&c->p, c->prm.ls_flags, c->tours + (size_t)a * c->T,
+                                                 c->sizes[a], &ev, &c->ls_seed);
+        c->ls_evals += ev;
+        return;
+    }
     int nthreads = c->prm.threads > 0 ? c->prm.threads : 1;
     long long ev = 0;
     (void)nthreads;
@@ -1054,6 +1096,12 @@ void orc_colony_set_ants(orc_colony *c, const int *tours, int stride, const int 
 }
 
 long long orc_colony_fallbacks(const orc_colony *c) { return c->fallbacks; }
+int32_t orc_colony_ls_seed(const orc_colony *c) { return c->ls_seed; }
+void orc_colony_set_ls_seed(orc_colony *c, int32_t seed) { c->ls_seed = seed; }
+void orc_colony_set_pm_state(orc_colony *c, const int32_t *state)
+{
+    memcpy(c->pm, state, sizeof(int32_t) * (size_t)c->m * ORC_LANES);
+}
 
 long long orc_colony_ls_evals(const orc_colony *c) { return c->ls_evals; }
 

@@ -34,6 +34,8 @@ float    orc_ran01(int32_t *idum);                        /* utilities.cpp:271-2
 uint64_t orc_splitmix64(uint64_t x);
 int32_t  orc_pm_seed(uint64_t seed, int ant, int lane);
 void     orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+int32_t  orc_ls_seed(uint64_t seed);                      /* scan-order stream seed */
+int32_t  orc_pm_jump(int32_t z, uint64_t t);             /* state after t draws */
 float    orc_philox_u01(uint64_t seed, uint32_t iter, uint32_t ant, uint32_t step);
 
 /* ---------------- instance preprocessing ---------------- */
@@ -110,6 +112,9 @@ int   orc_relocate(const orc_problem *p, int *tour, int size, long long *evals);
 /* ls flags: 1 two_opt, 2 swap, 4 relocate; returns new length           */
 float orc_local_search(const orc_problem *p, int flags, int *tour, int size,
                        long long *evals);
+/* the same with 8: 2-opt in random scan order drawn from *seed           */
+float orc_local_search_seeded(const orc_problem *p, int flags, int *tour, int size,
+                              long long *evals, int32_t *seed);
 
 /* ---------------- solution checks (problem.cpp:226-319) ---------------- */
 float orc_tour_length(int n, const float *dist, const int *tour, int size);
@@ -150,6 +155,11 @@ void  orc_colony_set_ants(orc_colony *c, const int *tours, int stride, const int
 long long orc_colony_ls_evals(const orc_colony *c);
 /* full-row greedy picks of the construction steps so far */
 long long orc_colony_fallbacks(const orc_colony *c);
+/* scan-order stream of the random-order 2-opt (ls flag 8) */
+int32_t orc_colony_ls_seed(const orc_colony *c);
+void  orc_colony_set_ls_seed(orc_colony *c, int32_t seed);
+/* m*64 Park-Miller lane states */
+void  orc_colony_set_pm_state(orc_colony *c, const int32_t *state);
 void  orc_colony_pm_state(const orc_colony *c, int32_t *out);
 
 #ifdef __cplusplus

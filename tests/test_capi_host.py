@@ -142,7 +142,7 @@ def _create(inst, **kw):
 @pytest.mark.parametrize("kw", [
     dict(ras_ranks=0), dict(ras_ranks=17), dict(rho=0.0), dict(rho=1.5),
     dict(selection=1, k=0), dict(selection=1, k=65), dict(selection=0, rng=1),
-    dict(ls_flags=8), dict(tau0_rule=3), dict(nn_ls=200), dict(selection=7),
+    dict(ls_flags=16), dict(ls_flags=9), dict(tau0_rule=3), dict(nn_ls=200), dict(selection=7),
 ])
 def test_create_rejects_bad_params_before_device_work(kw):
     rc, h = _create(I.uniform_100(0), **kw)

@@ -302,6 +302,46 @@ def test_two_opt_and_swap_match_reference_localsearch(which):
     assert moves > 0
 
 
+def test_park_miller_jump_ahead():
+    """orc_pm_jump(z, t) = the state after t draws (16807^t z mod 2^31-1)."""
+    for z in (1, 12345, 2147483646, 987654321):
+        s = C.c_int32(z)
+        for t in range(1, 300):
+            oracle.lib.orc_ran01(s)
+            assert oracle.pm_jump(z, t) == s.value
+    assert oracle.pm_jump(1, 10000) == 1043618065      # Park & Miller's known answer
+    assert oracle.pm_jump(5, 2147483646) == 5          # the generator's full period
+
+
+@needs_ref
+@pytest.mark.parametrize("which", ["u100", "cmt6"])
+def test_random_order_two_opt_stream_over_ants(which):
+    """The reference's CPU 2-opt (LocalSearch::do_local_search(ant)) draws one
+    value per route position: size - 1 per ant.  Called ant after ant on one
+    stream, it equals the oracle's random-order 2-opt carrying the seed, and
+    equals each ant started at the jump-ahead of the draws before it (how
+    the device runs the ants in parallel)."""
+    import os
+    from helpers import VRPLIB
+    inst = I.uniform_100(12) if which == "u100" else ca.Instance.from_vrplib(os.path.join(VRPLIB, "CMT6.vrp"))
+    P = oracle_problem(inst)
+    tours = _tours(inst, P, 12, 5)
+    s_ref = C.c_int(4242)
+    s_or = 4242
+    before = 0
+    for a, tour in enumerate(tours):
+        t_ref = tour.copy()
+        start = s_ref.value
+        REF.ref_two_opt(*_ref_args(inst), C.byref(s_ref), t_ref, len(t_ref))
+        assert s_ref.value == oracle.pm_jump(start, len(tour) - 1), a
+        t_or, L_or, _, s_or = P.local_search_seeded(tour, 8, s_or)
+        np.testing.assert_array_equal(t_ref, t_or, err_msg=f"ant {a}")
+        assert s_or == s_ref.value
+        t_jump, _, _, _ = P.local_search_seeded(tour, 8, oracle.pm_jump(4242, before))
+        np.testing.assert_array_equal(t_jump, t_or)
+        before += len(tour) - 1
+
+
 @needs_ref
 def test_insertion_and_exchange_apply_match_reference():
     rng = np.random.default_rng(1)
