@@ -66,12 +66,12 @@ struct ProfSlot {
 // device problem/state structures pointing into the instance's arena.
 struct InstanceData {
     int n = 0, P = 0, T = 0;
-    std::vector<float> h_dist;
-    std::vector<int> h_dem;
     int Q = 0;
     float D = INFINITY, svc = 0.0f;
     uint64_t seed = 0;
     int32_t ls_seed0 = 1;     // initial state of the scan-order stream
+    int setup_beta = 2;       // eta^beta on the device for beta 1 or 2 (0: uploaded)
+    bool setup_prior = false, setup_int16 = false, nnl_given = false;
     DevProblem dp{};
     DevState ds{};
     void *arena = nullptr;
@@ -97,6 +97,7 @@ struct aco_ctx {
     size_t lds_static_cons = 0, lds_static_ls = 0, lds_static_fused = 0;
     int max_n = 0;
     bool int16_rows = true;   // every distance an integer in [0, 65535]
+    void *setup_scratch = nullptr;   // candidate lists during aco_create
 };
 
 #define HIP_TRY(expr)                                                                      \
@@ -358,7 +359,7 @@ static int check_params(aco_params &prm, int min_n)
 // Instance checks: finite, non-negative, symmetric distances; every
 // customer servable by a vehicle of its own (otherwise construction could
 // never visit it: kernel.cl:438-479 would not terminate).
-static int check_instance(const aco_problem *pb, int width, std::vector<int> &nnl)
+static int check_instance(const aco_problem *pb, int width, std::vector<int> &nnl, bool &int16)
 {
     const int n = pb->n;
     if (n < 3 || n > kMaxNodes) return set_error(ACO_ERR_ARG, "aco_create: need 3 <= n <= 4096");
@@ -366,11 +367,13 @@ static int check_instance(const aco_problem *pb, int width, std::vector<int> &nn
     if (pb->capacity <= 0) return set_error(ACO_ERR_ARG, "capacity must be positive");
     if (isnan(pb->max_dist) || !(pb->service_time >= 0.0f) || !isfinite(pb->service_time))
         return set_error(ACO_ERR_ARG, "bad max_dist or service_time");
+    int16 = true;
     for (int i = 0; i < n; ++i)
         for (int j = 0; j < n; ++j) {
             const float d = pb->dist[(size_t)i * n + j];
             if (!isfinite(d) || d < 0.0f) return set_error(ACO_ERR_ARG, "distances must be finite and >= 0");
             if (d != pb->dist[(size_t)j * n + i]) return set_error(ACO_ERR_ARG, "distance matrix is not symmetric");
+            int16 = int16 && d <= 65535.0f && d == floorf(d);
         }
     for (int i = 0; i < n; ++i)
         if (pb->demand[i] < 0) return set_error(ACO_ERR_ARG, "negative demand");
@@ -382,8 +385,9 @@ static int check_instance(const aco_problem *pb, int width, std::vector<int> &nn
         if (!(t <= pb->max_dist))
             return set_error(ACO_ERR_INFEASIBLE, "customer " + std::to_string(i) + " beyond max_dist");
     }
-    nnl.assign((size_t)n * width, 0);
+    nnl.clear();
     if (pb->nn_list) {
+        nnl.assign((size_t)n * width, 0);
         if (pb->nn_width < width) return set_error(ACO_ERR_ARG, "nn_width smaller than max(k, nn_ls)");
         for (int i = 0; i < n; ++i)
             for (int h = 0; h < width; ++h) {
@@ -446,10 +450,13 @@ int aco_create_batch(const aco_problem *pbs, int32_t count, const aco_params *pr
     if (rc) return rc;
     std::vector<std::vector<int>> nnl(count);
     std::vector<int> nn_ls(count);
+    std::vector<char> int16(count);
     for (int b = 0; b < count; ++b) {
         nn_ls[b] = prm.nn_ls > 0 ? prm.nn_ls : pbs[b].n - 2;
-        rc = check_instance(&pbs[b], std::max(prm.k, nn_ls[b]), nnl[b]);
+        bool i16 = false;
+        rc = check_instance(&pbs[b], std::max(prm.k, nn_ls[b]), nnl[b], i16);
         if (rc) return rc;
+        int16[b] = i16;
     }
 
     int ndev = 0;
@@ -480,11 +487,11 @@ int aco_create_batch(const aco_problem *pbs, int32_t count, const aco_params *pr
     CHK(local_search_static_lds(c->lds_static_ls));
     CHK(fused_static_lds(c->lds_static_fused));
     c->ls = LsShape{4, 16, 1, 0, 0, 0};
-    if (local_search_lds_bytes(max_n, 16, 0) + c->lds_static_ls > prop.sharedMemPerBlock)
+    if (local_search_lds_bytes(max_n, 16, 0, prm.ls_flags) + c->lds_static_ls > prop.sharedMemPerBlock)
         c->ls = LsShape{4, 8, 0, 0, 0, 0};
     const size_t lds_need =
         std::max(construct_lds_bytes(max_n, prm.selection) + c->lds_static_cons,
-                 prm.ls_flags ? local_search_lds_bytes(max_n, c->ls.ring, 0) + c->lds_static_ls : 0);
+                 prm.ls_flags ? local_search_lds_bytes(max_n, c->ls.ring, 0, prm.ls_flags) + c->lds_static_ls : 0);
     if (lds_need > prop.sharedMemPerBlock)
         return fail(set_error(ACO_ERR_ARG, "instance too large for LDS: needs " + std::to_string(lds_need) +
                                                " bytes, device has " + std::to_string(prop.sharedMemPerBlock)));
@@ -492,22 +499,25 @@ int aco_create_batch(const aco_problem *pbs, int32_t count, const aco_params *pr
     CHK(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, c->device));
     c->tick_to_s = rate_khz > 0 ? 1.0f / (1000.0f * (float)rate_khz) : 0.0f;
     CHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    for (int b = 0; b < count; ++b) c->int16_rows = c->int16_rows && int16[b];
+    // device candidate lists (n x max(k, nn_ls) per instance): given ones
+    // are uploaded, the others built on the device from the uploaded
+    // distances (compute_nn_lists, vrpHelper.cpp:188-236); freed at the end
+    std::vector<int *> d_nnl(count, nullptr);
+    std::vector<int> nnl_width(count);
     {
-        // candidate lists of the instances given without them
-        // (compute_nn_lists, vrpHelper.cpp:188-236), all at once
-        std::vector<int> nv, wv;
-        std::vector<const float *> dv;
-        std::vector<int *> ov;
+        size_t total = 0;
         for (int b = 0; b < count; ++b) {
-            if (pbs[b].nn_list) continue;
-            nv.push_back(pbs[b].n);
-            wv.push_back(std::max(prm.k, nn_ls[b]));
-            dv.push_back(pbs[b].dist);
-            ov.push_back(nnl[b].data());
+            nnl_width[b] = std::max(prm.k, nn_ls[b]);
+            total += ((size_t)pbs[b].n * nnl_width[b] + 63) & ~size_t(63);
         }
-        if (!nv.empty()) {
-            const int rc2 = gpu_nn_lists(c->stream, (int)nv.size(), nv.data(), dv.data(), wv.data(), 0, ov.data());
-            if (rc2) return fail(rc2);
+        void *q = nullptr;
+        CHK(hipMalloc(&q, total * 4));
+        c->setup_scratch = q;
+        size_t off = 0;
+        for (int b = 0; b < count; ++b) {
+            d_nnl[b] = static_cast<int *>(q) + off;
+            off += ((size_t)pbs[b].n * nnl_width[b] + 63) & ~size_t(63);
         }
     }
 
@@ -535,20 +545,13 @@ int aco_create_batch(const aco_problem *pbs, int32_t count, const aco_params *pr
     for (int b = 0; b < count; ++b) {
         const aco_problem *pb = &pbs[b];
         InstanceData &I = c->inst[b];
-        const int n = pb->n, width = std::max(k, nn_ls[b]), nl = nn_ls[b];
+        const int n = pb->n, nl = nn_ls[b];
         const int nnP = (nl + 3) & ~3;
         I.n = n;
         // row pitch: every row of the fp32 and 16-bit matrices starts on a
         // 16-byte boundary (local search copies rows in 16-byte pieces)
         I.P = (n + 7) & ~7;
         I.T = 2 * n;
-        I.h_dist.assign(pb->dist, pb->dist + (size_t)n * n);
-        for (float d : I.h_dist)
-            if (!(d >= 0.0f && d <= 65535.0f && d == floorf(d))) {
-                c->int16_rows = false;
-                break;
-            }
-        I.h_dem.assign(pb->demand, pb->demand + n);
         I.Q = pb->capacity;
         I.D = pb->max_dist;
         I.svc = pb->service_time;
@@ -562,56 +565,36 @@ int aco_create_batch(const aco_problem *pbs, int32_t count, const aco_params *pr
         carve_instance(I, m, k, nnP, ras, prm.bsf_capacity, static_cast<unsigned char *>(I.arena));
         I.ds.clock0 = clock0;
 
-        // host-side derived arrays
-        std::vector<float> heur((size_t)n * n), d0(n);
-        for (size_t e = 0; e < (size_t)n * n; ++e) {
-            const float y = 1.0f / (pb->dist[e] + 0.1f);
-            float h = prm.beta == 2.0f ? y * y : prm.beta == 1.0f ? y : powf(y, prm.beta);
-            if (pb->prior) h = h * pb->prior[e];
-            heur[e] = h;
-        }
-        for (int i = 0; i < n; ++i) d0[i] = pb->dist[(size_t)i * n];
-        std::vector<int> rank(n);
-        tree_rank(n, rank.data());
-        std::vector<int> nnk((size_t)n * std::max(k, 1), 0), nnls((size_t)n * nnP, 0);
-        std::vector<float> nnkdist((size_t)n * std::max(k, 1), 0.0f), nnd((size_t)n * nnP, INFINITY);
-        std::vector<unsigned short> nnpos((size_t)n * n, 0xFFFF);
-        const std::vector<int> &L = nnl[b];
-        for (int i = 0; i < n; ++i) {
-            for (int h = 0; h < k; ++h) {
-                nnk[(size_t)i * k + h] = L[(size_t)i * width + h];
-                nnkdist[(size_t)i * k + h] = pb->dist[(size_t)i * n + L[(size_t)i * width + h]];
-            }
-            for (int h = 0; h < nl; ++h) {
-                const int v = L[(size_t)i * width + h];
-                nnls[(size_t)i * nnP + h] = v;
-                nnd[(size_t)i * nnP + h] = pb->dist[(size_t)i * n + v];
-                nnpos[(size_t)i * n + v] = (unsigned short)h;
-            }
-        }
+        // uploads: the distance matrix, demands, the reduction ranks, and the
+        // prior (into heur, multiplied on the device) or, for beta other
+        // than 1 and 2, eta^beta itself (host powf, as the CPU oracle)
         DevProblem &dp = I.dp;
         auto h2d = [&](void *dst, const void *src, size_t bytes_) { return up.put(dst, src, bytes_); };
         CHK(up.put2d((void *)dp.dist, (size_t)P * 4, pb->dist, (size_t)n * 4, n));
-        CHK(up.put2d((void *)dp.heur, (size_t)P * 4, heur.data(), (size_t)n * 4, n));
-        CHK(h2d((void *)dp.d0, d0.data(), (size_t)n * 4));
-        CHK(h2d((void *)dp.dem, pb->demand, (size_t)n * 4));
-        CHK(h2d((void *)dp.nnk, nnk.data(), nnk.size() * 4));
-        CHK(h2d((void *)dp.nnkd, nnkdist.data(), nnkdist.size() * 4));
-        CHK(h2d((void *)dp.nnls, nnls.data(), nnls.size() * 4));
-        CHK(h2d((void *)dp.nnd, nnd.data(), nnd.size() * 4));
-        CHK(h2d((void *)dp.rank, rank.data(), (size_t)n * 4));
-        CHK(h2d((void *)dp.nnpos, nnpos.data(), nnpos.size() * 2));
-        std::vector<unsigned short> d16((size_t)n * P, 0);
-        bool fits16 = true;
-        for (size_t e = 0; e < (size_t)n * n && fits16; ++e) {
-            const float d = pb->dist[e];
-            fits16 = d >= 0.0f && d <= 65535.0f && d == floorf(d);
-            if (fits16) d16[(e / n) * P + e % n] = (unsigned short)d;
+        const int beta_code = prm.beta == 2.0f ? 2 : prm.beta == 1.0f ? 1 : 0;
+        if (beta_code == 0) {
+            std::vector<float> heur((size_t)n * n);
+            for (size_t e = 0; e < (size_t)n * n; ++e) {
+                const float y = 1.0f / (pb->dist[e] + 0.1f);
+                float h = powf(y, prm.beta);
+                if (pb->prior) h = h * pb->prior[e];
+                heur[e] = h;
+            }
+            CHK(up.put2d((void *)dp.heur, (size_t)P * 4, heur.data(), (size_t)n * 4, n));
+        } else if (pb->prior) {
+            CHK(up.put2d((void *)dp.heur, (size_t)P * 4, pb->prior, (size_t)n * 4, n));
         }
-        if (fits16) CHK(h2d((void *)dp.dist16, d16.data(), d16.size() * 2));
+        CHK(h2d((void *)dp.dem, pb->demand, (size_t)n * 4));
+        std::vector<int> rank(n);
+        tree_rank(n, rank.data());
+        CHK(h2d((void *)dp.rank, rank.data(), (size_t)n * 4));
         CHK(h2d((void *)dp.rank64, r64.data(), kLanes * 4));
         CHK(h2d((void *)dp.rank64inv, r64inv.data(), kLanes * 4));
-
+        if (!nnl[b].empty()) CHK(h2d(d_nnl[b], nnl[b].data(), nnl[b].size() * 4));
+        I.setup_beta = beta_code;
+        I.setup_prior = pb->prior != nullptr;
+        I.setup_int16 = int16[b] != 0;
+        I.nnl_given = !nnl[b].empty();
         dp.n = n; dp.P = P; dp.k = k; dp.nn_ls = nl; dp.nnP = nnP; dp.m = m; dp.T = I.T; dp.ras = ras;
         dp.Q = pb->capacity; dp.D = pb->max_dist; dp.svc = pb->service_time;
         dp.alpha = prm.alpha; dp.rho = prm.rho;
@@ -627,6 +610,36 @@ int aco_create_batch(const aco_problem *pbs, int32_t count, const aco_params *pr
             if (!(Lnn > 0.0f)) return fail(set_error(ACO_ERR_INFEASIBLE, "nearest-neighbour tour failed"));
             dp.tau0_nn = 1.0f / ((float)n * Lnn);
         }
+    }
+
+    // candidate lists of the instances given without them, from the device
+    // distances, then every derived array of every instance
+    {
+        std::vector<int> nv, pv, wv;
+        std::vector<const float *> dv;
+        std::vector<int *> ov;
+        for (int b = 0; b < count; ++b) {
+            if (c->inst[b].nnl_given) continue;
+            nv.push_back(c->inst[b].n);
+            pv.push_back(c->inst[b].P);
+            wv.push_back(nnl_width[b]);
+            dv.push_back(c->inst[b].dp.dist);
+            ov.push_back(d_nnl[b]);
+        }
+        CHK(hipStreamSynchronize(c->stream));   // distances uploaded
+        if (!nv.empty()) {
+            const int rc2 =
+                gpu_nn_lists_dev(c->stream, (int)nv.size(), nv.data(), dv.data(), pv.data(), wv.data(), 0, ov.data());
+            if (rc2) return fail(rc2);
+        }
+        for (int b = 0; b < count; ++b) {
+            const InstanceData &I = c->inst[b];
+            CHK(launch_setup_rows(I.dp, I.setup_beta, I.setup_prior ? 1 : 0, I.setup_int16 ? 1 : 0, c->stream));
+            CHK(launch_setup_nn(I.dp, d_nnl[b], nnl_width[b], c->stream));
+        }
+        CHK(hipStreamSynchronize(c->stream));
+        (void)hipFree(c->setup_scratch);
+        c->setup_scratch = nullptr;
     }
 
     // device tables of the batch
@@ -698,6 +711,7 @@ int aco_create_batch(const aco_problem *pbs, int32_t count, const aco_params *pr
     CHK(hipMemcpyAsync(dbase, ant_base.data(), 4 * ant_base.size(), hipMemcpyHostToDevice, c->stream));
     CHK(hipStreamSynchronize(c->stream));
     bt.cons_lds_budget = c->lds_limit - c->lds_static_cons;
+    bt.cus = prop.multiProcessorCount;
     bt.probs = dprobs;
     bt.states = dstates;
     bt.ant_inst = dinst;
@@ -722,6 +736,7 @@ int aco_destroy(aco_ctx *c)
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
     for (void *p : c->allocs) (void)hipFree(p);
+    if (c->setup_scratch) (void)hipFree(c->setup_scratch);
     for (int i = 0; i < c->lss.count; ++i) {
         (void)hipStreamDestroy(c->lss.aux[i]);
         (void)hipEventDestroy(c->lss.join[i]);
@@ -1001,6 +1016,34 @@ int aco_get_rng_state(aco_ctx *c, int32_t *state)
     return ACO_OK;
 }
 
+int aco_get_setup_at(aco_ctx *c, int32_t b, float *heur, float *d0, int32_t *nnk, float *nnkd, int32_t *nnls,
+                     float *nnd, uint16_t *nnpos, uint16_t *dist16)
+{
+    int rc = need_inst(c, b, false);
+    if (rc) return rc;
+    const InstanceData &I = c->inst[b];
+    const DevProblem &dp = I.dp;
+    const int n = I.n, k = c->prm.k, nl = dp.nn_ls;
+    if (heur)
+        HIP_TRY(hipMemcpy2DAsync(heur, (size_t)n * 4, dp.heur, (size_t)I.P * 4, (size_t)n * 4, n, hipMemcpyDeviceToHost,
+                                 c->stream));
+    if (d0) HIP_TRY(hipMemcpyAsync(d0, dp.d0, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+    if (nnk && k > 0) HIP_TRY(hipMemcpyAsync(nnk, dp.nnk, (size_t)n * k * 4, hipMemcpyDeviceToHost, c->stream));
+    if (nnkd && k > 0) HIP_TRY(hipMemcpyAsync(nnkd, dp.nnkd, (size_t)n * k * 4, hipMemcpyDeviceToHost, c->stream));
+    if (nnls)
+        HIP_TRY(hipMemcpy2DAsync(nnls, (size_t)nl * 4, dp.nnls, (size_t)dp.nnP * 4, (size_t)nl * 4, n,
+                                 hipMemcpyDeviceToHost, c->stream));
+    if (nnd)
+        HIP_TRY(hipMemcpy2DAsync(nnd, (size_t)nl * 4, dp.nnd, (size_t)dp.nnP * 4, (size_t)nl * 4, n,
+                                 hipMemcpyDeviceToHost, c->stream));
+    if (nnpos) HIP_TRY(hipMemcpyAsync(nnpos, dp.nnpos, (size_t)n * n * 2, hipMemcpyDeviceToHost, c->stream));
+    if (dist16 && c->int16_rows)
+        HIP_TRY(hipMemcpy2DAsync(dist16, (size_t)n * 2, dp.dist16, (size_t)I.P * 2, (size_t)n * 2, n,
+                                 hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return ACO_OK;
+}
+
 int aco_set_rng_state(aco_ctx *c, const int32_t *state)
 {
     int rc = need(c, false);
@@ -1074,7 +1117,8 @@ static void drop_graph(aco_ctx *c)
 
 static bool shape_fits(const aco_ctx *c, const LsShape &sh)
 {
-    if (!ls_shape_valid(sh) || local_search_lds_bytes(c->max_n, sh.ring, sh.row16) + c->lds_static_ls > c->lds_limit)
+    if (!ls_shape_valid(sh) ||
+        local_search_lds_bytes(c->max_n, sh.ring, sh.row16, c->prm.ls_flags) + c->lds_static_ls > c->lds_limit)
         return false;
     if (sh.row16 && !c->int16_rows) return false;
     if (sh.fused && (!fused_iteration_ok(c->bt, c->prm.selection, c->prm.ls_flags, c->m, sh) ||

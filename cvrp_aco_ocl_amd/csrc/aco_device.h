@@ -195,6 +195,7 @@ struct DevBatch {
     int ls_maxn[kLsClasses];          // largest n in class c
     int *fq_next;                     // fused iteration: next work item (reset before each launch)
     size_t cons_lds_budget;           // dynamic LDS the staged construction may use (host side)
+    int cus;                          // compute units of the device (host side)
 };
 
 enum { kTau0Arg = 0, kTau0Bsf = 1, kTau0Nn = 2 };
@@ -247,13 +248,21 @@ __host__ __device__ inline int pm_jump(int z, unsigned long long t)
 // out, one thread per row, the reference's sort steps); ACO_OK or an error.
 int gpu_nn_lists(hipStream_t st, int count, const int *n, const float *const *dist, const int *width,
                  int tie, int *const *out);
+// the same from device distances (rows of pitch pitch[b]) into device arrays
+int gpu_nn_lists_dev(hipStream_t st, int count, const int *n, const float *const *dist, const int *pitch,
+                     const int *width, int tie, int *const *out);
+// instance set-up on the device (aco_setup.hip): eta^beta (beta_code 1 or 2;
+// 0 = uploaded), the prior already in heur multiplied in, d0, 16-bit rows;
+// candidate and neighbour tables from the lists nnl (n x width)
+hipError_t launch_setup_rows(const DevProblem &p, int beta_code, int has_prior, int int16, hipStream_t st);
+hipError_t launch_setup_nn(const DevProblem &p, const int *nnl, int width, hipStream_t st);
 hipError_t launch_statistics(const DevBatch &b, float tick_to_s, hipStream_t st);
 hipError_t launch_pheromone_update(const DevBatch &b, hipStream_t st);
 hipError_t launch_gather(const DevBatch &b, hipStream_t st);
 hipError_t launch_pheromone_init(const DevBatch &b, int mode, float tau0, hipStream_t st);
 hipError_t launch_stamp_clock(const DevBatch &b, hipStream_t st);
 size_t construct_lds_bytes(int n, int selection);
-size_t local_search_lds_bytes(int n, int ring, int row16);
+size_t local_search_lds_bytes(int n, int ring, int row16, int flags);
 // Fused iteration (aco_kernels.hip): candidate roulette without a route
 // length limit, W = 4 or 8 waves dividing the ants per instance.
 bool fused_iteration_ok(const DevBatch &b, int selection, int ls_flags, int ants_per_instance, const LsShape &sh);

@@ -145,71 +145,71 @@ __device__ __forceinline__ bool feasible_mask(const DevProblem &p, const AntLds 
 
 // Full-row greedy pick when the roulette has no positive mass: largest
 // total_info among feasible nodes, ties to the lowest index; 0 = none.
-// Nodes are taken kGreedyChunk per lane at a time (kGreedyChunk * 64 per
-// pass).  The visited flags and capacity test come from LDS first; a pass
-// whose nodes are all infeasible reads nothing from global memory (a route
-// that is full returns to the depot without touching the row), otherwise
-// the row values (and, with a length limit, distances) of the feasible
-// nodes are loaded together, one memory latency per pass.
-constexpr int kGreedyChunk = 16;
-
-// The first pass's row values, loaded by the caller ahead of time (at the
-// start of every step of the unstaged kernel, so a step that falls back
-// finds them already in flight).
-struct GreedyPre {
-    float tv[kGreedyChunk];
-};
-
-__device__ __forceinline__ void greedy_preload(const DevProblem &p, const float *tirow, int lane, GreedyPre &g)
+// The visited flags and capacity test come from LDS first; a pass whose
+// nodes are all infeasible reads nothing from global memory (a route that
+// is full returns to the depot without touching the row), otherwise the
+// row values (and, with a length limit, distances) are loaded together,
+// one memory latency per pass.  Two shapes:
+//   WIDE   16 nodes per lane per pass (1024 per pass), every load of the
+//          pass issued (clamped index, masked afterwards): fewest round
+//          trips, for a device with spare memory bandwidth (few ants)
+//   narrow 8 nodes per lane per pass, a chunk of 64 nodes none of which is
+//          feasible loads the row's first element instead (one cached
+//          line): fewer bytes, for a busy device
+template <bool HAS_D, bool WIDE>
+__device__ __forceinline__ void greedy_pass(const DevProblem &p, const AntLds &L, const float *tirow,
+                                            const float *drow, int load, float rdist, int lane, int base,
+                                            u64 &best)
 {
+    constexpr int CH = WIDE ? 16 : 8;
+    unsigned fm = 0;   // bit u: node base + u*64 + lane passes the LDS tests
+    unsigned need = 0; // bit u: some lane of chunk u does (uniform)
 #pragma unroll
-    for (int u = 0; u < kGreedyChunk; ++u) {
-        const int i = u * kLanes + lane;
-        g.tv[u] = gload(tirow + (i < p.n ? i : p.n - 1));
+    for (int u = 0; u < CH; ++u) {
+        const int i = base + u * kLanes + lane;
+        const int ic = i < p.n ? i : p.n - 1;   // in-row index, masked below
+        const bool f = (i < p.n) & (L.vis[ic] == 0) & (load + L.dem[ic] <= p.Q);
+        fm |= f ? 1u << u : 0u;
+        if (!WIDE) need |= __ballot(f) ? 1u << u : 0u;
+    }
+    if (WIDE ? !__ballot(fm != 0) : !need) return;
+    float tv[CH], dv[CH];
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+        const int i = base + u * kLanes + lane;
+        const int ia = WIDE ? (i < p.n ? i : p.n - 1) : (((need >> u) & 1u) && i < p.n ? i : 0);
+        tv[u] = gload(tirow + ia);
+        dv[u] = HAS_D ? gload(drow + ia) : 0.0f;
+    }
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+        const int i = base + u * kLanes + lane;
+        bool f = (fm >> u) & 1u;
+        if (HAS_D) {
+            float t = rdist + (dv[u] + p.svc);
+            t = t + L.d0[i < p.n ? i : p.n - 1];
+            f = f & (t <= p.D);
+        }
+        if (f) {
+            const u64 key = ((u64)(f2u(tv[u]) + 1u) << 32) | (u64)(0xFFFFFFFFu - (u32)i);
+            best = key > best ? key : best;
+        }
     }
 }
 
-template <bool HAS_D, bool PRE = false>
-__device__ int greedy_pick(const DevProblem &p, const AntLds &L, const float *tirow,
-                           const float *drow, int load, float rdist, int lane,
-                           const GreedyPre *pre = nullptr)
+template <bool HAS_D, bool WIDE = false, bool PEEL = WIDE>
+__device__ __forceinline__ int greedy_pick(const DevProblem &p, const AntLds &L, const float *tirow,
+                           const float *drow, int load, float rdist, int lane)
 {
+    constexpr int PASS = (WIDE ? 16 : 8) * kLanes;
     u64 best = 0;
-    for (int base = 0; base < p.n; base += kGreedyChunk * kLanes) {
-        unsigned fm = 0;   // bit u: node base + u*64 + lane passes the LDS tests
-#pragma unroll
-        for (int u = 0; u < kGreedyChunk; ++u) {
-            const int i = base + u * kLanes + lane;
-            const int ic = i < p.n ? i : p.n - 1;   // in-row index, masked below
-            const bool f = (i < p.n) & (L.vis[ic] == 0) & (load + L.dem[ic] <= p.Q);
-            fm |= f ? 1u << u : 0u;
-        }
-        if (!__ballot(fm != 0)) continue;
-        // every load of the pass unconditional (clamped index, masked by fm
-        // below), so they are all in flight together
-        float tv[kGreedyChunk], dv[kGreedyChunk];
-#pragma unroll
-        for (int u = 0; u < kGreedyChunk; ++u) {
-            const int i = base + u * kLanes + lane;
-            const int ic = i < p.n ? i : p.n - 1;
-            tv[u] = PRE && base == 0 ? pre->tv[u] : gload(tirow + ic);
-            dv[u] = HAS_D ? gload(drow + ic) : 0.0f;
-        }
-#pragma unroll
-        for (int u = 0; u < kGreedyChunk; ++u) {
-            const int i = base + u * kLanes + lane;
-            bool f = (fm >> u) & 1u;
-            if (HAS_D) {
-                float t = rdist + (dv[u] + p.svc);
-                t = t + L.d0[i < p.n ? i : p.n - 1];
-                f = f & (t <= p.D);
-            }
-            if (f) {
-                const u64 key = ((u64)(f2u(tv[u]) + 1u) << 32) | (u64)(0xFFFFFFFFu - (u32)i);
-                best = key > best ? key : best;
-            }
-        }
-    }
+    // PEEL: the first pass on its own, with constant offsets from the row's
+    // start and scalar-based loads (faster for the large instances of the
+    // wide shape and the staged kernel; one loop body suits the small
+    // instances of a batch better)
+    if (PEEL) greedy_pass<HAS_D, WIDE>(p, L, tirow, drow, load, rdist, lane, 0, best);
+    for (int base = PEEL ? PASS : 0; base < p.n; base += PASS)
+        greedy_pass<HAS_D, WIDE>(p, L, tirow, drow, load, rdist, lane, base, best);
     best = wave_max_u64(best);
     return best ? (int)(0xFFFFFFFFu - (u32)(best & 0xFFFFFFFFull)) : 0;
 }
@@ -339,7 +339,7 @@ __global__ __launch_bounds__(64) void k_construct_iroulette(DevBatch bt)
 // demands and depot distances and this ant's visited flags (depot set).
 // The step loop synchronises the wave only, so the waves of a block may
 // build different ants.
-template <bool HAS_D, int RNG, bool SPEC = false>
+template <bool HAS_D, int RNG, bool WIDE = false>
 __device__ void construct_knn_ant(const DevProblem &p, const DevState &s, int a, const AntLds &L, int lane)
 {
     const int n = p.n, k = p.k;
@@ -361,9 +361,6 @@ __device__ void construct_knn_ant(const DevProblem &p, const DevState &s, int a,
         const int j = gload(p.nnk + e);
         const float tj = gload(s.tiknn + e);
         const float dcj = HAS_D ? gload(p.nnkd + e) : 0.0f;
-        // the row's values for a fallback, requested with the candidates
-        GreedyPre pre;
-        if (SPEC) greedy_preload(p, s.tinfo + (size_t)cur * p.P, lane, pre);
         const int dj = L.dem[j];   // the chosen lane's value is read out below
         const bool ok = (lane < k) & feasible_mask<HAS_D>(p, L, j, dj, dcj, load, rdist);
         const float pj = tj * (ok ? 1.0f : 0.0f);
@@ -387,7 +384,7 @@ __device__ void construct_knn_ant(const DevProblem &p, const DevState &s, int a,
             dcw = HAS_D ? lane_f32(dcj, wc) : 0.0f;
         } else {
             const float *drow = p.dist + (size_t)cur * p.P;
-            w = greedy_pick<HAS_D, SPEC>(p, L, s.tinfo + (size_t)cur * p.P, drow, load, rdist, lane, &pre);
+            w = greedy_pick<HAS_D, WIDE>(p, L, s.tinfo + (size_t)cur * p.P, drow, load, rdist, lane);
             fb++;
             w = uniform(w);
             // scalar reads keep their waits on this path (see the staged
@@ -428,7 +425,7 @@ __device__ void construct_knn_ant(const DevProblem &p, const DevState &s, int a,
     }
 }
 
-template <bool HAS_D, int RNG>
+template <bool HAS_D, int RNG, bool WIDE>
 __global__ __launch_bounds__(64) void k_construct_knn(DevBatch bt)
 {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -441,7 +438,7 @@ __global__ __launch_bounds__(64) void k_construct_knn(DevBatch bt)
         L.vis[i] = i == 0;
     }
     __syncthreads();
-    construct_knn_ant<HAS_D, RNG, true>(p, s, a, L, lane);
+    construct_knn_ant<HAS_D, RNG, WIDE>(p, s, a, L, lane);
 }
 
 // The same roulette with the instance's candidate table staged in LDS, for
@@ -543,7 +540,7 @@ __global__ __launch_bounds__(64 * A) void k_construct_knn_lds(DevBatch bt, int r
             w = lane_i32(j, wc);
             dw = lane_i32(dj, wc);
         } else {
-            w = greedy_pick<false>(p, L, s.tinfo + (size_t)cur * p.P, nullptr, load, 0.0f, lane);
+            w = greedy_pick<false, false, true>(p, L, s.tinfo + (size_t)cur * p.P, nullptr, load, 0.0f, lane);
             fb++;
             w = uniform(w);
             // scalar, so the wait for this read stays on this path: a vector
@@ -623,9 +620,14 @@ static hipError_t max_static_lds(const void *const *fns, int count, size_t &byte
 hipError_t construct_static_lds(size_t &bytes)
 {
     const void *fns[] = {(const void *)k_construct_iroulette<false>, (const void *)k_construct_iroulette<true>,
-                         (const void *)k_construct_knn<false, kRngPhilox>, (const void *)k_construct_knn<true, kRngPhilox>,
-                         (const void *)k_construct_knn<false, kRngParkMiller>,
-                         (const void *)k_construct_knn<true, kRngParkMiller>,
+                         (const void *)k_construct_knn<false, kRngPhilox, true>,
+                         (const void *)k_construct_knn<true, kRngPhilox, true>,
+                         (const void *)k_construct_knn<false, kRngParkMiller, true>,
+                         (const void *)k_construct_knn<true, kRngParkMiller, true>,
+                         (const void *)k_construct_knn<false, kRngPhilox, false>,
+                         (const void *)k_construct_knn<true, kRngPhilox, false>,
+                         (const void *)k_construct_knn<false, kRngParkMiller, false>,
+                         (const void *)k_construct_knn<true, kRngParkMiller, false>,
                          (const void *)k_construct_knn_lds<kRngPhilox, 1>, (const void *)k_construct_knn_lds<kRngPhilox, 2>,
                          (const void *)k_construct_knn_lds<kRngPhilox, 4>,
                          (const void *)k_construct_knn_lds<kRngParkMiller, 1>,
@@ -655,12 +657,20 @@ hipError_t launch_construct(const DevBatch &b, int selection, int rng, int apb, 
     if (selection == kSelIRoulette) {
         if (b.has_d) hipLaunchKernelGGL(k_construct_iroulette<true>, grid, block, lds, st, b);
         else hipLaunchKernelGGL(k_construct_iroulette<false>, grid, block, lds, st, b);
-    } else if (rng == kRngPhilox) {
-        if (b.has_d) hipLaunchKernelGGL((k_construct_knn<true, kRngPhilox>), grid, block, lds, st, b);
-        else hipLaunchKernelGGL((k_construct_knn<false, kRngPhilox>), grid, block, lds, st, b);
     } else {
-        if (b.has_d) hipLaunchKernelGGL((k_construct_knn<true, kRngParkMiller>), grid, block, lds, st, b);
-        else hipLaunchKernelGGL((k_construct_knn<false, kRngParkMiller>), grid, block, lds, st, b);
+        // the wide fallback pass while a step waits on its own memory
+        // latency (at most one ant's wave per SIMD); with more ants the
+        // device is busy and the narrow pass's fewer bytes win
+        const bool spec = b.ants <= 4 * b.cus;
+#define CK(D, R, S) hipLaunchKernelGGL((k_construct_knn<D, R, S>), grid, block, lds, st, b)
+        if (rng == kRngPhilox) {
+            if (b.has_d) { if (spec) CK(true, kRngPhilox, true); else CK(true, kRngPhilox, false); }
+            else { if (spec) CK(false, kRngPhilox, true); else CK(false, kRngPhilox, false); }
+        } else {
+            if (b.has_d) { if (spec) CK(true, kRngParkMiller, true); else CK(true, kRngParkMiller, false); }
+            else { if (spec) CK(false, kRngParkMiller, true); else CK(false, kRngParkMiller, false); }
+        }
+#undef CK
     }
     return hipGetLastError();
 }
@@ -1053,7 +1063,7 @@ bool fused_iteration_ok(const DevBatch &b, int selection, int ls_flags, int ants
 size_t fused_lds_bytes(int n, const LsShape &sh)
 {
     const size_t cons = (size_t)n * 8 + (size_t)sh.waves * ((n + 15) & ~15) + 16;
-    const size_t ls = local_search_lds_bytes(n, sh.ring, sh.row16);
+    const size_t ls = local_search_lds_bytes(n, sh.ring, sh.row16, 0);
     return cons > ls ? cons : ls;
 }
 

@@ -5,9 +5,12 @@
 
 namespace cvrp {
 
-size_t local_search_lds_bytes(int n, int ring, int row16) { return ls_offsets(n, ring, row16 ? 2 : 4).total; }
+size_t local_search_lds_bytes(int n, int ring, int row16, int flags)
+{
+    return ls_offsets(n, ring, row16 ? 2 : 4, (flags & kLs2OptRandom) != 0).total;
+}
 
-template <int W, typename RT>
+template <int W, typename RT, bool R2>
 __global__ __launch_bounds__(W * 64) void k_local_search(DevBatch bt, int first, int flags, int ring, int prefetch)
 {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -19,17 +22,29 @@ __global__ __launch_bounds__(W * 64) void k_local_search(DevBatch bt, int first,
     // in-flight row copies included
     const DevProblem p = bt.probs[inst];
     const DevState s = bt.states[inst];
-    local_search_ant<W, RT>(p, s, g - bt.ant_base[inst], flags, ring, prefetch, smem);
+    local_search_ant<W, RT, R2>(p, s, g - bt.ant_base[inst], flags, ring, prefetch, smem);
 }
 
 // Largest static LDS (the kernel's own __shared__ arrays, beside the
 // dynamic allocation) of the local-search kernels.
 hipError_t local_search_static_lds(size_t &bytes)
 {
-    const void *fns[] = {(const void *)k_local_search<4, float>,  (const void *)k_local_search<4, unsigned short>,
-                         (const void *)k_local_search<8, float>,  (const void *)k_local_search<8, unsigned short>,
-                         (const void *)k_local_search<12, float>, (const void *)k_local_search<12, unsigned short>,
-                         (const void *)k_local_search<16, float>, (const void *)k_local_search<16, unsigned short>};
+    const void *fns[] = {(const void *)k_local_search<4, float, false>,
+                         (const void *)k_local_search<4, unsigned short, false>,
+                         (const void *)k_local_search<8, float, false>,
+                         (const void *)k_local_search<8, unsigned short, false>,
+                         (const void *)k_local_search<12, float, false>,
+                         (const void *)k_local_search<12, unsigned short, false>,
+                         (const void *)k_local_search<16, float, false>,
+                         (const void *)k_local_search<16, unsigned short, false>,
+                         (const void *)k_local_search<4, float, true>,
+                         (const void *)k_local_search<4, unsigned short, true>,
+                         (const void *)k_local_search<8, float, true>,
+                         (const void *)k_local_search<8, unsigned short, true>,
+                         (const void *)k_local_search<12, float, true>,
+                         (const void *)k_local_search<12, unsigned short, true>,
+                         (const void *)k_local_search<16, float, true>,
+                         (const void *)k_local_search<16, unsigned short, true>};
     bytes = 0;
     for (const void *f : fns) {
         hipFuncAttributes a;
@@ -82,20 +97,22 @@ hipError_t launch_local_search(const DevBatch &b, int flags, const LsShape &sh, 
     }
     for (int cl = 0; cl < b.ls_classes; ++cl) {
         const int first = b.ls_first[cl], count = b.ls_first[cl + 1] - first;
-        const size_t lds = local_search_lds_bytes(b.ls_maxn[cl], sh.ring, sh.row16);
+        const size_t lds = local_search_lds_bytes(b.ls_maxn[cl], sh.ring, sh.row16, flags);
         const hipStream_t s = fork && cl > 0 ? par->aux[cl - 1] : st;
         if (s != st) {
             const hipError_t e = hipStreamWaitEvent(s, par->fork, 0);
             if (e != hipSuccess) return e;
         }
-#define KLS(W, RT) hipLaunchKernelGGL((k_local_search<W, RT>), dim3(count), dim3(W * kLanes), lds, s, b, first, \
-                                      flags, sh.ring, sh.prefetch)
+#define KLS(W, RT, R2) hipLaunchKernelGGL((k_local_search<W, RT, R2>), dim3(count), dim3(W * kLanes), lds, s, b, \
+                                          first, flags, sh.ring, sh.prefetch)
+#define KLS2(W, RT) do { if (flags & kLs2OptRandom) KLS(W, RT, true); else KLS(W, RT, false); } while (0)
         switch (sh.waves) {
-        case 16: if (sh.row16) KLS(16, unsigned short); else KLS(16, float); break;
-        case 12: if (sh.row16) KLS(12, unsigned short); else KLS(12, float); break;
-        case 8: if (sh.row16) KLS(8, unsigned short); else KLS(8, float); break;
-        default: if (sh.row16) KLS(4, unsigned short); else KLS(4, float); break;
+        case 16: if (sh.row16) KLS2(16, unsigned short); else KLS2(16, float); break;
+        case 12: if (sh.row16) KLS2(12, unsigned short); else KLS2(12, float); break;
+        case 8: if (sh.row16) KLS2(8, unsigned short); else KLS2(8, float); break;
+        default: if (sh.row16) KLS2(4, unsigned short); else KLS2(4, float); break;
         }
+#undef KLS2
 #undef KLS
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;

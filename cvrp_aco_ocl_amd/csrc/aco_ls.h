@@ -60,7 +60,9 @@ struct LsOffsets {
     size_t tour, dtw, rl, rd, dem, rows, res, pos, rof, ridx, zpos, rc, perm, dlb, total;
 };
 
-__host__ __device__ inline LsOffsets ls_offsets(int n, int ring, int rowbytes)
+// perm (the random scan order of ACO_LS_2OPT_RANDOM) takes LDS only when
+// that flag is set.
+__host__ __device__ inline LsOffsets ls_offsets(int n, int ring, int rowbytes, bool perm = false)
 {
     const size_t T = 2 * (size_t)n, N = (size_t)n;
     LsOffsets o;
@@ -78,15 +80,15 @@ __host__ __device__ inline LsOffsets ls_offsets(int n, int ring, int rowbytes)
     o.ridx = at; at = ls_align16(at + 2 * (T + kScanPad));
     o.zpos = at; at = ls_align16(at + 2 * (N + 1));
     o.rc = at;   at = ls_align16(at + 2 * N);
-    o.perm = at; at = ls_align16(at + 2 * T);
+    o.perm = at; at = ls_align16(at + (perm ? 2 * T : 0));
     o.dlb = at;  at = ls_align16(at + N);
     o.total = at;
     return o;
 }
 
-__device__ __forceinline__ LsLds ls_bind(int n, int ring, int rowbytes, unsigned char *b)
+__device__ __forceinline__ LsLds ls_bind(int n, int ring, int rowbytes, bool perm, unsigned char *b)
 {
-    const LsOffsets o = ls_offsets(n, ring, rowbytes);
+    const LsOffsets o = ls_offsets(n, ring, rowbytes, perm);
     LsLds L;
     L.tour = (int *)(b + o.tour); L.dtw = (float *)(b + o.dtw);
     L.rl = (int *)(b + o.rl); L.rd = (float *)(b + o.rd); L.dem = (int *)(b + o.dem);
@@ -1153,13 +1155,15 @@ __device__ __forceinline__ void ls_relocate(const DevProblem &p, LsLds &L, const
 // Local search of ant a of one instance by the W waves of the calling block,
 // LDS at smem (local_search_lds_bytes).  The caller synchronises the block
 // before reusing smem.
-template <int W, typename RT>
+// R2: the instantiation with the random-order 2-opt (ACO_LS_2OPT_RANDOM);
+// kept out of the others, whose code it would otherwise enlarge.
+template <int W, typename RT, bool R2 = false>
 __device__ void local_search_ant(const DevProblem &p, const DevState &s, int a, int flags, int ring,
                                  int prefetch, unsigned char *smem)
 {
     constexpr int NT = LsW<W>::threads;
     const int tid = threadIdx.x;
-    LsLds L = ls_bind(p.n, ring, (int)sizeof(RT), smem);
+    LsLds L = ls_bind(p.n, ring, (int)sizeof(RT), R2, smem);
     const RowRing<RT> rr = {reinterpret_cast<RT *>(L.rows), ring - 1, row_stride<RT>(p.n)};
     const bool pf_on = prefetch != 0;
     // uniform: loop bounds, row windows and masks stay in scalar registers
@@ -1195,7 +1199,7 @@ __device__ void local_search_ant(const DevProblem &p, const DevState &s, int a, 
         }
         __syncthreads();
     }
-    if (flags & kLs2OptRandom) {
+    if constexpr (R2) {
         // CPU variant's scan order: one Park-Miller stream per instance,
         // consumed ant after ant and route after route (one draw per route
         // position, size - 1 per ant) as sequential calls would; ant a

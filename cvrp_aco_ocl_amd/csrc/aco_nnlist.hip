@@ -21,11 +21,11 @@
 namespace cvrp {
 
 struct NnJob {
-    const float *dist;   // n x n, row-major
+    const float *dist;   // n rows of pitch dpitch, row-major
     float *v;            // n x n sort keys
     int *ix;             // n x n node ids
     int *out;            // n x width
-    int n, width, first_row;
+    int n, width, first_row, dpitch;
 };
 
 constexpr int kNnStack = 32;   // > log2(n) + 1 for every n the library accepts
@@ -95,7 +95,7 @@ __global__ __launch_bounds__(256) void k_nn_lists(const NnJob *jobs, int njobs, 
     }
     const NnJob J = jobs[lo];
     const int node = row - J.first_row, n = J.n;
-    const float *d = J.dist + (size_t)node * n;
+    const float *d = J.dist + (size_t)node * J.dpitch;
     float *v = J.v + (size_t)node * n;
     int *ix = J.ix + (size_t)node * n;
     for (int i = 0; i < n; ++i) {
@@ -219,7 +219,7 @@ int gpu_nn_lists(hipStream_t st, int count, const int *n, const float *const *di
         hipError_t err = hipSuccess;
         for (int j = 0; j < nj && err == hipSuccess; ++j) {
             const int b = b0 + j;
-            jobs[j] = NnJob{ddist + eo, dv + eo, dix + eo, dout + oo, n[b], width[b], ro};
+            jobs[j] = NnJob{ddist + eo, dv + eo, dix + eo, dout + oo, n[b], width[b], ro, n[b]};
             err = hipMemcpyAsync(ddist + eo, dist[b], (size_t)n[b] * n[b] * 4, hipMemcpyHostToDevice, st);
             eo += (size_t)n[b] * n[b];
             oo += (size_t)n[b] * width[b];
@@ -236,6 +236,54 @@ int gpu_nn_lists(hipStream_t st, int count, const int *n, const float *const *di
             const int b = b0 + j;
             err = hipMemcpyAsync(out[b], dout + oo, (size_t)n[b] * width[b] * 4, hipMemcpyDeviceToHost, st);
             oo += (size_t)n[b] * width[b];
+        }
+        const hipError_t es = hipStreamSynchronize(st);
+        if (err == hipSuccess) err = es;
+        (void)hipFree(buf);
+        if (err != hipSuccess) return set_error(ACO_ERR_HIP, std::string("nn lists: ") + hipGetErrorString(err));
+        b0 = b1;
+    }
+    return ACO_OK;
+}
+
+// The same lists from distance matrices already on the device (n[b] rows
+// of pitch pitch[b]) into device arrays out[b] (n[b] x width[b]); sort
+// scratch allocated here, in chunks as above.  Enqueued on st and waited
+// for.
+int gpu_nn_lists_dev(hipStream_t st, int count, const int *n, const float *const *dist, const int *pitch,
+                     const int *width, int tie, int *const *out)
+{
+    int b0 = 0;
+    while (b0 < count) {
+        size_t elems = 0;
+        int b1 = b0, rows = 0;
+        while (b1 < count && (b1 == b0 || elems + (size_t)n[b1] * n[b1] <= kNnChunkElems)) {
+            elems += (size_t)n[b1] * n[b1];
+            rows += n[b1];
+            ++b1;
+        }
+        const int nj = b1 - b0;
+        const size_t bytes = elems * 8 + (size_t)nj * sizeof(NnJob) + 512;
+        unsigned char *buf = nullptr;
+        if (hipMalloc(&buf, bytes) != hipSuccess)
+            return set_error(ACO_ERR_HIP, "nn lists: device allocation of " + std::to_string(bytes) + " bytes failed");
+        float *dv = reinterpret_cast<float *>(buf);
+        int *dix = reinterpret_cast<int *>(dv + elems);
+        NnJob *djobs = reinterpret_cast<NnJob *>(
+            (reinterpret_cast<uintptr_t>(dix + elems) + 255) & ~uintptr_t(255));
+        std::vector<NnJob> jobs(nj);
+        size_t eo = 0;
+        int ro = 0;
+        for (int j = 0; j < nj; ++j) {
+            const int b = b0 + j;
+            jobs[j] = NnJob{dist[b], dv + eo, dix + eo, out[b], n[b], width[b], ro, pitch[b]};
+            eo += (size_t)n[b] * n[b];
+            ro += n[b];
+        }
+        hipError_t err = hipMemcpyAsync(djobs, jobs.data(), sizeof(NnJob) * nj, hipMemcpyHostToDevice, st);
+        if (err == hipSuccess) {
+            hipLaunchKernelGGL(k_nn_lists, dim3((rows + 255) / 256), dim3(256), 0, st, djobs, nj, rows, tie);
+            err = hipGetLastError();
         }
         const hipError_t es = hipStreamSynchronize(st);
         if (err == hipSuccess) err = es;

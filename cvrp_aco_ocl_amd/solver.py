@@ -86,6 +86,21 @@ class Instance:
         return out
 
     @staticmethod
+    def distances_batch(xs, ys, mode=_lib.DIST_NINT) -> list:
+        """Distance matrices of many coordinate sets in one pass on the GPU
+        (aco_compute_distances_batch)."""
+        xs = [np.ascontiguousarray(x, np.float64) for x in xs]
+        ys = [np.ascontiguousarray(y, np.float64) for y in ys]
+        outs = [np.empty((len(x), len(x)), np.float32) for x in xs]
+        ns = np.array([len(x) for x in xs], np.int32)
+        xp = (C.c_void_p * len(xs))(*[a.ctypes.data for a in xs])
+        yp = (C.c_void_p * len(ys))(*[a.ctypes.data for a in ys])
+        op = (C.c_void_p * len(outs))(*[a.ctypes.data for a in outs])
+        check(lib.aco_compute_distances_batch(len(xs), _ptr(ns), C.cast(xp, C.c_void_p), C.cast(yp, C.c_void_p),
+                                              mode, C.cast(op, C.c_void_p)), "aco_compute_distances_batch")
+        return outs
+
+    @staticmethod
     def nn_lists_batch(insts, nn, tie: int = 0) -> list:
         """nn_lists of many instances in one pass on the GPU
         (aco_compute_nn_lists_batch); nn is one width or one per instance."""
@@ -287,6 +302,23 @@ class GpuACO:
     def rng_state(self):
         out = np.empty((self.m, 64), np.int32)
         check(lib.aco_get_rng_state(self.h, _ptr(out)), "aco_get_rng_state")
+        return out
+
+    def setup_arrays(self, b: int = 0) -> dict:
+        """The arrays aco_create derived on the device (aco_get_setup_at)."""
+        it = self.insts[b]
+        n, k = it.n, self.params.k
+        nl = self.params.nn_ls if self.params.nn_ls > 0 else n - 2
+        out = {"heur": np.empty((n, n), np.float32), "d0": np.empty(n, np.float32),
+               "nnk": np.empty((n, max(k, 1)), np.int32), "nnkd": np.empty((n, max(k, 1)), np.float32),
+               "nnls": np.empty((n, nl), np.int32), "nnd": np.empty((n, nl), np.float32),
+               "nnpos": np.empty((n, n), np.uint16), "dist16": np.zeros((n, n), np.uint16)}
+        check(lib.aco_get_setup_at(self.h, b, *[_ptr(out[key]) for key in
+                                               ("heur", "d0", "nnk", "nnkd", "nnls", "nnd", "nnpos",
+                                                "dist16")]), "aco_get_setup_at")
+        if k == 0:
+            out["nnk"] = out["nnk"][:, :0]
+            out["nnkd"] = out["nnkd"][:, :0]
         return out
 
     def set_rng_state(self, state):

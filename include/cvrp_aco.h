@@ -130,6 +130,17 @@ int aco_get_pheromone_at(aco_ctx *ctx, int32_t instance, float *tau, float *tota
 int aco_set_pheromone_at(aco_ctx *ctx, int32_t instance, const float *tau,
                          const float *total_info);
 
+/* The arrays aco_create derived on the device from an instance's distances
+ * (init_problem -> compute_distances / compute_nn_lists, vrpHelper.cpp:160-236,
+ * and the heuristic of compute_total_info, kernel.cl:782-789): eta^beta
+ * (times the prior) n*n, the depot column n, the candidate rows n*k and their
+ * distances, the 2-opt neighbour rows n*nn_ls and their distances, nnpos
+ * n*n (position of j in i's neighbour row, 0xFFFF if absent) and the 16-bit
+ * distances n*n (filled only when every distance is an integer in
+ * [0, 65535]).  Any output may be NULL. */
+int aco_get_setup_at(aco_ctx *ctx, int32_t instance, float *heur, float *d0, int32_t *nnk, float *nnkd,
+                     int32_t *nnls, float *nnd, uint16_t *nnpos, uint16_t *dist16);
+
 /* g_ACO::init_aco (g_aco.cpp:63-93) */
 int aco_init(aco_ctx *ctx);
 /* g_ACO::run_aco_iteration (g_aco.cpp:122-134): construct -> local search
@@ -236,6 +247,12 @@ const char *aco_last_error(void);
 #define ACO_DIST_GEO    3   /* geo_distance (vrpHelper.cpp:99-136)                       */
 #define ACO_DIST_ATT    4   /* att_distance (vrpHelper.cpp:138-158)                      */
 int aco_compute_distances(int32_t n, const double *x, const double *y, int32_t mode, float *dist);
+/* The same distances for `count` instances on the current HIP device:
+ * x[b], y[b] (n[b] doubles) into dist[b] (n[b] x n[b]), host arrays.
+ * ACO_DIST_FLOAT, _NINT, _CEIL and _ATT only (equal to aco_compute_distances
+ * element for element); GEO stays on the host. */
+int aco_compute_distances_batch(int32_t count, const int32_t *n, const double *const *x,
+                                const double *const *y, int32_t mode, float *const *dist);
 /* compute_nn_lists (vrpHelper.cpp:188-236): self and depot excluded;
  * tie 0 = the reference's quicksort order, tie 1 = (distance, index) */
 int aco_compute_nn_lists(int32_t n, const float *dist, int32_t nn, int32_t tie, int32_t *out);
