@@ -1,5 +1,7 @@
 // Candidate (nearest-neighbour) lists of many instances on the GPU:
-// compute_nn_lists (vrpHelper.cpp:188-236), one thread per row.
+// compute_nn_lists (vrpHelper.cpp:188-236), one thread per row, the row
+// sorted in LDS (k_nn_lists; at most 60 KB and 64 rows per block) or,
+// for the entry point that allocates nothing, in a global scratch array.
 //
 // Each row is sorted with the reference's own steps, sort2
 // (utilities.cpp:246-267): the middle element is swapped to the front as
@@ -22,8 +24,6 @@ namespace cvrp {
 
 struct NnJob {
     const float *dist;   // n rows of pitch dpitch, row-major
-    float *v;            // n x n sort keys
-    int *ix;             // n x n node ids
     int *out;            // n x width
     int n, width, first_row, dpitch;
 };
@@ -35,7 +35,9 @@ __device__ __forceinline__ bool nn_less(float a, int ia, float b, int ib, int ti
     return a < b || (tie != 0 && a == b && ia < ib);
 }
 
-__device__ void nn_sort_row(float *v, int *ix, int n, int tie)
+// Sorts keys v[0..n) with node ids ix[0..n) (global or LDS arrays).
+template <typename I>
+__device__ void nn_sort_row(float *v, I *ix, int n, int tie)
 {
     int sl[kNnStack], sr[kNnStack];
     int sp = 0, l = 0, r = n - 1;
@@ -53,7 +55,7 @@ __device__ void nn_sort_row(float *v, int *ix, int n, int tie)
         v[mid] = v[l];
         ix[mid] = ix[l];
         v[l] = pv;
-        ix[l] = pi;
+        ix[l] = (I)pi;
         int last = l;
         for (int k = l + 1; k <= r; ++k) {
             const float vk = v[k];
@@ -63,13 +65,13 @@ __device__ void nn_sort_row(float *v, int *ix, int n, int tie)
                 v[k] = v[last];
                 ix[k] = ix[last];
                 v[last] = vk;
-                ix[last] = ik;
+                ix[last] = (I)ik;
             }
         }
         v[l] = v[last];
         ix[l] = ix[last];
         v[last] = pv;
-        ix[last] = pi;
+        ix[last] = (I)pi;
         if (last - l + 1 < r - last) {
             sl[sp] = last + 1;
             sr[sp] = r;
@@ -83,8 +85,12 @@ __device__ void nn_sort_row(float *v, int *ix, int n, int tie)
     }
 }
 
-__global__ __launch_bounds__(256) void k_nn_lists(const NnJob *jobs, int njobs, int rows, int tie)
+// One row per thread, sorted in LDS: keys as floats, node ids as 16-bit
+// values (n <= 4096), row r of the block at r * nmax.  The rows of a block
+// may belong to different instances.
+__global__ __launch_bounds__(64) void k_nn_lists(const NnJob *jobs, int njobs, int rows, int tie, int nmax)
 {
+    extern __shared__ __align__(16) unsigned char smem[];
     const int row = blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= rows) return;
     int lo = 0, hi = njobs - 1;
@@ -96,16 +102,43 @@ __global__ __launch_bounds__(256) void k_nn_lists(const NnJob *jobs, int njobs, 
     const NnJob J = jobs[lo];
     const int node = row - J.first_row, n = J.n;
     const float *d = J.dist + (size_t)node * J.dpitch;
-    float *v = J.v + (size_t)node * n;
-    int *ix = J.ix + (size_t)node * n;
+    float *v = reinterpret_cast<float *>(smem) + (size_t)threadIdx.x * nmax;
+    unsigned short *ix = reinterpret_cast<unsigned short *>(smem + (size_t)blockDim.x * nmax * 4) +
+                         (size_t)threadIdx.x * nmax;
     for (int i = 0; i < n; ++i) {
         v[i] = d[i];
-        ix[i] = i;
+        ix[i] = (unsigned short)i;
     }
     v[node] = INFINITY;   // a node is not its own neighbour
     v[0] = INFINITY;      // nor is the depot anyone's
     nn_sort_row(v, ix, n, tie);
     for (int h = 0; h < J.width; ++h) J.out[(size_t)node * J.width + h] = ix[h];
+}
+
+// Rows per block of k_nn_lists for `rows` rows of at most nmax nodes: the
+// rows of a wave run their sorts side by side (divergent), so a row gets a
+// wave of its own while there are fewer rows than SIMDs (4 per CU), and
+// rows are packed up to 64 per wave beyond that, within 60 KB of LDS per
+// block (6 bytes per row element).
+static int nn_rows_per_block(int rows, int nmax, int cus)
+{
+    const int simds = 4 * (cus > 0 ? cus : 1);
+    int r = (rows + simds - 1) / simds;
+    const int cap = (60 << 10) / (6 * nmax);
+    r = r > cap ? cap : r;
+    return r < 1 ? 1 : (r > 64 ? 64 : r);
+}
+
+static hipError_t launch_nn_lists(const NnJob *djobs, int nj, int rows, int tie, int nmax, hipStream_t st)
+{
+    int dev = 0, cus = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return e;
+    const int rpb = nn_rows_per_block(rows, nmax, cus);
+    const size_t lds = (size_t)rpb * nmax * 6;
+    hipLaunchKernelGGL(k_nn_lists, dim3((rows + rpb - 1) / rpb), dim3(rpb), lds, st, djobs, nj, rows, tie, nmax);
+    return hipGetLastError();
 }
 
 // ---- the same lists without device allocation ----
@@ -182,7 +215,8 @@ static int scratch_address(void **ptr)
     return ACO_OK;
 }
 
-// Instances are processed in chunks of at most this many matrix elements.
+// Instances are processed in chunks of at most this many matrix elements
+// (the host path stages the distance matrices on the device).
 constexpr size_t kNnChunkElems = size_t(1) << 26;
 
 int gpu_nn_lists(hipStream_t st, int count, const int *n, const float *const *dist, const int *width,
@@ -195,22 +229,21 @@ int gpu_nn_lists(hipStream_t st, int count, const int *n, const float *const *di
     while (b0 < count) {
         // one chunk: instances b0 .. b1-1
         size_t elems = 0, outs = 0;
-        int b1 = b0, rows = 0;
+        int b1 = b0, rows = 0, nmax = 0;
         while (b1 < count && (b1 == b0 || elems + (size_t)n[b1] * n[b1] <= kNnChunkElems)) {
             elems += (size_t)n[b1] * n[b1];
             outs += (size_t)n[b1] * width[b1];
             rows += n[b1];
+            nmax = n[b1] > nmax ? n[b1] : nmax;
             ++b1;
         }
         const int nj = b1 - b0;
-        const size_t bytes = elems * 12 + outs * 4 + (size_t)nj * sizeof(NnJob) + 1024;
+        const size_t bytes = elems * 4 + outs * 4 + (size_t)nj * sizeof(NnJob) + 1024;
         unsigned char *buf = nullptr;
         if (hipMalloc(&buf, bytes) != hipSuccess)
             return set_error(ACO_ERR_HIP, "nn lists: device allocation of " + std::to_string(bytes) + " bytes failed");
         float *ddist = reinterpret_cast<float *>(buf);
-        float *dv = ddist + elems;
-        int *dix = reinterpret_cast<int *>(dv + elems);
-        int *dout = dix + elems;
+        int *dout = reinterpret_cast<int *>(ddist + elems);
         NnJob *djobs = reinterpret_cast<NnJob *>(
             (reinterpret_cast<uintptr_t>(dout + outs) + 255) & ~uintptr_t(255));
         std::vector<NnJob> jobs(nj);
@@ -219,7 +252,7 @@ int gpu_nn_lists(hipStream_t st, int count, const int *n, const float *const *di
         hipError_t err = hipSuccess;
         for (int j = 0; j < nj && err == hipSuccess; ++j) {
             const int b = b0 + j;
-            jobs[j] = NnJob{ddist + eo, dv + eo, dix + eo, dout + oo, n[b], width[b], ro, n[b]};
+            jobs[j] = NnJob{ddist + eo, dout + oo, n[b], width[b], ro, n[b]};
             err = hipMemcpyAsync(ddist + eo, dist[b], (size_t)n[b] * n[b] * 4, hipMemcpyHostToDevice, st);
             eo += (size_t)n[b] * n[b];
             oo += (size_t)n[b] * width[b];
@@ -227,10 +260,7 @@ int gpu_nn_lists(hipStream_t st, int count, const int *n, const float *const *di
         }
         if (err == hipSuccess)
             err = hipMemcpyAsync(djobs, jobs.data(), sizeof(NnJob) * nj, hipMemcpyHostToDevice, st);
-        if (err == hipSuccess) {
-            hipLaunchKernelGGL(k_nn_lists, dim3((rows + 255) / 256), dim3(256), 0, st, djobs, nj, rows, tie);
-            err = hipGetLastError();
-        }
+        if (err == hipSuccess) err = launch_nn_lists(djobs, nj, rows, tie, nmax, st);
         oo = 0;
         for (int j = 0; j < nj && err == hipSuccess; ++j) {
             const int b = b0 + j;
@@ -247,50 +277,28 @@ int gpu_nn_lists(hipStream_t st, int count, const int *n, const float *const *di
 }
 
 // The same lists from distance matrices already on the device (n[b] rows
-// of pitch pitch[b]) into device arrays out[b] (n[b] x width[b]); sort
-// scratch allocated here, in chunks as above.  Enqueued on st and waited
-// for.
+// of pitch pitch[b]) into device arrays out[b] (n[b] x width[b]).  Enqueued
+// on st and waited for.
 int gpu_nn_lists_dev(hipStream_t st, int count, const int *n, const float *const *dist, const int *pitch,
                      const int *width, int tie, int *const *out)
 {
-    int b0 = 0;
-    while (b0 < count) {
-        size_t elems = 0;
-        int b1 = b0, rows = 0;
-        while (b1 < count && (b1 == b0 || elems + (size_t)n[b1] * n[b1] <= kNnChunkElems)) {
-            elems += (size_t)n[b1] * n[b1];
-            rows += n[b1];
-            ++b1;
-        }
-        const int nj = b1 - b0;
-        const size_t bytes = elems * 8 + (size_t)nj * sizeof(NnJob) + 512;
-        unsigned char *buf = nullptr;
-        if (hipMalloc(&buf, bytes) != hipSuccess)
-            return set_error(ACO_ERR_HIP, "nn lists: device allocation of " + std::to_string(bytes) + " bytes failed");
-        float *dv = reinterpret_cast<float *>(buf);
-        int *dix = reinterpret_cast<int *>(dv + elems);
-        NnJob *djobs = reinterpret_cast<NnJob *>(
-            (reinterpret_cast<uintptr_t>(dix + elems) + 255) & ~uintptr_t(255));
-        std::vector<NnJob> jobs(nj);
-        size_t eo = 0;
-        int ro = 0;
-        for (int j = 0; j < nj; ++j) {
-            const int b = b0 + j;
-            jobs[j] = NnJob{dist[b], dv + eo, dix + eo, out[b], n[b], width[b], ro, pitch[b]};
-            eo += (size_t)n[b] * n[b];
-            ro += n[b];
-        }
-        hipError_t err = hipMemcpyAsync(djobs, jobs.data(), sizeof(NnJob) * nj, hipMemcpyHostToDevice, st);
-        if (err == hipSuccess) {
-            hipLaunchKernelGGL(k_nn_lists, dim3((rows + 255) / 256), dim3(256), 0, st, djobs, nj, rows, tie);
-            err = hipGetLastError();
-        }
-        const hipError_t es = hipStreamSynchronize(st);
-        if (err == hipSuccess) err = es;
-        (void)hipFree(buf);
-        if (err != hipSuccess) return set_error(ACO_ERR_HIP, std::string("nn lists: ") + hipGetErrorString(err));
-        b0 = b1;
+    if (count == 0) return ACO_OK;
+    int rows = 0, nmax = 0;
+    std::vector<NnJob> jobs(count);
+    for (int b = 0; b < count; ++b) {
+        jobs[b] = NnJob{dist[b], out[b], n[b], width[b], rows, pitch[b]};
+        rows += n[b];
+        nmax = n[b] > nmax ? n[b] : nmax;
     }
+    NnJob *djobs = nullptr;
+    if (hipMalloc(&djobs, sizeof(NnJob) * count) != hipSuccess)
+        return set_error(ACO_ERR_HIP, "nn lists: device allocation failed");
+    hipError_t err = hipMemcpyAsync(djobs, jobs.data(), sizeof(NnJob) * count, hipMemcpyHostToDevice, st);
+    if (err == hipSuccess) err = launch_nn_lists(djobs, count, rows, tie, nmax, st);
+    const hipError_t es = hipStreamSynchronize(st);
+    if (err == hipSuccess) err = es;
+    (void)hipFree(djobs);
+    if (err != hipSuccess) return set_error(ACO_ERR_HIP, std::string("nn lists: ") + hipGetErrorString(err));
     return ACO_OK;
 }
 

@@ -61,3 +61,14 @@ def test_context_without_lists_equals_context_with_host_lists():
     for b in range(len(insts)):
         for x, y in zip(runs[0][b], runs[1][b]):
             np.testing.assert_array_equal(x, y)
+
+
+def test_batch_lists_at_the_node_limit():
+    """4096 nodes (the library's maximum; two rows per block of the LDS
+    sort) with many equal distances, next to a small instance in the same
+    launch."""
+    insts = [_tied(21, 4095, 64), _tied(22, 40, 6)]
+    widths = [it.n - 2 for it in insts]
+    got = ca.Instance.nn_lists_batch(insts, widths, 0)
+    for b, (it, w) in enumerate(zip(insts, widths)):
+        np.testing.assert_array_equal(got[b], it.nn_lists(w, 0), err_msg=f"instance {b} n={it.n}")
