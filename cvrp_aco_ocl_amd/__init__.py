@@ -14,3 +14,4 @@ from ._lib import (  # noqa: F401
 )
 from .solver import GpuACO, Instance  # noqa: F401
 from . import instances  # noqa: F401
+from . import decompose  # noqa: F401

@@ -49,6 +49,13 @@ class AcoParams(C.Structure):
     ]
 
 
+class AcoDecomposeParams(C.Structure):
+    _fields_ = [
+        ("subs", C.c_int32), ("sub_iterations", C.c_int32), ("rotation", C.c_int32),
+        ("pheromone_back", C.c_int32), ("sub_params", AcoParams),
+    ]
+
+
 class AcoPriorTrainParams(C.Structure):
     _fields_ = [
         ("epochs", C.c_int32), ("batch", C.c_int32), ("lr", C.c_float), ("lr_final", C.c_float),
@@ -152,6 +159,19 @@ _SIGNATURES = {
     "aco_prior_build": (C.c_int, [_P, C.c_int32, _P, _P, _P, C.c_int32, _P, _P, C.c_int32,
                                   C.c_int32, C.c_float, C.c_float, _P]),
     "aco_prior_apply": (C.c_int, [_P, _P, _P, _P, C.c_int32, C.c_float, C.c_float]),
+    # sub-problem decomposition (include/cvrp_decompose.h)
+    "aco_route_centers": (C.c_int, [C.c_int32, _P, _P, _P, _P, C.c_int32, C.c_int32, _P, _P, _P, _P, _P,
+                                    C.POINTER(C.c_int32)]),
+    "aco_partition_routes": (C.c_int, [C.c_int32, _P, _P, _P, _P, C.c_int32, C.c_int32, C.c_int32, _P, _P,
+                                       _P, _P, C.POINTER(C.c_int32)]),
+    "aco_sub_problem": (C.c_int, [C.c_int32, _P, _P, C.c_int32, _P, _P, _P]),
+    "aco_merge_sub_tours": (C.c_int, [C.c_int32, _P, C.c_int32, _P, _P, _P, _P, _P, _P, _P, C.c_int32,
+                                      C.POINTER(C.c_int32), C.POINTER(C.c_float), _P]),
+    "aco_set_ant_at": (C.c_int, [_P, C.c_int32, C.c_int32, _P, C.c_int32]),
+    "aco_pheromone_from_subs": (C.c_int, [_P, C.c_int32, _P, _P, _P, _P]),
+    "aco_decompose_round": (C.c_int, [_P, C.c_int32, C.POINTER(AcoProblem), _P, _P,
+                                      C.POINTER(AcoDecomposeParams), _P, C.c_int32, C.POINTER(C.c_int32),
+                                      C.POINTER(C.c_float), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
 }
 
 EXPORTED = tuple(_SIGNATURES)

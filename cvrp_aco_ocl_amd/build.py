@@ -33,7 +33,7 @@ DEVICE_FLAGS = [
 ]
 
 LIB_SOURCES = ["aco_kernels.hip", "aco_local_search.hip", "aco_prior.hip", "aco_nnlist.hip",
-               "aco_setup.hip", "aco_capi.cpp", "aco_host.cpp"]
+               "aco_setup.hip", "aco_capi.cpp", "aco_host.cpp", "aco_decompose.cpp"]
 CLI_SOURCES = ["g_aco.cpp", "main.cpp"]
 
 

@@ -16,6 +16,7 @@
 
 #include "aco_device.h"
 #include "aco_internal.h"
+#include "cvrp_decompose.h"
 
 namespace cvrp {
 
@@ -949,6 +950,77 @@ int aco_set_ants_at(aco_ctx *c, int32_t b, const int32_t *tours, int32_t stride,
 int aco_set_ants(aco_ctx *c, const int32_t *tours, int32_t stride, const int32_t *sizes, const float *lens)
 {
     return aco_set_ants_at(c, 0, tours, stride, sizes, lens);
+}
+
+int aco_set_ant_at(aco_ctx *c, int32_t b, int32_t ant, const int32_t *tour, int32_t size)
+{
+    int rc = need_inst(c, b, false);
+    if (rc) return rc;
+    const InstanceData &I = c->inst[b];
+    if (ant < 0 || ant >= c->m) return set_error(ACO_ERR_ARG, "aco_set_ant_at: ant out of range");
+    if (!tour || size < 3 || size > I.T) return set_error(ACO_ERR_ARG, "aco_set_ant_at: tour size out of range");
+    if (tour[0] != 0 || tour[size - 1] != 0) return set_error(ACO_ERR_ARG, "aco_set_ant_at: tour must start and end at the depot");
+    for (int q = 0; q < size; ++q)
+        if (tour[q] < 0 || tour[q] >= I.n) return set_error(ACO_ERR_ARG, "aco_set_ant_at: tour node out of range");
+    HIP_TRY(hipMemcpyAsync(I.ds.tours + (size_t)ant * I.T, tour, (size_t)size * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(I.ds.sizes + ant, &size, 4, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(launch_ant_length(I.dp, I.ds, ant, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return ACO_OK;
+}
+
+int aco_pheromone_from_subs(aco_ctx *c, int32_t b, aco_ctx *sc, const int32_t *offsets, const int32_t *real_nodes,
+                            const int32_t *map)
+{
+    int rc = need_inst(c, b, true);
+    if (rc) return rc;
+    if (!sc || !offsets || !real_nodes || !map) return set_error(ACO_ERR_ARG, "aco_pheromone_from_subs: null argument");
+    if (!sc->inited) return set_error(ACO_ERR_STATE, "aco_pheromone_from_subs: sub-colonies not initialised");
+    if (sc->device != c->device) return set_error(ACO_ERR_ARG, "aco_pheromone_from_subs: contexts on different devices");
+    const InstanceData &M = c->inst[b];
+    const int count = (int)sc->inst.size();
+    std::vector<SubTauJob> jobs(count);
+    std::vector<int32_t> reals;
+    std::vector<char> seen(M.n, 0);
+    int rows = 0;
+    for (int s = 0; s < count; ++s) {
+        const InstanceData &S = sc->inst[s];
+        const int g = map[s];
+        if (g < 0 || offsets[g + 1] - offsets[g] != S.n)
+            return set_error(ACO_ERR_ARG, "aco_pheromone_from_subs: sub-problem " + std::to_string(s) +
+                                              " does not match its node list");
+        const int32_t *r = real_nodes + offsets[g];
+        if (r[0] != 0) return set_error(ACO_ERR_ARG, "aco_pheromone_from_subs: node lists start at the depot");
+        for (int i = 1; i < S.n; ++i) {
+            if (r[i] <= 0 || r[i] >= M.n || seen[r[i]])
+                return set_error(ACO_ERR_ARG, "aco_pheromone_from_subs: customer out of range or in two sub-problems");
+            seen[r[i]] = 1;
+        }
+        jobs[s] = SubTauJob{S.ds.tau, nullptr, S.n, S.P, rows};
+        rows += S.n;
+        reals.insert(reals.end(), r, r + S.n);
+    }
+    void *buf = nullptr;
+    const size_t jb = (sizeof(SubTauJob) * count + 255) & ~size_t(255);
+    HIP_TRY(hipMalloc(&buf, jb + reals.size() * 4));
+    int32_t *dreal = reinterpret_cast<int32_t *>(static_cast<unsigned char *>(buf) + jb);
+    size_t ro = 0;
+    for (int s = 0; s < count; ++s) {
+        jobs[s].real = dreal + ro;
+        ro += (size_t)jobs[s].n;
+    }
+    hipError_t e = hipMemcpyAsync(buf, jobs.data(), sizeof(SubTauJob) * count, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dreal, reals.data(), reals.size() * 4, hipMemcpyHostToDevice, c->stream);
+    // the sub-colonies' last launches were on their own stream
+    if (e == hipSuccess) e = hipStreamSynchronize(sc->stream);
+    if (e == hipSuccess)
+        e = launch_tau_from_subs(static_cast<const SubTauJob *>(buf), count, rows, M.dp, M.ds, c->stream);
+    if (e == hipSuccess && c->bt.max_nk > 0) e = launch_gather(c->bt, c->stream);
+    const hipError_t es = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = es;
+    (void)hipFree(buf);
+    if (e != hipSuccess) return set_error(ACO_ERR_HIP, std::string("aco_pheromone_from_subs: ") + hipGetErrorString(e));
+    return ACO_OK;
 }
 
 int aco_get_bsf_at(aco_ctx *c, int32_t b, aco_bsf_record *rec, int32_t cap, int32_t *count)

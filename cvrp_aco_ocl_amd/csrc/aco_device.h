@@ -256,6 +256,16 @@ int gpu_nn_lists_dev(hipStream_t st, int count, const int *n, const float *const
 // candidate and neighbour tables from the lists nnl (n x width)
 hipError_t launch_setup_rows(const DevProblem &p, int beta_code, int has_prior, int int16, hipStream_t st);
 hipError_t launch_setup_nn(const DevProblem &p, const int *nnl, int width, hipStream_t st);
+// sub-colony pheromone into a master instance (rows of all jobs in one launch)
+struct SubTauJob {
+    const float *tau;   // sub tau, n rows of pitch P
+    const int *real;    // n master node ids, real[0] = depot
+    int n, P, first_row;
+};
+hipError_t launch_tau_from_subs(const SubTauJob *jobs, int njobs, int rows, const DevProblem &master,
+                                const DevState &ms, hipStream_t st);
+// lens[ant] of an instance = its tour's edges added in step order
+hipError_t launch_ant_length(const DevProblem &p, const DevState &s, int ant, hipStream_t st);
 hipError_t launch_statistics(const DevBatch &b, float tick_to_s, hipStream_t st);
 hipError_t launch_pheromone_update(const DevBatch &b, hipStream_t st);
 hipError_t launch_gather(const DevBatch &b, hipStream_t st);

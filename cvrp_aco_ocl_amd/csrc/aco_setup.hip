@@ -11,6 +11,9 @@
 //                 of j in i's neighbour row, 0xFFFF if absent)
 //   k_distances   distance matrices from coordinates (aco_compute_distances
 //                 for the EUC_2D, nint, CEIL_2D and ATT conventions)
+//   k_tau_from_subs, k_ant_length
+//                 write-backs of the sub-problem decomposition: sub-colony
+//                 pheromone into the master, the length of an injected ant
 //
 // Each value is the same float expression the host preparation and the
 // CPU oracle evaluate (no contraction, IEEE division), so the arrays are
@@ -82,6 +85,60 @@ hipError_t launch_setup_rows(const DevProblem &p, int beta_code, int has_prior, 
 hipError_t launch_setup_nn(const DevProblem &p, const int *nnl, int width, hipStream_t st)
 {
     hipLaunchKernelGGL(k_setup_nn, dim3(p.n), dim3(256), 0, st, p, nnl, width);
+    return hipGetLastError();
+}
+
+// ------------------------------------------- decomposition write-backs ----
+
+// Pheromone of finished sub-colonies into the master instance: one block per
+// sub-problem row i, tau[r_i][r_j] = sub tau[i][j] (j != i) and total_info
+// there as the pheromone pass computes it, tau^alpha * heur.
+__global__ __launch_bounds__(256) void k_tau_from_subs(const SubTauJob *jobs, int njobs, DevProblem p, DevState s)
+{
+    const int row = blockIdx.x;
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (jobs[mid].first_row <= row) lo = mid;
+        else hi = mid - 1;
+    }
+    const SubTauJob J = jobs[lo];
+    const int i = row - J.first_row;
+    const size_t base = (size_t)J.real[i] * p.P;
+    const float *src = J.tau + (size_t)i * J.P;
+    for (int j = threadIdx.x; j < J.n; j += 256) {
+        if (j == i) continue;
+        const size_t e = base + J.real[j];
+        const float t = src[j];
+        const float ta = p.alpha == 1.0f ? t : powf(t, p.alpha);
+        s.tau[e] = t;
+        s.tinfo[e] = ta * p.heur[e];
+    }
+}
+
+hipError_t launch_tau_from_subs(const SubTauJob *jobs, int njobs, int rows, const DevProblem &master,
+                                const DevState &ms, hipStream_t st)
+{
+    if (rows <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_tau_from_subs, dim3(rows), dim3(256), 0, st, jobs, njobs, master, ms);
+    return hipGetLastError();
+}
+
+// Step-order length of one ant's tour (one thread: the additions are
+// sequential, as in construction).
+__global__ void k_ant_length(DevProblem p, DevState s, int ant)
+{
+    if (threadIdx.x != 0) return;
+    const int *t = s.tours + (size_t)ant * p.T;
+    const int size = s.sizes[ant];
+    float len = 0.0f;
+    for (int q = 1; q < size; ++q) len = len + p.dist[(size_t)t[q - 1] * p.P + t[q]];
+    s.lens[ant] = len;
+}
+
+hipError_t launch_ant_length(const DevProblem &p, const DevState &s, int ant, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_ant_length, dim3(1), dim3(64), 0, st, p, s, ant);
     return hipGetLastError();
 }
 

@@ -197,6 +197,21 @@ class GpuACO:
         self.m = n_ants if n_ants > 0 else insts[0].n
         self.T = 2 * insts[0].n
 
+    # -- sub-problem decomposition (include/cvrp_decompose.h) ---------------
+    def set_ant(self, ant: int, tour, b: int = 0):
+        """aco_set_ant_at: ant `ant` becomes `tour` (length in step order)."""
+        t = np.ascontiguousarray(tour, np.int32)
+        check(lib.aco_set_ant_at(self.h, b, ant, _ptr(t), len(t)), "aco_set_ant_at")
+
+    def pheromone_from_subs(self, subs: "GpuACO", parts, mapping, b: int = 0):
+        """aco_pheromone_from_subs: instance s of ``subs`` holds the nodes
+        parts[mapping[s]][0] of instance b."""
+        off = np.cumsum([0] + [len(r) for r, _ in parts]).astype(np.int32)
+        real = np.ascontiguousarray(np.concatenate([r for r, _ in parts]), np.int32)
+        mp = np.ascontiguousarray(mapping, np.int32)
+        check(lib.aco_pheromone_from_subs(self.h, b, subs.h, _ptr(off), _ptr(real), _ptr(mp)),
+              "aco_pheromone_from_subs")
+
     # -- lifetime ----------------------------------------------------------
     def close(self):
         if getattr(self, "h", None):

@@ -95,6 +95,7 @@ def _load():
         "orc_colony_pheromone_update": (None, [C.c_void_p]),
         "orc_colony_iteration": (None, [C.c_void_p]),
         "orc_colony_iter": (C.c_int, [C.c_void_p]),
+        "orc_colony_next_iteration": (None, [C.c_void_p]),
         "orc_colony_best": (C.c_float, [C.c_void_p, C.c_void_p, C.POINTER(C.c_int)]),
         "orc_colony_ants": (C.c_int, [C.c_void_p, _i32p, C.c_int, _i32p, _f32p]),
         "orc_colony_bsf": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
@@ -107,6 +108,15 @@ def _load():
         "orc_colony_set_ls_seed": (None, [C.c_void_p, C.c_int32]),
         "orc_colony_set_pm_state": (None, [C.c_void_p, _i32p]),
         "orc_colony_pm_state": (None, [C.c_void_p, _i32p]),
+        "orc_route_centers": (C.c_int, [C.c_int, _f32p, _f32p, _i32p, _i32p, C.c_int, C.c_int,
+                                        _i32p, _i32p, _f32p, _f32p, _i32p]),
+        "orc_partition_routes": (C.c_int, [C.c_int, _f32p, _f32p, _i32p, _i32p, C.c_int, C.c_int,
+                                           C.c_int, _i32p, _i32p, _i32p, _i32p]),
+        "orc_sub_problem": (None, [C.c_int, _f32p, _i32p, C.c_int, _i32p, _f32p, _i32p]),
+        "orc_merge_sub_tours": (C.c_int, [C.c_int, _f32p, C.c_int, _i32p, _i32p, _i32p, _i32p,
+                                          _i32p, _i32p, _i32p, C.POINTER(C.c_float), _i32p]),
+        "orc_pheromone_from_sub": (None, [C.c_int, C.c_float, _f32p, _f32p, _f32p, C.c_int,
+                                          _i32p, _f32p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -370,6 +380,10 @@ class Colony:
     def iteration(self):
         lib.orc_colony_iteration(self.h)
 
+    def next_iteration(self):
+        """The caller's iteration++ after the four phases called one by one."""
+        lib.orc_colony_next_iteration(self.h)
+
     @property
     def iter(self):
         return lib.orc_colony_iter(self.h)
@@ -433,6 +447,80 @@ class Colony:
 
 
 # ----------------------------------------------------------------------------
+# sub-problem decomposition
+# ----------------------------------------------------------------------------
+
+def _f32(a):
+    return np.ascontiguousarray(a, np.float32)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, np.int32)
+
+
+def route_centers(x, y, dem, tour):
+    """orc_route_centers: (beg, end, cx, cy, demand) per route."""
+    x, y, dem, tour = _f32(x), _f32(y), _i32(dem), _i32(tour)
+    cap = len(tour)
+    beg, end, rdem = (np.empty(cap, np.int32) for _ in range(3))
+    cx, cy = np.empty(cap, np.float32), np.empty(cap, np.float32)
+    R = lib.orc_route_centers(len(x), x, y, dem, tour, len(tour), cap, beg, end, cx, cy, rdem)
+    assert R >= 0
+    return beg[:R], end[:R], cx[:R], cy[:R], rdem[:R]
+
+
+def partition_routes(x, y, dem, tour, subs=0, rotation=0):
+    """orc_partition_routes: list of (real_nodes, sub_tour) per sub-problem."""
+    x, y, dem, tour = _f32(x), _f32(y), _i32(dem), _i32(tour)
+    n, size = len(x), len(tour)
+    off = np.empty(size + 2, np.int32)
+    toff = np.empty(size + 2, np.int32)
+    real = np.empty(n + size, np.int32)
+    st = np.empty(2 * size + 2, np.int32)
+    G = lib.orc_partition_routes(n, x, y, dem, tour, size, subs, rotation, off, real, toff, st)
+    assert G >= 1
+    return [(real[off[g]:off[g + 1]].copy(), st[toff[g]:toff[g + 1]].copy()) for g in range(G)]
+
+
+def sub_problem(dist, dem, real_nodes):
+    """orc_sub_problem: (sub_dist, sub_dem)."""
+    dist, dem, real = _f32(dist), _i32(dem), _i32(real_nodes)
+    sn = len(real)
+    sd = np.empty((sn, sn), np.float32)
+    sdem = np.empty(sn, np.int32)
+    lib.orc_sub_problem(len(dem), dist, dem, sn, real, sd, sdem)
+    return sd, sdem
+
+
+def merge_sub_tours(dist, parts, bests):
+    """orc_merge_sub_tours: parts = [(real_nodes, sub_tour)], bests = [best
+    tour of each sub (local numbering) or an empty array]; returns (merged
+    tour, length, replaced flags)."""
+    dist = _f32(dist)
+    n = dist.shape[0]
+    off = np.cumsum([0] + [len(r) for r, _ in parts]).astype(np.int32)
+    toff = np.cumsum([0] + [len(t) for _, t in parts]).astype(np.int32)
+    boff = np.cumsum([0] + [len(b) for b in bests]).astype(np.int32)
+    real = _i32(np.concatenate([r for r, _ in parts]))
+    st = _i32(np.concatenate([t for _, t in parts]))
+    bt = _i32(np.concatenate([np.asarray(b, np.int32) for b in bests] + [np.zeros(1, np.int32)]))
+    out = np.empty(2 * n + 2, np.int32)
+    ln = C.c_float()
+    rep = np.empty(len(parts), np.int32)
+    sz = lib.orc_merge_sub_tours(n, dist, len(parts), off, real, toff, st, boff, bt, out, C.byref(ln), rep)
+    return out[:sz].copy(), ln.value, rep
+
+
+def pheromone_from_sub(alpha, heur, tau, tinfo, real_nodes, sub_tau):
+    """orc_pheromone_from_sub, in place on tau / tinfo (float32 n x n)."""
+    assert tau.dtype == np.float32 and tau.flags.c_contiguous
+    assert tinfo.dtype == np.float32 and tinfo.flags.c_contiguous
+    real = _i32(real_nodes)
+    lib.orc_pheromone_from_sub(tau.shape[0], alpha, _f32(heur), tau, tinfo, len(real), real,
+                               _f32(sub_tau))
+
+
+# ----------------------------------------------------------------------------
 # reference CPU build (oracle/_ref), present where /root/reference was built
 # ----------------------------------------------------------------------------
 
@@ -456,4 +544,8 @@ def load_ref():
     r.ref_read_instance.argtypes = [C.c_char_p, C.c_int, _f32p, _f32p, _i32p,
                                     C.POINTER(C.c_int), C.POINTER(C.c_float),
                                     C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    r.ref_route_centers.argtypes = [C.c_int, _f32p, _f32p, _i32p, _i32p, C.c_int, _i32p, _i32p,
+                                    _f32p, _f32p, _i32p]
+    r.ref_sub_problem.argtypes = [C.c_int, _f32p, _f32p, _i32p, C.c_int, C.c_float, C.c_float, _f32p,
+                                  C.c_int, _i32p, _f32p, _i32p, C.c_void_p]
     return r

@@ -1045,6 +1045,11 @@ void orc_colony_iteration(orc_colony *c)
     c->iteration++;
 }
 
+/* The caller's inst->iteration++ after an iteration (main.cpp:80-88), for
+ * replays that call the four phases one by one (orc_colony_iteration does
+ * it itself). */
+void orc_colony_next_iteration(orc_colony *c) { c->iteration++; }
+
 int orc_colony_iter(const orc_colony *c) { return c->iteration; }
 
 float orc_colony_best(const orc_colony *c, int *tour, int *size)
@@ -1108,4 +1113,170 @@ long long orc_colony_ls_evals(const orc_colony *c) { return c->ls_evals; }
 void orc_colony_pm_state(const orc_colony *c, int32_t *out)
 {
     memcpy(out, c->pm, sizeof(int32_t) * (size_t)c->m * ORC_LANES);
+}
+
+/* =========================================================================
+ * Sub-problem decomposition (vrpHelper.cpp:274-294, problem.cpp:84-130,
+ * problem.h:136-139; partition and merge rules have no reference
+ * counterpart, see oracle.h)
+ * ========================================================================= */
+
+int orc_route_centers(int n, const float *x, const float *y, const int *dem, const int *tour, int size,
+                      int cap, int *beg, int *end, float *cx, float *cy, int *rdem)
+{
+    (void)n;
+    int R = 0, p = 0;
+    for (int q = 1; q < size; ++q) {
+        if (tour[q] != 0) continue;
+        if (q - p >= 2) {
+            if (R >= cap) return -1;
+            int d = 0;
+            for (int j = p; j < q; ++j) d += dem[tour[j]];
+            float fx = 0.0f, fy = 0.0f;
+            for (int j = p; j < q; ++j) {
+                const int t = tour[j];
+                fx = (float)((double)fx + (double)(x[t] * (float)dem[t]) * 1.0 / (double)d);
+                fy = (float)((double)fy + (double)(y[t] * (float)dem[t]) * 1.0 / (double)d);
+            }
+            beg[R] = p;
+            end[R] = q;
+            cx[R] = fx;
+            cy[R] = fy;
+            rdem[R] = d;
+            ++R;
+        }
+        p = q;
+    }
+    return R;
+}
+
+typedef struct {
+    double ang;
+    int r;
+} orc_route_key;
+
+static int route_key_cmp(const void *a, const void *b)
+{
+    const orc_route_key *u = (const orc_route_key *)a, *v = (const orc_route_key *)b;
+    if (u->ang < v->ang) return -1;
+    if (u->ang > v->ang) return 1;
+    return (u->r > v->r) - (u->r < v->r);
+}
+
+int orc_partition_routes(int n, const float *x, const float *y, const int *dem, const int *tour, int size,
+                         int subs, int rotation, int *offsets, int *real_nodes, int *toffsets, int *sub_tours)
+{
+    const int cap = size;
+    int *beg = (int *)malloc(sizeof(int) * cap * 3);
+    int *end = beg + cap, *rdem = end + cap;
+    float *cx = (float *)malloc(sizeof(float) * cap * 2), *cy = cx + cap;
+    orc_route_key *key = (orc_route_key *)malloc(sizeof(orc_route_key) * cap);
+    int *group = (int *)malloc(sizeof(int) * cap);
+    const int R = orc_route_centers(n, x, y, dem, tour, size, cap, beg, end, cx, cy, rdem);
+    if (R < 1) {
+        free(beg); free(cx); free(key); free(group);
+        return -1;
+    }
+    for (int r = 0; r < R; ++r) {
+        double px = cx[r], py = cy[r];
+        if (rdem[r] == 0) {
+            px = 0.0;
+            py = 0.0;
+            for (int j = beg[r] + 1; j < end[r]; ++j) {
+                px += x[tour[j]];
+                py += y[tour[j]];
+            }
+            px /= (double)(end[r] - beg[r] - 1);
+            py /= (double)(end[r] - beg[r] - 1);
+        }
+        key[r].ang = atan2(py - (double)y[0], px - (double)x[0]);
+        key[r].r = r;
+    }
+    qsort(key, R, sizeof(orc_route_key), route_key_cmp);
+    int G = subs > 0 ? subs : n / 50;
+    if (G < 1) G = 1;
+    if (G > R) G = R;
+    int total = 0;
+    for (int r = 0; r < R; ++r) total += end[r] - beg[r] - 1;
+    const int start = ((rotation % R) + R) % R;
+    int g = 0, acc = 0;
+    for (int q = 0; q < R; ++q) {
+        const int r = key[(start + q) % R].r;
+        group[q] = g;
+        acc += end[r] - beg[r] - 1;
+        const int left_routes = R - q - 1, left_groups = G - 1 - g;
+        if (left_groups > 0 && ((long long)acc * G >= (long long)(g + 1) * total || left_routes == left_groups)) ++g;
+    }
+    int no = 0, to = 0, q = 0;
+    for (int gg = 0; gg < G; ++gg) {
+        offsets[gg] = no;
+        toffsets[gg] = to;
+        real_nodes[no++] = 0;
+        sub_tours[to++] = 0;
+        int local = 1;
+        for (; q < R && group[q] == gg; ++q) {
+            const int r = key[(start + q) % R].r;
+            for (int j = beg[r] + 1; j < end[r]; ++j) {
+                real_nodes[no++] = tour[j];
+                sub_tours[to++] = local++;
+            }
+            sub_tours[to++] = 0;
+        }
+    }
+    offsets[G] = no;
+    toffsets[G] = to;
+    free(beg); free(cx); free(key); free(group);
+    return G;
+}
+
+void orc_sub_problem(int n, const float *dist, const int *dem, int sub_n, const int *real_nodes,
+                     float *sub_dist, int *sub_dem)
+{
+    for (int i = 0; i < sub_n; ++i) {
+        const int ri = real_nodes[i];
+        for (int j = 0; j < sub_n; ++j) sub_dist[(size_t)i * sub_n + j] = dist[(size_t)ri * n + real_nodes[j]];
+        sub_dem[i] = dem[ri];
+    }
+}
+
+static float mapped_length(int n, const float *dist, const int *real, const int *t, int size)
+{
+    float len = 0.0f;
+    for (int q = 0; q + 1 < size; ++q) len += dist[(size_t)real[t[q]] * n + real[t[q + 1]]];
+    return len;
+}
+
+int orc_merge_sub_tours(int n, const float *dist, int count, const int *offsets, const int *real_nodes,
+                        const int *toffsets, const int *sub_tours, const int *boffsets, const int *best_tours,
+                        int *out, float *len, int *replaced)
+{
+    int o = 0;
+    out[o++] = 0;
+    for (int s = 0; s < count; ++s) {
+        const int *real = real_nodes + offsets[s];
+        const int *orig = sub_tours + toffsets[s];
+        const int osz = toffsets[s + 1] - toffsets[s];
+        const int *best = best_tours + boffsets[s];
+        const int bsz = boffsets[s + 1] - boffsets[s];
+        const int take = bsz > 0 && mapped_length(n, dist, real, best, bsz) < mapped_length(n, dist, real, orig, osz);
+        replaced[s] = take;
+        const int *t = take ? best : orig;
+        const int sz = take ? bsz : osz;
+        for (int q = 1; q < sz; ++q) out[o++] = real[t[q]];
+    }
+    *len = orc_tour_length(n, dist, out, o);
+    return o;
+}
+
+void orc_pheromone_from_sub(int n, float alpha, const float *heur, float *tau, float *tinfo, int sub_n,
+                            const int *real_nodes, const float *sub_tau)
+{
+    for (int i = 0; i < sub_n; ++i)
+        for (int j = 0; j < sub_n; ++j) {
+            if (i == j) continue;
+            const size_t e = (size_t)real_nodes[i] * n + real_nodes[j];
+            const float t = sub_tau[(size_t)i * sub_n + j];
+            tau[e] = t;
+            tinfo[e] = total_info(t, alpha, heur[e]);
+        }
 }

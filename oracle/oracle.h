@@ -144,6 +144,7 @@ void  orc_colony_local_search(orc_colony *c);
 void  orc_colony_statistics(orc_colony *c);
 void  orc_colony_pheromone_update(orc_colony *c);
 void  orc_colony_iteration(orc_colony *c);
+void orc_colony_next_iteration(orc_colony *c);   /* caller's iteration++ (main.cpp:80-88) */
 int   orc_colony_iter(const orc_colony *c);
 float orc_colony_best(const orc_colony *c, int *tour, int *size);
 int   orc_colony_ants(const orc_colony *c, int *tours, int stride, int *sizes, float *lens);
@@ -161,6 +162,50 @@ void  orc_colony_set_ls_seed(orc_colony *c, int32_t seed);
 /* m*64 Park-Miller lane states */
 void  orc_colony_set_pm_state(orc_colony *c, const int32_t *state);
 void  orc_colony_pm_state(const orc_colony *c, int32_t *out);
+
+
+/* ---------------- sub-problem decomposition ---------------- */
+/* compute_route_centers (vrpHelper.cpp:274-294): per route r of a
+ * depot-separated tour (routes without customers skipped), beg[r] / end[r]
+ * = positions of its opening and closing depot, the summed demand and the
+ * demand-weighted centre (each term x*d*1.0/D in double, accumulated into a
+ * float as the reference's Point).  Returns the number of routes, -1 if cap
+ * is too small. */
+int orc_route_centers(int n, const float *x, const float *y, const int *dem, const int *tour, int size,
+                      int cap, int *beg, int *end, float *cx, float *cy, int *rdem);
+/* No reference counterpart (the reference keeps num_subs = n/50 and a
+ * per-sub real_nodes list, problem.cpp:218, problem.h:136-139, but never
+ * partitions): routes ordered by the angle of their centre about the
+ * depot (atan2 in double; a route of zero demand uses the plain mean of
+ * its customers), ties by route order; cut circularly from position
+ * rotation mod R into G = clamp(subs > 0 ? subs : n/50, 1, R) contiguous
+ * groups: a group closes after a route once its running customer count
+ * times G reaches (g+1) times the total, or when exactly one route per
+ * remaining group is left.  Group g: real_nodes[offsets[g] .. offsets[g+1])
+ * = depot, then its routes' customers (routes in cut order, customers in
+ * tour order); its routes as a tour of the sub-problem (local numbering)
+ * in sub_tours[toffsets[g] .. toffsets[g+1]).  Returns G, -1 on bad
+ * input. */
+int orc_partition_routes(int n, const float *x, const float *y, const int *dem, const int *tour, int size,
+                         int subs, int rotation, int *offsets, int *real_nodes, int *toffsets, int *sub_tours);
+/* init_sub_problem (problem.cpp:84-130): sub distances and demands
+ * gathered from the master's through real_nodes. */
+void orc_sub_problem(int n, const float *dist, const int *dem, int sub_n, const int *real_nodes,
+                     float *sub_dist, int *sub_dem);
+/* No reference counterpart: sub s's best tour replaces its original routes
+ * when its length (in step order over the sub's distances) is strictly
+ * smaller than theirs; the merged tour is the groups in order, each the
+ * chosen tour mapped to master numbering, depot shared between groups.
+ * Returns the merged size; *len in step order; replaced[s] 0/1. */
+int orc_merge_sub_tours(int n, const float *dist, int count, const int *offsets, const int *real_nodes,
+                        const int *toffsets, const int *sub_tours, const int *boffsets, const int *best_tours,
+                        int *out, float *len, int *replaced);
+/* Pheromone of a finished sub-colony into the master (the use problem.h:138-139
+ * gives best_pheromone; no code of the reference does it): for i != j of
+ * the sub, tau[r_i][r_j] = sub_tau[i][j], total_info recomputed there as
+ * compute_total_info (kernel.cl:782-789) does: tau^alpha * heur. */
+void orc_pheromone_from_sub(int n, float alpha, const float *heur, float *tau, float *tinfo, int sub_n,
+                            const int *real_nodes, const float *sub_tau);
 
 #ifdef __cplusplus
 }

@@ -164,4 +164,59 @@ int ref_read_instance(const char *path, int cap, float *x, float *y, int *dem, i
     return n;
 }
 
+
+// compute_route_centers (vrpHelper.cpp:274-294) on routes r = 0..R-1
+// spanning tour[beg[r]] .. tour[end[r]]; only nodeptr is read.
+void ref_route_centers(int n, const float *x, const float *y, const int *dem, const int *tour, int R,
+                       const int *beg, const int *end, float *cx, float *cy, int *rdem)
+{
+    Problem *p = new Problem(0);
+    p->num_node = n;
+    p->nodeptr = (Point *)malloc(sizeof(Point) * n);
+    for (int i = 0; i < n; ++i) {
+        p->nodeptr[i].x = x[i];
+        p->nodeptr[i].y = y[i];
+        p->nodeptr[i].demand = dem[i];
+    }
+    vector<RouteCenter *> centers;
+    for (int r = 0; r < R; ++r) {
+        RouteCenter *c = new RouteCenter();
+        c->beg = beg[r];
+        c->end = end[r];
+        centers.push_back(c);
+    }
+    compute_route_centers(p, const_cast<int *>(tour), centers);
+    for (int r = 0; r < R; ++r) {
+        cx[r] = centers[r]->coord->x;
+        cy[r] = centers[r]->coord->y;
+        rdem[r] = centers[r]->coord->demand;
+        delete centers[r]->coord;
+        delete centers[r];
+    }
+    free(p->nodeptr);
+    delete p;
+}
+
+// init_sub_problem (problem.cpp:84-130): the sub-problem of the master
+// instance (x, y, dem, Q, D, svc, dist n*n) on real_nodes[0..sub_n); its
+// distances, demands and neighbour lists (width sub_n-2, compute_nn_lists
+// on the gathered distances).
+void ref_sub_problem(int n, const float *x, const float *y, const int *dem, int Q, float D, float svc,
+                     const float *dist, int sub_n, const int *real_nodes, float *sub_dist, int *sub_dem,
+                     int *sub_nn)
+{
+    Problem *master = make_problem(n, x, y, dem, Q, D, svc, dist);
+    Problem *sub = new Problem(1);
+    sub->num_node = sub_n;
+    for (int i = 0; i < sub_n; ++i) sub->real_nodes.push_back(real_nodes[i]);
+    init_sub_problem(master, sub);
+    const int nn = sub_n - 2;
+    for (int i = 0; i < sub_n; ++i) {
+        for (int j = 0; j < sub_n; ++j) sub_dist[(size_t)i * sub_n + j] = sub->distance[i][j];
+        sub_dem[i] = sub->nodeptr[i].demand;
+        if (sub_nn)
+            for (int j = 0; j < nn; ++j) sub_nn[(size_t)i * nn + j] = sub->nn_list[i][j];
+    }
+}
+
 }  // extern "C"

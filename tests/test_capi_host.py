@@ -3,6 +3,7 @@ include/cvrp_aco.h declares; host-side preparation equals the oracle and
 the reference build; invalid arguments are rejected before any device
 work."""
 import ctypes as C
+import glob
 import math
 import os
 import re
@@ -17,7 +18,7 @@ from cvrp_aco_ocl_amd import _lib, instances as I
 from helpers import VRPLIB
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADERS = [os.path.join(ROOT, "include", h) for h in ("cvrp_aco.h", "cvrp_prior.h")]
+HEADERS = sorted(glob.glob(os.path.join(ROOT, "include", "*.h")))
 
 
 def _declared():
