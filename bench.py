@@ -17,8 +17,10 @@ best-so-far trajectory of a 2000-iteration run (time to a 2% gap), the same
 workload under the reference's rules (I-Roulette over the full row), the
 CPU oracle on this host (reference rules on the CPU share, on one thread,
 and the candidate-roulette comparator), and the other configurations as
-extra keys: the 481-node clustered instance, 256 batched instances, and the
-learned prior against plain eta on 64 batched instances.
+extra keys: the 481-node clustered instance, 256 batched instances, the
+learned prior against plain eta on 64 batched instances, and a 3001-node
+instance (past the local search's LDS limit) improved by sub-problem
+decomposition rounds.
 """
 from __future__ import annotations
 
@@ -70,6 +72,9 @@ def parse():
                     help="plain iterations that fix the reference cost C*")
     ap.add_argument("--batch-instances", type=int, default=256,
                     help="instances of the batched leg (0 = skip)")
+    ap.add_argument("--decompose-iterations", type=int, default=10,
+                    help="3001-node decomposition leg: master iterations, a round after each "
+                         "(0 = skip)")
     ap.add_argument("--mlp-probe", type=int, default=0, metavar="INSTANCES",
                     help="only run the learned prior's MLP on that many evaluation instances")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -380,6 +385,40 @@ def leg_clustered(ca, instances, args):
     return out
 
 
+def leg_decompose(ca, instances, args):
+    """3001 nodes, past the local-search LDS limit: master colony without
+    local search (256 ants, k=16) against the same master with a
+    decomposition round after every iteration (n/50 sub-problems capped by
+    the route count, 75 iterations of 64 ants with 2-opt+swap+relocate per
+    round, all sub-colonies of a round in one batched context)."""
+    from cvrp_aco_ocl_amd import decompose as dc
+    inst = instances.clustered(args.seed + 11, customers=3000, nclust=12, sigma=40.0, routes=40)
+    it = args.decompose_iterations
+    master = dict(n_ants=256, k=16, ls_flags=0, seed=args.seed + 4)
+    t0 = time.perf_counter()
+    with ca.GpuACO(inst, selection=ca.SEL_KNN, rng=ca.RNG_PHILOX, **master) as g:
+        g.init_aco()
+        g.run(it)
+        g.synchronize()
+        plain_len = g.best()[1]
+    plain_wall = time.perf_counter() - t0
+    res = dc.solve_decomposed(inst, it, sub_iterations=75, master=master, sub=dict(n_ants=64, k=16,
+                                                                                   seed=args.seed + 6))
+    rounds = res["rounds"]
+    out = {"workload": "3001 nodes: 3000 customers in 12 Gaussian clusters (sigma 40), Q for 40 "
+                       "routes; master 256 ants k=16 without local search (the local search's LDS "
+                       "rows do not fit), a decomposition round after each iteration",
+           "iterations": it, "plain_length": round(float(plain_len), 1), "plain_wall_s": round(plain_wall, 3),
+           "decomposed_length": round(res["length"], 1), "decomposed_wall_s": round(res["wall_s"], 3),
+           "sub_problems_per_round": rounds[0][2] if rounds else 0,
+           "replaced_per_round": [r[3] for r in rounds],
+           "wall_s_per_round": round((res["wall_s"] - plain_wall) / max(1, len(rounds)), 4),
+           "valid": bool(inst.check_solution(res["tour"]))}
+    log(f"decomposition 3001: plain {plain_len:.0f} -> decomposed {res['length']:.0f} "
+        f"in {res['wall_s']:.2f}s ({len(rounds)} rounds)")
+    return out
+
+
 def leg_batch(ca, instances, args):
     binst = instances.batch(0, count=args.batch_instances)
     ns = [it.n for it in binst]
@@ -522,6 +561,7 @@ def main():
                 ref_gpu["ant_solutions_per_s"] / cpu["value"], 2)
     clustered = leg_clustered(ca, instances, args) if args.clustered_steps > 0 else None
     batch = leg_batch(ca, instances, args) if args.batch_instances > 0 else None
+    decomposed = leg_decompose(ca, instances, args) if args.decompose_iterations > 0 else None
     prior_leg = leg_prior(instances, args) if args.prior_instances > 0 else None
 
     out = {
@@ -558,6 +598,7 @@ def main():
         "reference_rules_gpu": ref_gpu,
         "cpu_baseline": cpu,
         "clustered481": clustered,
+        "decomposed3001": decomposed,
         "batch256": batch,
         "learned_prior": prior_leg,
     }

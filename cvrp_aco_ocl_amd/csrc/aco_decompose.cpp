@@ -122,7 +122,9 @@ int aco_partition_routes(int32_t n, const float *x, const float *y, const int32_
         order[r] = {atan2(py - (double)y[0], px - (double)x[0]), r};
     }
     std::sort(order.begin(), order.end());
-    int G = subs > 0 ? subs : n / 50;
+    // default: n/50 (problem.cpp:218), at most R/2 so that every
+    // sub-problem holds two routes or more (customers can change routes)
+    int G = subs > 0 ? subs : std::min(n / 50, R / 2);
     G = std::max(1, std::min(G, R));
     long long customers = 0;
     for (const Route &r : rs) customers += r.end - r.beg - 1;

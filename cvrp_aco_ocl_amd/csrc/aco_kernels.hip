@@ -371,7 +371,10 @@ __device__ void construct_knn_ant(const DevProblem &p, const DevState &s, int a,
         if (total > 0.0f) {
             const float u = RNG == kRngPhilox ? lane_f32(ublk, (step - 1) & 63) : pm_ran01(z);
             const float r = u * total;
-            const u64 m1 = __ballot(lane < k && x > r);
+            // only lanes with mass: the scan's additions are grouped by the DPP
+            // tree, so a lane without mass can round above its left
+            // neighbour's prefix and must not be taken
+            const u64 m1 = __ballot(pj > 0.0f && x > r);
             int wc;
             if (m1) {
                 wc = __ffsll((unsigned long long)m1) - 1;
@@ -529,7 +532,10 @@ __global__ __launch_bounds__(64 * A) void k_construct_knn_lds(DevBatch bt, int r
         if (total > 0.0f) {
             const float u = RNG == kRngPhilox ? lane_f32(ublk, (step - 1) & 63) : pm_ran01(z);
             const float r = u * total;
-            const u64 m1 = __ballot(lane < k && x > r);
+            // only lanes with mass: the scan's additions are grouped by the DPP
+            // tree, so a lane without mass can round above its left
+            // neighbour's prefix and must not be taken
+            const u64 m1 = __ballot(pj > 0.0f && x > r);
             int wc;
             if (m1) {
                 wc = __ffsll((unsigned long long)m1) - 1;

@@ -3,6 +3,8 @@
 // with status 1, as the reference's check_error does (g_aco.cpp:607-614).
 #include "g_aco.h"
 
+#include "cvrp_decompose.h"
+
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -129,6 +131,54 @@ void GpuACO::update_best_so_far_from_device()
     tour.resize(size);
     instance.best_tour = tour;
     instance.best_length = len;
+}
+
+float GpuACO::decompose_round(int round, std::vector<int> &merged)
+{
+    const Problem &in = instance;
+    aco_problem pb{};
+    pb.n = in.num_node;
+    pb.dist = in.distance.data();
+    pb.demand = in.demand.data();
+    pb.capacity = in.vehicle_capacity;
+    pb.max_dist = in.max_distance;
+    pb.service_time = in.service_time;
+    std::vector<float> x(in.x.begin(), in.x.end()), y(in.y.begin(), in.y.end());   // Point floats
+    aco_decompose_params dp{};
+    dp.subs = in.num_subs;
+    dp.sub_iterations = in.sub_iterations;
+    dp.rotation = round;
+    dp.pheromone_back = in.pheromone_back ? 1 : 0;
+    aco_default_params(&dp.sub_params);
+    aco_params &sp = dp.sub_params;
+    sp.alpha = in.alpha;
+    sp.beta = in.beta;
+    sp.rho = in.rho;
+    sp.n_ants = in.sub_ants;
+    sp.ras_ranks = in.ras_ranks;
+    sp.selection = ACO_SEL_KNN;
+    sp.rng = ACO_RNG_PHILOX;
+    sp.k = in.k > 0 ? in.k : 16;
+    sp.ls_flags = ACO_LS_2OPT | ACO_LS_SWAP | ACO_LS_RELOCATE;
+    sp.seed = in.seed + 1000003ull * (unsigned long long)(round + 1);
+    merged.assign(2 * (size_t)in.num_node + 2, 0);
+    int32_t size = 0, subs = 0, replaced = 0;
+    float len = 0.0f;
+    check(aco_decompose_round(ctx, 0, &pb, x.data(), y.data(), &dp, merged.data(), (int32_t)merged.size(), &size,
+                              &len, &subs, &replaced),
+          "decompose_round");
+    merged.resize(size);
+    return len;
+}
+
+void GpuACO::run_aco_iteration_with(const std::vector<int> &ant)
+{
+    construct_solutions();
+    if (instance.ls_flags) local_search();
+    const int m = instance.n_ants > 0 ? instance.n_ants : instance.num_node;
+    check(aco_set_ant_at(ctx, 0, m - 1, ant.data(), (int32_t)ant.size()), "run_aco_iteration_with");
+    update_statistics();
+    pheromone_update();
 }
 
 aco_counters GpuACO::counters()

@@ -41,6 +41,16 @@ struct Problem {
     unsigned long long seed = 1;
     bool use_graph = true;
 
+    // sub-problem decomposition (problem.h:136-139, problem.cpp:216-218):
+    // every master_iterations iterations a round over num_subs sub-problems
+    // (0: num_node/50), sub_iterations iterations of sub_ants ants each
+    bool decompose = false;
+    int num_subs = 0;
+    int master_iterations = 1;
+    int sub_iterations = 75;
+    int sub_ants = 64;
+    bool pheromone_back = false;
+
     std::vector<int> best_tour;  // best_so_far_ant
     float best_length = 0.0f;
     float best_time = 0.0f;
@@ -71,6 +81,12 @@ public:
     void pheromone_init(float trail_0);
 
     void update_best_so_far_from_device();
+
+    // One decomposition round (cvrp_decompose.h, aco_decompose_round) on the
+    // best-so-far; returns the merged tour's length, the tour in `merged`.
+    float decompose_round(int round, std::vector<int> &merged);
+    // run_aco_iteration with `ant` put in as the last ant after local search
+    void run_aco_iteration_with(const std::vector<int> &ant);
     const std::vector<aco_bsf_record> &records() const { return records_; }
     aco_counters counters();
 

@@ -8,6 +8,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <chrono>
 #include <string>
 
@@ -33,6 +34,10 @@ static void usage(const char *prog)
             "  --nn-ls D                         2-opt neighbour depth (nodes-2)\n"
             "  --tau0 first|nn                   initial trail rule (first)\n"
             "  --dist float|nint                 EUC_2D distances (float, as the reference)\n"
+            "  --decompose                       sub-problem rounds on the best-so-far\n"
+            "  --subs N  --sub-iterations I      sub-problems (nodes/50) and their iterations (75)\n"
+            "  --sub-ants M  --master-iterations P  ants per sub-colony (64), iterations per round (1)\n"
+            "  --pheromone-back                  sub-colony trails written into the master\n"
             "  --seed S  --no-graph  --report DIR\n"
             "  --sol DIR                         best tour of all tries as DIR/<name>.sol\n",
             prog);
@@ -105,6 +110,12 @@ int main(int argc, char **argv)
         else if (s == "--nn-ls") defaults.nn_ls = atoi(val());
         else if (s == "--seed") defaults.seed = strtoull(val(), nullptr, 10);
         else if (s == "--no-graph") defaults.use_graph = false;
+        else if (s == "--decompose") defaults.decompose = true;
+        else if (s == "--subs") defaults.num_subs = atoi(val());
+        else if (s == "--sub-iterations") defaults.sub_iterations = atoi(val());
+        else if (s == "--sub-ants") defaults.sub_ants = atoi(val());
+        else if (s == "--master-iterations") defaults.master_iterations = atoi(val());
+        else if (s == "--pheromone-back") defaults.pheromone_back = true;
         else if (s == "--report") report_dir = val();
         else if (s == "--sol") sol_dir = val();
         else if (s == "--selection") {
@@ -145,16 +156,41 @@ int main(int argc, char **argv)
         inst.seed = defaults.seed + (unsigned long long)ntry;
         GpuACO aco(inst);
         aco.init_aco();
-        // termination_condition (main.cpp:29-34), on the device best-so-far
+        // termination_condition (main.cpp:29-34), on the device best-so-far;
+        // with --decompose a round after every master_iterations iterations,
+        // its merged tour put into the next iteration when it is shorter
+        std::vector<int> pending;
+        float pending_len = INFINITY;
+        int round = 0;
         while (inst.iteration < inst.max_iteration && now_s() - t0 < inst.max_runtime) {
-            aco.run_aco_iteration();
+            if (!pending.empty()) {
+                aco.run_aco_iteration_with(pending);
+                pending.clear();
+                pending_len = INFINITY;
+            } else {
+                aco.run_aco_iteration();
+            }
             inst.iteration++;
+            if (inst.decompose && inst.iteration % std::max(1, inst.master_iterations) == 0) {
+                std::vector<int> merged;
+                const float len = aco.decompose_round(round++, merged);
+                aco.update_best_so_far_from_device();
+                if (len < inst.best_length) {
+                    pending = merged;
+                    pending_len = len;
+                }
+            }
             if (inst.optimum > 0.0f) {
                 aco.update_best_so_far_from_device();
-                if (fabsf(inst.best_length - inst.optimum) < 10 * 0.001f) break;
+                const float best = std::min(inst.best_length, pending_len);
+                if (fabsf(best - inst.optimum) < 10 * 0.001f) break;
             }
         }
         aco.exit_aco();
+        if (!pending.empty() && pending_len < inst.best_length) {
+            inst.best_tour = pending;
+            inst.best_length = pending_len;
+        }
         // exit_report (io.cpp:238-291)
         const bool ok = check_solution(inst, inst.best_tour);
         printf("try %d: Best Length %.2f Iterations %d At time %.3f Tot.time %.3f valid %s\n", ntry,

@@ -46,7 +46,9 @@ int aco_route_centers(int32_t n, const float *x, const float *y, const int32_t *
  * by the angle of their centre about the depot (atan2 in double; a route
  * of zero demand uses the plain mean of its customers), ties by their order
  * in the tour, and cut circularly, starting at position rotation mod R, into
- * G = clamp(subs > 0 ? subs : n/50, 1, R) runs of consecutive routes; a run
+ * G = clamp(subs > 0 ? subs : min(n/50, R/2), 1, R) runs of consecutive
+ * routes (the default n/50 of problem.cpp:218, at most half the routes so
+ * that every sub-problem can move customers between two routes or more); a run
  * closes after a route once its customers times G reach (g+1) times all
  * customers, or when exactly one route per remaining run is left.
  * Sub-problem g: real_nodes[offsets[g] .. offsets[g+1]) = 0 (the depot),
@@ -114,7 +116,7 @@ int aco_pheromone_from_subs(aco_ctx *master, int32_t instance, aco_ctx *subs, co
  * best-so-far is left as it is: the caller puts the merged tour into the
  * colony (aco_set_ant_at before the next aco_update_statistics). */
 typedef struct aco_decompose_params {
-    int32_t    subs;             /* sub-problems; 0 = n/50 (problem.cpp:218)          */
+    int32_t    subs;             /* sub-problems; 0 = min(n/50, routes/2)             */
     int32_t    sub_iterations;   /* per round; 0 = 75 (problem.cpp:217)              */
     int32_t    rotation;         /* first route of the circular cut                  */
     int32_t    pheromone_back;   /* 1: aco_pheromone_from_subs after the sub-colonies */

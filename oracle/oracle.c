@@ -326,7 +326,9 @@ static int greedy_pick(const orc_problem *p, const float *tinfo, const unsigned 
  *   (Hillis-Steele with offsets 1,2,4,8 inside each row of 16 lanes, then
  *   row 0's total into row 1 and row 2's into row 3, then lane 31's prefix
  *   into rows 2 and 3), r = u * prefix[k-1],
- *   first candidate with prefix > r (else the last with positive mass);
+ *   first candidate with positive mass and prefix > r (else the last with
+ *   positive mass: the tree's groupings round differently, so a masked
+ *   lane's prefix can exceed its left neighbour's and must not be taken);
  *   zero mass falls back to the greedy pick over the full row, and to the
  *   depot when no node is feasible. */
 int orc_construct_ant(const orc_problem *p, const float *tinfo, const float *tiknn,
@@ -388,7 +390,7 @@ int orc_construct_ant(const orc_problem *p, const float *tinfo, const float *tik
                                         : orc_ran01(&pm[0]);
                 float r = u * total;
                 int wc = -1;
-                for (int c = 0; c < k; ++c) if (x[c] > r) { wc = c; break; }
+                for (int c = 0; c < k; ++c) if (pv[c] > 0.0f && x[c] > r) { wc = c; break; }
                 if (wc < 0)
                     for (int c = k - 1; c >= 0; --c) if (pv[c] > 0.0f) { wc = c; break; }
                 w = cj[wc];
@@ -1193,7 +1195,7 @@ int orc_partition_routes(int n, const float *x, const float *y, const int *dem, 
         key[r].r = r;
     }
     qsort(key, R, sizeof(orc_route_key), route_key_cmp);
-    int G = subs > 0 ? subs : n / 50;
+    int G = subs > 0 ? subs : (n / 50 < R / 2 ? n / 50 : R / 2);
     if (G < 1) G = 1;
     if (G > R) G = R;
     int total = 0;

@@ -178,7 +178,7 @@ int orc_route_centers(int n, const float *x, const float *y, const int *dem, con
  * partitions): routes ordered by the angle of their centre about the
  * depot (atan2 in double; a route of zero demand uses the plain mean of
  * its customers), ties by route order; cut circularly from position
- * rotation mod R into G = clamp(subs > 0 ? subs : n/50, 1, R) contiguous
+ * rotation mod R into G = clamp(subs > 0 ? subs : min(n/50, R/2), 1, R) contiguous
  * groups: a group closes after a route once its running customer count
  * times G reaches (g+1) times the total, or when exactly one route per
  * remaining group is left.  Group g: real_nodes[offsets[g] .. offsets[g+1])

@@ -134,7 +134,7 @@ def test_partition_equals_oracle_and_invariants(case, subs, rotation):
         np.testing.assert_array_equal(st, ost)
     routes = _angles(inst, t)
     R = len(routes)
-    G = max(1, min(subs if subs > 0 else inst.n // 50, R))
+    G = max(1, min(subs if subs > 0 else min(inst.n // 50, R // 2), R))
     assert len(parts) == G
     allc = np.concatenate([r[1:] for r, _ in parts])
     np.testing.assert_array_equal(np.sort(allc), np.arange(1, inst.n))

@@ -49,3 +49,21 @@ def test_cli_reference_defaults_with_distance_limit():
 def test_cli_rejects_missing_file():
     r = _run(["--tries", "1", "/nonexistent.vrp"])
     assert r.returncode == 1 and "read_instance_file" in r.stderr
+
+
+def test_cli_decomposition_rounds(tmp_path):
+    """--decompose on X-n1001 without local search on the master: rounds of
+    sub-colonies after every iteration; the result is valid and shorter than
+    the same run without rounds."""
+    path = os.path.join(VRPLIB, "X-n1001-k43.vrp")
+    if not os.path.exists(path):
+        pytest.skip("X-n1001-k43 not in the fixtures")
+    base = ["--tries", "1", "--iterations", "4", "--ants", "128", "--selection", "knn", "--k", "16",
+            "--dist", "nint", "--seed", "3"]
+    plain = _run(base + [path])
+    dec = _run(base + ["--decompose", "--sub-iterations", "25", "--pheromone-back", path])
+    assert plain.returncode == 0 and dec.returncode == 0, dec.stderr
+    lp = float(plain.stdout.split("Best Length ")[1].split()[0])
+    ld = float(dec.stdout.split("Best Length ")[1].split()[0])
+    assert "valid yes" in dec.stdout
+    assert ld < lp
