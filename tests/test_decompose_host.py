@@ -228,3 +228,30 @@ def test_decomposition_rejects_bad_input():
         dc.sub_problem(inst, np.array([3, 0, 5], np.int32))      # depot not first
     with pytest.raises(ca.AcoError):
         dc.sub_problem(inst, np.array([0, 5, 5], np.int32))      # repeated node
+
+
+def test_pheromone_from_sub_round_trip_and_locality():
+    """orc_pheromone_from_sub is the inverse of the sub-problem gather: the
+    master's own trail gathered through real_nodes and written back leaves
+    tau and total_info unchanged; a changed sub trail lands exactly on the
+    sub-problem's ordered pairs (i != j) with total_info = tau * heur."""
+    inst = I.clustered(4, customers=120)
+    parts = oracle.partition_routes(inst.x, inst.y, inst.demand, _tour(inst), 3, 0)
+    real = parts[1][0]
+    heur = oracle.heuristic(inst.dist, 2.0)
+    r = np.random.default_rng(1)
+    tau = r.uniform(1e-4, 1e-2, (inst.n, inst.n)).astype(np.float32)
+    ti = (tau * heur).astype(np.float32)
+    t2, i2 = tau.copy(), ti.copy()
+    oracle.pheromone_from_sub(1.0, heur, t2, i2, real, tau[np.ix_(real, real)])
+    np.testing.assert_array_equal(t2, tau)
+    np.testing.assert_array_equal(i2, ti)
+    sub = r.uniform(1.0, 2.0, (len(real), len(real))).astype(np.float32)
+    oracle.pheromone_from_sub(1.0, heur, t2, i2, real, sub)
+    mask = np.zeros_like(tau, bool)
+    mask[np.ix_(real, real)] = True
+    mask[real, real] = False
+    np.testing.assert_array_equal(t2[~mask], tau[~mask])
+    np.testing.assert_array_equal(t2[np.ix_(real, real)][~np.eye(len(real), dtype=bool)],
+                                  sub[~np.eye(len(real), dtype=bool)])
+    np.testing.assert_array_equal(i2[mask], (t2 * heur)[mask])

