@@ -37,7 +37,13 @@ constexpr int kResWords = 80;
 
 // Candidates of one scan trip, and the padding of the LDS tour and edge
 // arrays past the tour's end that lets a trip read without clamping.
-constexpr int kScanUnroll = 4;
+// Candidates per lane per scan trip.  Smaller trips waste less work past an
+// early move and let waves above a lower move stop sooner; larger ones hide
+// more latency.  Measured best: swap 2; relocate 2 with 12 or 16 waves per
+// ant (one ant per CU), 4 with fewer (several ants per CU).
+constexpr int kSwapUnroll = 2;
+template <int W> constexpr int kRelocateUnroll = W >= 12 ? 2 : 4;
+constexpr int kScanUnroll = 4;   // the largest of the above
 constexpr int kScanPad = kScanUnroll * 64 + 4;
 
 template <int W> struct LsW {
@@ -727,7 +733,8 @@ __device__ __forceinline__ int customers_upto(const LsLds &L, int x)
     return x - L.ridx[x] - (L.tour[x] == 0 ? 1 : 0);
 }
 
-// A trip of a scan covers kScanUnroll chunks of 64 consecutive candidates,
+// A trip of a scan covers U chunks of 64 consecutive candidates (U =
+// kSwapUnroll or kRelocateUnroll<W>; the arrays are padded for the largest),
 // one per lane.  Each lane reads its candidate's neighbours in the tour and
 // edge arrays directly (past the tour's end the padding reads as depots and
 // zero edges), and the gain and the mask of improving candidates are formed
@@ -777,7 +784,7 @@ __device__ __forceinline__ int swap_scan(const DevProblem &p, const LsLds &L, in
                                          int lane, const ScanStop &stop, int &evc)
 {
     using V = typename RowSum<RT>::T;
-    constexpr int U = kScanUnroll;
+    constexpr int U = kSwapUnroll;
     const int n1 = L.tour[i];
     const int sn1 = L.tour[i + 1], r1 = L.ridx[i];
     const float a1 = rget(rowA, n1), b1 = rget(rowB, sn1);
@@ -964,13 +971,12 @@ __device__ __forceinline__ bool relocate_feasible(const DevProblem &p, const LsL
 // with the other masks, and only a chunk with an improving feasible
 // candidate leaves the common path.  Otherwise every route bit is set and
 // the full check runs for improving candidates.
-template <typename RT>
+template <int U, typename RT>
 __device__ __forceinline__ int relocate_scan(const DevProblem &p, const LsLds &L, int S, int R, int i,
                                              const RT *rowP, const RT *rowB, int lane,
                                              const ScanStop &stop, int &evc)
 {
     using V = typename RowSum<RT>::T;
-    constexpr int U = kScanUnroll;
     const int n1 = L.tour[i];
     const int r1 = L.ridx[i], sn1 = L.tour[i + 1];
     const float a1 = L.dtw[i - 1], b1 = L.dtw[i], c1 = rget(rowP, sn1);
@@ -1060,7 +1066,7 @@ __device__ __forceinline__ void ls_relocate(const DevProblem &p, LsLds &L, const
         if (scan) scan = __builtin_amdgcn_readfirstlane(L.tour[i]) != 0;
         if (scan) scan = __builtin_amdgcn_readfirstlane(L.rc[L.ridx[i]]) >= 2;
         if (scan)
-            j = relocate_scan(p, L, S, R, i, rr.at(i - 1), rr.at(i), lane, stop, evc);
+            j = relocate_scan<kRelocateUnroll<W>>(p, L, S, R, i, rr.at(i - 1), rr.at(i), lane, stop, evc);
         if (j >= 0 && lane == 0) atomicMax(reinterpret_cast<unsigned *>(L.res + LsW<W>::res_first), stop.mine);
         int *res = L.res + W * parity;
         parity ^= 1;
@@ -1157,6 +1163,17 @@ __device__ __forceinline__ void ls_relocate(const DevProblem &p, LsLds &L, const
 // before reusing smem.
 // R2: the instantiation with the random-order 2-opt (ACO_LS_2OPT_RANDOM);
 // kept out of the others, whose code it would otherwise enlarge.
+// Issue priority of wave w of W by quarters: the lowest quarter 3, the
+// highest 0 (s_setprio takes an immediate).
+__device__ __forceinline__ void scan_priority(int w, int W)
+{
+    const int q = __builtin_amdgcn_readfirstlane((w * 4) / W);
+    if (q == 0) __builtin_amdgcn_s_setprio(3);
+    else if (q == 1) __builtin_amdgcn_s_setprio(2);
+    else if (q == 2) __builtin_amdgcn_s_setprio(1);
+    else __builtin_amdgcn_s_setprio(0);
+}
+
 template <int W, typename RT, bool R2 = false>
 __device__ void local_search_ant(const DevProblem &p, const DevState &s, int a, int flags, int ring,
                                  int prefetch, unsigned char *smem)
@@ -1235,6 +1252,10 @@ __device__ void local_search_ant(const DevProblem &p, const DevState &s, int a, 
         }
         __syncthreads();
     }
+    // swap and relocate: wave w scans position i0 + w of a round, and the
+    // lowest position with a move decides it; the waves above it mostly do
+    // work that is discarded.  Lower waves issue first (timing only).
+    if (flags & (kLsSwap | kLsRelocate)) scan_priority(tid >> 6, W);
     if (flags & kLsSwap) {
         ls_edges<W>(p, L, 0, S - 2, tid);
         __syncthreads();
@@ -1247,6 +1268,7 @@ __device__ void local_search_ant(const DevProblem &p, const DevState &s, int a, 
         ls_routes<W>(p, L, S, tid, true);
         ls_relocate<W, RT>(p, L, rr, pf_on, S, tid, round, ev, moves);
     }
+    __builtin_amdgcn_s_setprio(0);
     ls_edges<W>(p, L, 0, S - 2, tid);
     __syncthreads();
     for (int q = tid; q < S; q += NT) gt[q] = L.tour[q];
