@@ -82,6 +82,9 @@ def parse():
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes) and exit")
     ap.add_argument("--lds-probe", action="store_true",
                     help="only run the LDS bandwidth microbenchmark and exit")
+    ap.add_argument("--ls-phases", action="store_true",
+                    help="only split the workload's local search into its phases (2-opt, swap, "
+                         "relocate, all three) on the same constructed tours and exit")
     return ap.parse_args()
 
 
@@ -113,6 +116,44 @@ def lds_peak():
     if rc != 0:
         raise RuntimeError(f"probe_lds_bandwidth failed ({rc})")
     return gbps.value, ms.value
+
+
+def ls_phases(shape) -> None:
+    """Local-search time of the workload per phase: the tours of iteration 6
+    searched with 2-opt alone, swap alone (after the 2-opt), relocate alone
+    (after both), and all three, each from the same start; HIP-event time of
+    the one launch, evaluated moves and applied moves per ant."""
+    import cvrp_aco_ocl_amd as ca
+    from cvrp_aco_ocl_amd import instances
+    inst = instances.mixed_1000(0)
+    m, k = WORKLOAD["m"], WORKLOAD["k"]
+    mk = lambda flags: ca.GpuACO(inst, n_ants=m, k=k, seed=1, selection=ca.SEL_KNN, rng=ca.RNG_PHILOX,
+                                 ls_flags=flags, use_graph=False)
+    g = mk(7)
+    g.init_aco()
+    g.run(5)
+    g.construct_solutions()
+    start = tuple(a.copy() for a in g.ants())
+    g.close()
+    state, out = start, {}
+    for flags, name in ((1, "2opt"), (2, "swap"), (4, "relocate"), (7, "all")):
+        h = mk(flags)
+        h.init_aco()
+        if shape:
+            h.set_launch_shape(*shape[:5])
+        h.set_ants(*(start if flags == 7 else state))
+        h.profile_enable(True)
+        h.local_search()
+        h.update_statistics()
+        ms, _ = h.profile("local_search")
+        c = h.counters()
+        after = tuple(a.copy() for a in h.ants())
+        h.close()
+        out[name] = {"ms": round(ms, 4), "evals_per_ant": round(c["ls_evals"] / m, 1),
+                     "moves_per_ant": round(c["ls_moves"] / m, 1)}
+        if flags in (1, 2):
+            state = after
+    print(json.dumps({"ls_phases": out, "shape": list(shape) if shape else None}))
 
 
 def mlp_probe(instances_count: int) -> None:
@@ -485,6 +526,9 @@ def main():
         return
     if args.mlp_probe:
         mlp_probe(args.mlp_probe)
+        return
+    if args.ls_phases:
+        ls_phases([int(v) for v in args.shape.split(",")] if args.shape else None)
         return
     import cvrp_aco_ocl_amd as ca
     from cvrp_aco_ocl_amd import instances
