@@ -255,3 +255,29 @@ def test_pheromone_from_sub_round_trip_and_locality():
     np.testing.assert_array_equal(t2[np.ix_(real, real)][~np.eye(len(real), dtype=bool)],
                                   sub[~np.eye(len(real), dtype=bool)])
     np.testing.assert_array_equal(i2[mask], (t2 * heur)[mask])
+
+
+def test_merge_keeps_route_length_limit_and_service_time():
+    """CMT6 (float distances, DISTANCE and SERVICE_TIME): sub-problems carry
+    the limit and the service time; tours valid in the sub-problem merge
+    into a tour valid in the master, equal to the oracle's merge."""
+    import os
+    from helpers import VRPLIB
+    inst = ca.Instance.from_vrplib(os.path.join(VRPLIB, "CMT6.vrp"))
+    P = oracle.Problem(inst.dist, inst.demand, inst.capacity, inst.max_dist, inst.service_time)
+    t = P.nn_tour()[0]
+    parts = dc.partition_routes(inst, t, 3, 2)
+    bests = []
+    for real, st in parts:
+        sub = dc.sub_problem(inst, real)
+        assert sub.max_dist == inst.max_dist and sub.service_time == inst.service_time
+        sp = oracle.Problem(sub.dist, sub.demand, sub.capacity, sub.max_dist, sub.service_time)
+        b = sp.nn_tour()[0]
+        assert sp.check(b) and sp.check(st)
+        bests.append(b)
+    m, L, rep = dc.merge_sub_tours(inst, parts, bests)
+    om, oL, orep = oracle.merge_sub_tours(inst.dist, parts, bests)
+    np.testing.assert_array_equal(m, om)
+    assert L == oL
+    np.testing.assert_array_equal(rep, orep)
+    assert inst.check_solution(m)

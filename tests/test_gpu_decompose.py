@@ -120,6 +120,29 @@ def test_decompose_round_equals_oracle_replay(subs, rotation):
     assert length <= blen
 
 
+def test_decompose_round_with_route_length_limit_equals_oracle():
+    """CMT6: float distances, a route-length limit and service times; the
+    sub-problems inherit both, every merged route stays within the limit."""
+    import os
+    from helpers import VRPLIB
+    inst = ca.Instance.from_vrplib(os.path.join(VRPLIB, "CMT6.vrp"))
+    assert np.isfinite(inst.max_dist) and inst.service_time > 0
+    m_sub, k_sub, iters, seed = 16, 10, 6, 3
+    with ca.GpuACO(inst, n_ants=24, k=12, selection=ca.SEL_KNN, rng=ca.RNG_PHILOX, ls_flags=7) as g:
+        g.init_aco()
+        g.run(3)
+        bsf, blen = g.best()
+        tour, length, nsub, rep = dc.decompose_round(g, inst, subs=2, sub_iterations=iters, rotation=1,
+                                                     params=dc.sub_params(n_ants=m_sub, k=k_sub, seed=seed))
+    parts = oracle.partition_routes(inst.x, inst.y, inst.demand, bsf, 2, 1)
+    bests, _, _ = _sub_colonies(inst, parts, m=m_sub, k=k_sub, ls_flags=7, seed=seed, iters=iters)
+    otour, olen, orep = oracle.merge_sub_tours(inst.dist, parts, bests)
+    log(f"CMT6 round: {nsub} sub-problems, {rep} replaced, {blen} -> {length}")
+    np.testing.assert_array_equal(tour, otour)
+    assert length == olen and rep == int(orep.sum())
+    assert inst.check_solution(tour)
+
+
 def _oracle_solve_decomposed(inst, iterations, *, master, sub, subs, sub_iterations, pheromone_back):
     P = oracle_problem(inst, k=master["k"])
     col = oracle.Colony(P, master["n_ants"], selection=oracle.SEL_KNN, rng=oracle.RNG_PHILOX,
