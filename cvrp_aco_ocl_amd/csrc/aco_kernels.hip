@@ -257,6 +257,8 @@ __device__ __forceinline__ float tour_length_in_order(const DevProblem &p, const
     const DevState s = bt.states[inst_];                    \
     const int a = g_ - bt.ant_base[inst_];
 
+constexpr int kIrChunk = 16;   // I-Roulette: nodes per lane whose loads are issued together
+
 template <bool HAS_D>
 __global__ __launch_bounds__(64) void k_construct_iroulette(DevBatch bt)
 {
@@ -278,15 +280,31 @@ __global__ __launch_bounds__(64) void k_construct_iroulette(DevBatch bt)
     while (vcnt < n) {
         const float *tirow = s.tinfo + (size_t)cur * p.P;
         const float *drow = p.dist + (size_t)cur * p.P;
+        // the row's values (and distances) for 16 nodes per lane are loaded
+        // together, then the lane draws for them in node order: one memory
+        // latency per 1024 nodes instead of one per node
         u64 best = 0;
-        for (int i = lane; i < n; i += kLanes) {
-            const float r = pm_ran01(z) + 0.001f;
-            const float pp = tirow[i] * 100.0f;
-            const float dci = HAS_D ? drow[i] : 0.0f;
-            const bool c = feasible<HAS_D>(p, L, i, dci, load, rdist);
-            const float prob = c ? r * pp : 0.0f;
-            const u64 key = ((u64)f2u(prob) << 32) | ((u64)(4095 - L.rank[i]) << 12) | (u64)i;
-            best = key > best ? key : best;
+        for (int base = 0; base < n; base += kIrChunk * kLanes) {
+            float tv[kIrChunk], dv[kIrChunk];
+#pragma unroll
+            for (int u = 0; u < kIrChunk; ++u) {
+                const int i = base + u * kLanes + lane;
+                const int ic = i < n ? i : n - 1;
+                tv[u] = gload(tirow + ic);
+                dv[u] = HAS_D ? gload(drow + ic) : 0.0f;
+            }
+#pragma unroll
+            for (int u = 0; u < kIrChunk; ++u) {
+                const int i = base + u * kLanes + lane;
+                if (i < n) {
+                    const float r = pm_ran01(z) + 0.001f;
+                    const float pp = tv[u] * 100.0f;
+                    const bool c = feasible<HAS_D>(p, L, i, dv[u], load, rdist);
+                    const float prob = c ? r * pp : 0.0f;
+                    const u64 key = ((u64)f2u(prob) << 32) | ((u64)(4095 - L.rank[i]) << 12) | (u64)i;
+                    best = key > best ? key : best;
+                }
+            }
         }
         best = wave_max_u64(best);
         step++;
