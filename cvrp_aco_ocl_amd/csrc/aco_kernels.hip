@@ -112,6 +112,13 @@ size_t construct_lds_bytes(int n, int selection)
     return (size_t)n * (4 + 4 + (selection == kSelIRoulette ? 4 : 0) + 1) + 16;
 }
 
+// The four-wave I-Roulette kernel's exchange of wave maxima (2 x 4 u64) after
+// the per-ant arrays, 8-byte aligned.
+__host__ __device__ __forceinline__ size_t iroulette4_exchange_offset(int n)
+{
+    return ((size_t)n * 13 + 16 + 7) & ~size_t(7);
+}
+
 // kernel.cl:445-452 mask: not visited, load + d_i <= Q and, with a route
 // length limit, rdist + (d(cur,i) + svc) + d(i,0) <= D.
 template <bool HAS_D>
@@ -346,6 +353,133 @@ __global__ __launch_bounds__(64) void k_construct_iroulette(DevBatch bt)
         s.ant_moves[a] = 0;
     }
     s.pm[a * kLanes + lane] = z;
+}
+
+// The same construction with four waves per ant, for launches with at most
+// one ant per CU (otherwise the single-wave kernel keeps the CU's other SIMDs
+// busy with other ants).  A lane's draws stay its own Park-Miller stream:
+// node l + 64u uses the stream's (u+1)-th draw of the step, z * 16807^(u+1)
+// mod (2^31 - 1), so wave w takes u = w, w+4, ... starting from the jump
+// 16807^(w+1) and stepping by 16807^4; after the step every wave advances
+// the lane's state by the lane's node count.  The same keys as the
+// single-wave kernel; the waves' maxima meet in LDS (one barrier per step,
+// two buffers by step parity), and every wave sets the visited flag itself,
+// so it reads its own write in the next step.
+__device__ __forceinline__ int pm_mulmod(int z, unsigned a)
+{
+    // z * a mod (2^31 - 1) for z, a in [1, 2^31 - 2]: 2^31 = 1 (mod 2^31 - 1),
+    // and the product is never 0 mod the prime, so one subtraction suffices
+    const u64 prod = (u64)(u32)z * (u64)a;
+    u32 r = (u32)(prod & 0x7FFFFFFFull) + (u32)(prod >> 31);
+    r = r >= 0x7FFFFFFFu ? r - 0x7FFFFFFFu : r;
+    return (int)r;
+}
+
+constexpr int kIrWaves = 4;
+
+template <bool HAS_D>
+__global__ __launch_bounds__(256) void k_construct_iroulette4(DevBatch bt)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    ANT_PROLOGUE
+    const int n = p.n, tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    AntLds L = ant_lds(smem, n, true);
+    u64 *sh = reinterpret_cast<u64 *>(smem + iroulette4_exchange_offset(n));   // [2][kIrWaves]
+    for (int i = tid; i < n; i += kIrWaves * kLanes) {
+        L.dem[i] = p.dem[i];
+        L.d0[i] = p.d0[i];
+        L.rank[i] = p.rank[i];
+        L.vis[i] = i == 0;
+    }
+    __syncthreads();
+    int z = s.pm[a * kLanes + lane];
+    const unsigned a4 = (unsigned)pm_jump(1, kIrWaves), aw = (unsigned)pm_jump(1, (unsigned long long)w + 1);
+    const int cnt = lane < n ? (n - lane + kLanes - 1) / kLanes : 0;   // the lane's nodes (draws) per step
+    const unsigned acnt = (unsigned)pm_jump(1, (unsigned long long)cnt);
+    int *tour = s.tours + (size_t)a * p.T;
+    int cur = 0, load = 0, vcnt = 1, step = 0, fb = 0, tbuf = 0;
+    float rdist = 0.0f, tdist = 0.0f;
+    constexpr int C = kIrChunk / kIrWaves;   // a wave's nodes per lane per pass
+    while (vcnt < n) {
+        const float *tirow = s.tinfo + (size_t)cur * p.P;
+        const float *drow = p.dist + (size_t)cur * p.P;
+        u64 best = 0;
+        int zz = pm_mulmod(z, aw);
+        // pass over u = base + w + 4j (node lane + 64u), j < C
+        for (int ub = 0; ub * kLanes < n; ub += kIrChunk) {
+            float tv[C], dv[C];
+#pragma unroll
+            for (int j = 0; j < C; ++j) {
+                const int i = lane + kLanes * (ub + w + kIrWaves * j);
+                const int ic = i < n ? i : n - 1;
+                tv[j] = gload(tirow + ic);
+                dv[j] = HAS_D ? gload(drow + ic) : 0.0f;
+            }
+#pragma unroll
+            for (int j = 0; j < C; ++j) {
+                const int i = lane + kLanes * (ub + w + kIrWaves * j);
+                if (i < n) {
+                    const float r = (float)zz * 0x1p-31f + 0.001f;
+                    zz = pm_mulmod(zz, a4);
+                    const float pp = tv[j] * 100.0f;
+                    const bool c = feasible<HAS_D>(p, L, i, dv[j], load, rdist);
+                    const float prob = c ? r * pp : 0.0f;
+                    const u64 key = ((u64)f2u(prob) << 32) | ((u64)(4095 - L.rank[i]) << 12) | (u64)i;
+                    best = key > best ? key : best;
+                }
+            }
+        }
+        best = wave_max_u64(best);
+        u64 *ex = sh + (step & 1) * kIrWaves;
+        if (lane == 0) ex[w] = best;
+        __syncthreads();
+#pragma unroll
+        for (int v = 0; v < kIrWaves; ++v) {
+            const u64 o = ex[v];
+            best = o > best ? o : best;
+        }
+        z = pm_mulmod(z, acnt);
+        step++;
+        int wn;
+        if ((best >> 32) == 0) {
+            if (cur == 0) {
+                wn = greedy_pick<HAS_D>(p, L, tirow, drow, load, rdist, lane);
+                fb++;
+            } else {
+                wn = 0;
+            }
+        } else {
+            wn = (int)(best & 0xFFFull);
+        }
+        wn = uniform(wn);
+        const float dcw = drow[wn];
+        if (wn == 0) {
+            load = 0;
+            rdist = 0.0f;
+        } else {
+            load += L.dem[wn];
+            rdist = rdist + (dcw + p.svc);
+            vcnt++;
+        }
+        if (lane == 0) L.vis[wn] = 1;
+        tdist = tdist + dcw;
+        if (w == 0) tour_put(tour, tbuf, step, wn, lane);
+        cur = wn;
+    }
+    tdist = tdist + L.d0[cur];
+    step++;
+    if (w == 0) {
+        tour_close(tour, tbuf, step, lane);
+        if (lane == 0) {
+            s.sizes[a] = step + 1;
+            s.lens[a] = tdist;
+            s.ant_fb[a] = fb;
+            s.ant_evals[a] = 0;
+            s.ant_moves[a] = 0;
+        }
+        s.pm[a * kLanes + lane] = z;
+    }
 }
 
 // Roulette over the k-nearest candidate row: lane c holds candidate c's
@@ -644,6 +778,7 @@ static hipError_t max_static_lds(const void *const *fns, int count, size_t &byte
 hipError_t construct_static_lds(size_t &bytes)
 {
     const void *fns[] = {(const void *)k_construct_iroulette<false>, (const void *)k_construct_iroulette<true>,
+                         (const void *)k_construct_iroulette4<false>, (const void *)k_construct_iroulette4<true>,
                          (const void *)k_construct_knn<false, kRngPhilox, true>,
                          (const void *)k_construct_knn<true, kRngPhilox, true>,
                          (const void *)k_construct_knn<false, kRngParkMiller, true>,
@@ -679,8 +814,17 @@ hipError_t launch_construct(const DevBatch &b, int selection, int rng, int apb, 
     const size_t lds = construct_lds_bytes(b.max_n, selection);
     const dim3 grid(b.ants), block(kLanes);
     if (selection == kSelIRoulette) {
-        if (b.has_d) hipLaunchKernelGGL(k_construct_iroulette<true>, grid, block, lds, st, b);
-        else hipLaunchKernelGGL(k_construct_iroulette<false>, grid, block, lds, st, b);
+        if (b.ants <= b.cus) {
+            // at most one ant per CU: four waves per ant
+            const size_t lds4 = iroulette4_exchange_offset(b.max_n) + 2 * kIrWaves * sizeof(u64);
+            const dim3 block4(kIrWaves * kLanes);
+            if (b.has_d) hipLaunchKernelGGL(k_construct_iroulette4<true>, grid, block4, lds4, st, b);
+            else hipLaunchKernelGGL(k_construct_iroulette4<false>, grid, block4, lds4, st, b);
+        } else if (b.has_d) {
+            hipLaunchKernelGGL(k_construct_iroulette<true>, grid, block, lds, st, b);
+        } else {
+            hipLaunchKernelGGL(k_construct_iroulette<false>, grid, block, lds, st, b);
+        }
     } else {
         // the wide fallback pass while a step waits on its own memory
         // latency (at most one ant's wave per SIMD); with more ants the

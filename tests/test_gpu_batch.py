@@ -14,12 +14,15 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("case", [
-    dict(selection=ca.SEL_KNN, rng=ca.RNG_PHILOX, k=16, ls=7, tau0=ca.TAU0_FIRST_COLONY),
-    dict(selection=ca.SEL_IROULETTE, rng=ca.RNG_PARK_MILLER, k=0, ls=ca.LS_2OPT, tau0=ca.TAU0_NN_TOUR),
+    dict(selection=ca.SEL_KNN, rng=ca.RNG_PHILOX, k=16, ls=7, tau0=ca.TAU0_FIRST_COLONY, m=16),
+    # at most one ant per CU: the four-wave I-Roulette kernel
+    dict(selection=ca.SEL_IROULETTE, rng=ca.RNG_PARK_MILLER, k=0, ls=ca.LS_2OPT, tau0=ca.TAU0_NN_TOUR, m=16),
+    # 320 ants, more than the device's CUs: the single-wave I-Roulette kernel
+    dict(selection=ca.SEL_IROULETTE, rng=ca.RNG_PARK_MILLER, k=0, ls=ca.LS_2OPT, tau0=ca.TAU0_FIRST_COLONY, m=64),
 ])
 def test_batch_matches_per_instance_oracle(case):
     insts = I.batch(3, count=5)
-    m, iters, seed = 16, 3, 40
+    m, iters, seed = case["m"], 3, 40
     log(f"batch of {[x.n for x in insts]} {case}")
     with ca.GpuACO(insts, n_ants=m, k=case["k"], seed=seed, selection=case["selection"],
                    rng=case["rng"], ls_flags=case["ls"], tau0_rule=case["tau0"]) as g:
