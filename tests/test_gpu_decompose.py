@@ -227,3 +227,17 @@ def test_decomposition_past_the_local_search_limit():
         assert after <= before and nsub >= 1
     assert sum(rep for *_, rep in res["rounds"]) > 0
     assert res["length"] < res["rounds"][0][0]
+
+
+def test_decomposition_at_the_node_limit():
+    """4096 nodes (the library's maximum): two rounds over min(n/50, R/2)
+    sub-problems; every merged tour valid, the length never higher."""
+    inst = I.clustered(12, customers=4095, nclust=16, sigma=30.0, routes=60)
+    res = dc.solve_decomposed(inst, 2, sub_iterations=10,
+                              master=dict(n_ants=128, k=16, ls_flags=0, seed=2),
+                              sub=dict(n_ants=32, k=16, seed=3))
+    log(f"n=4096: rounds {res['rounds']}, {res['wall_s']:.2f}s")
+    assert inst.check_solution(res["tour"])
+    for before, after, nsub, rep in res["rounds"]:
+        assert after <= before and nsub >= 1
+    assert res["length"] <= res["rounds"][0][0]
