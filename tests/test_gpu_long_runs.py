@@ -9,6 +9,13 @@
   and the fallback path is reached.
 * The 1001-node mixed workload (256 ants, k = 32, 2-opt + swap +
   relocate) for 20 graph-replayed iterations under property checks.
+* The same workload at full size against the oracle colony, under the
+  candidate roulette (the launch shape the autotuner picks for it: 12
+  local-search waves per ant, 32-row ring of 16-bit rows with prefetch)
+  and under the reference's rules (I-Roulette with Park-Miller lane
+  streams, four waves per ant since the ants fit one per CU): tours,
+  lengths, tau, total_info, records and evaluated-move counts bit-equal
+  after every iteration.
 * An instance at the local-search LDS boundary: the largest n that
   aco_create accepts runs whole iterations; one more node is refused with
   the LDS message (static LDS of the kernels counted).
@@ -93,6 +100,42 @@ def test_mixed_1001_twenty_graph_iterations():
         assert rec[-1][1] == lens[m]
         cnt = g.counters()
         assert cnt["iterations"] == 21 and cnt["ls_moves"] > 0
+
+
+@pytest.mark.parametrize("rules", ["candidate_roulette", "reference"])
+def test_mixed_1001_matches_oracle_colony(rules):
+    inst = I.mixed_1000(0)
+    m, iters, seed = 256, 3, 11
+    if rules == "reference":
+        sel, rng, k = ca.SEL_IROULETTE, ca.RNG_PARK_MILLER, 32
+    else:
+        sel, rng, k = ca.SEL_KNN, ca.RNG_PHILOX, 32
+    log(f"1001 nodes vs oracle colony: {rules}")
+    P = oracle_problem(inst, k=k)
+    col = oracle.Colony(P, m, selection=sel, rng=rng, ls_flags=7, seed=seed, threads=THREADS)
+    col.init()
+    with ca.GpuACO(inst, n_ants=m, k=k, seed=seed, selection=sel, rng=rng, ls_flags=7,
+                   use_graph=True) as g:
+        g.init_aco()
+        if rules == "candidate_roulette":
+            g.set_launch_shape(12, 32, 1, 0, 1)
+        for it in range(iters):
+            g.run_aco_iteration()
+            col.iteration()
+            gt, gs, gl = g.ants()
+            ot, osz, ol = col.ants()
+            np.testing.assert_array_equal(gs, osz, err_msg=f"sizes at {it}")
+            for a in range(m + 1):
+                np.testing.assert_array_equal(gt[a, : gs[a]], ot[a, : osz[a]],
+                                              err_msg=f"iteration {it} ant {a}")
+            np.testing.assert_array_equal(gl, ol)
+            assert g.counters()["ls_evals"] == col.ls_evals(), it
+            log(f"iteration {it}: best {gl[m]}, evaluated moves {col.ls_evals()}")
+        gtau, gti = g.pheromone()
+        otau, oti = col.tau()
+        np.testing.assert_array_equal(gtau, otau)
+        np.testing.assert_array_equal(gti, oti)
+        assert [(r[0], r[1]) for r in g.bsf_records()] == [(r[0], r[1]) for r in col.bsf()]
 
 
 def _uniform(customers, seed):
