@@ -921,6 +921,28 @@ int aco_get_ants(aco_ctx *c, int32_t *tours, int32_t stride, int32_t *sizes, flo
     return aco_get_ants_at(c, 0, tours, stride, sizes, lens);
 }
 
+// A tour the kernels accept as an ant's solution: depot first and last,
+// every customer exactly once, no empty route (construction never makes
+// one).  Returns the reason it is not, or nullptr.
+static const char *tour_structure_error(const int32_t *t, int size, int n, std::vector<char> &seen)
+{
+    if (t[0] != 0 || t[size - 1] != 0) return "tours must start and end at the depot";
+    seen.assign(n, 0);
+    int customers = 0;
+    for (int q = 0; q < size; ++q) {
+        const int v = t[q];
+        if (v < 0 || v >= n) return "tour node out of range";
+        if (v == 0) {
+            if (q > 0 && t[q - 1] == 0) return "tour has an empty route";
+            continue;
+        }
+        if (seen[v]) return "tour visits a customer twice";
+        seen[v] = 1;
+        ++customers;
+    }
+    return customers == n - 1 ? nullptr : "tour misses a customer";
+}
+
 int aco_set_ants_at(aco_ctx *c, int32_t b, const int32_t *tours, int32_t stride, const int32_t *sizes,
                     const float *lens)
 {
@@ -929,14 +951,13 @@ int aco_set_ants_at(aco_ctx *c, int32_t b, const int32_t *tours, int32_t stride,
     if (!tours || !sizes || !lens || stride < 1) return set_error(ACO_ERR_ARG, "null argument");
     const InstanceData &I = c->inst[b];
     const int m1 = c->m + 1;
+    std::vector<char> seen;
     for (int a = 0; a < m1; ++a) {
         const int sz = sizes[a];
         if (a == c->m && sz == 0) continue;  // empty best-so-far slot
         if (sz < 3 || sz > I.T || sz > stride) return set_error(ACO_ERR_ARG, "tour size out of range");
-        const int32_t *t = tours + (size_t)a * stride;
-        if (t[0] != 0 || t[sz - 1] != 0) return set_error(ACO_ERR_ARG, "tours must start and end at the depot");
-        for (int q = 0; q < sz; ++q)
-            if (t[q] < 0 || t[q] >= I.n) return set_error(ACO_ERR_ARG, "tour node out of range");
+        if (const char *e = tour_structure_error(tours + (size_t)a * stride, sz, I.n, seen))
+            return set_error(ACO_ERR_ARG, std::string(e) + " (ant " + std::to_string(a) + ")");
     }
     std::vector<int32_t> buf((size_t)m1 * I.T, 0);
     for (int a = 0; a < m1; ++a) memcpy(&buf[(size_t)a * I.T], tours + (size_t)a * stride, (size_t)sizes[a] * 4);
@@ -959,9 +980,9 @@ int aco_set_ant_at(aco_ctx *c, int32_t b, int32_t ant, const int32_t *tour, int3
     const InstanceData &I = c->inst[b];
     if (ant < 0 || ant >= c->m) return set_error(ACO_ERR_ARG, "aco_set_ant_at: ant out of range");
     if (!tour || size < 3 || size > I.T) return set_error(ACO_ERR_ARG, "aco_set_ant_at: tour size out of range");
-    if (tour[0] != 0 || tour[size - 1] != 0) return set_error(ACO_ERR_ARG, "aco_set_ant_at: tour must start and end at the depot");
-    for (int q = 0; q < size; ++q)
-        if (tour[q] < 0 || tour[q] >= I.n) return set_error(ACO_ERR_ARG, "aco_set_ant_at: tour node out of range");
+    std::vector<char> seen;
+    if (const char *e = tour_structure_error(tour, size, I.n, seen))
+        return set_error(ACO_ERR_ARG, std::string("aco_set_ant_at: ") + e);
     HIP_TRY(hipMemcpyAsync(I.ds.tours + (size_t)ant * I.T, tour, (size_t)size * 4, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(I.ds.sizes + ant, &size, 4, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(launch_ant_length(I.dp, I.ds, ant, c->stream));

@@ -204,6 +204,9 @@ int aco_merge_sub_tours(int32_t n, const float *dist, int32_t count, const int32
         const int bsz = best_offsets[s + 1] - best_offsets[s];
         const int32_t *best = bsz > 0 ? best_tours + best_offsets[s] : nullptr;
         if (sn < 1 || osz < 2) return set_error(ACO_ERR_ARG, "aco_merge_sub_tours: empty sub-problem");
+        if (real[0] != 0) return set_error(ACO_ERR_ARG, "aco_merge_sub_tours: node lists start at the depot");
+        for (int i = 1; i < sn; ++i)
+            if (real[i] <= 0 || real[i] >= n) return set_error(ACO_ERR_ARG, "aco_merge_sub_tours: node list entry out of range");
         for (int q = 0; q < osz; ++q)
             if (orig[q] < 0 || orig[q] >= sn) return set_error(ACO_ERR_ARG, "aco_merge_sub_tours: sub tour out of range");
         for (int q = 0; q < bsz; ++q)

@@ -165,7 +165,9 @@ int aco_get_iteration(aco_ctx *ctx, int32_t *iteration);
 int aco_get_best(aco_ctx *ctx, int32_t *tour, int32_t capacity, int32_t *size, float *length);
 /* all m ants and the best-so-far slot m: tours[(m+1)*stride] */
 int aco_get_ants(aco_ctx *ctx, int32_t *tours, int32_t stride, int32_t *sizes, float *lengths);
-/* loads m+1 tours (slot m = best-so-far); used to replay a state */
+/* loads m+1 tours (slot m = best-so-far; size 0 leaves it empty); used to
+ * replay a state.  Each tour starts and ends at the depot and visits every
+ * customer exactly once with no empty route, else ACO_ERR_ARG. */
 int aco_set_ants(aco_ctx *ctx, const int32_t *tours, int32_t stride, const int32_t *sizes,
                  const float *lengths);
 /* best-so-far records (g_aco.cpp:95-117); *count = records written */

@@ -82,8 +82,9 @@ int aco_merge_sub_tours(int32_t n, const float *dist, int32_t count, const int32
                         const int32_t *best_tours, int32_t *out, int32_t capacity, int32_t *size,
                         float *length, int32_t *replaced);
 
-/* One ant of instance `instance` (0 <= ant < n_ants) set to `tour`, its
- * length added in step order as construction does.  No counterpart in the
+/* One ant of instance `instance` (0 <= ant < n_ants) set to `tour` (depot
+ * first and last, every customer once, no empty route), its length added in
+ * step order as construction does.  No counterpart in the
  * reference: between aco_construct / aco_local_search and
  * aco_update_statistics it puts a solution (e.g. a merged decomposition
  * result) into the colony, which then ranks and deposits it like any ant. */

@@ -228,6 +228,12 @@ def test_decomposition_rejects_bad_input():
         dc.sub_problem(inst, np.array([3, 0, 5], np.int32))      # depot not first
     with pytest.raises(ca.AcoError):
         dc.sub_problem(inst, np.array([0, 5, 5], np.int32))      # repeated node
+    t = oracle.Problem(inst.dist, inst.demand, inst.capacity).nn_tour()[0]
+    parts = dc.partition_routes(inst, t, 2)
+    r0, t0 = parts[0]
+    for bad in (np.r_[r0[:-1], inst.n], np.r_[r0[:-1], -1], np.r_[5, r0[1:]]):
+        with pytest.raises(ca.AcoError):                        # node list entry out of range
+            dc.merge_sub_tours(inst, [(bad.astype(np.int32), t0)] + parts[1:], [[]] * len(parts))
 
 
 def test_pheromone_from_sub_round_trip_and_locality():

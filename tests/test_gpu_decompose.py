@@ -64,7 +64,10 @@ def test_set_ant_matches_oracle_length():
         assert sizes[5] == len(tour)
         np.testing.assert_array_equal(tours[5, : sizes[5]], tour)
         assert lens[5] == np.float32(oracle.tour_length(inst.dist, tour))
-        for bad in ([1, 2, 0], [0, 5, 9999, 0]):
+        dup = tour.copy()
+        dup[2] = dup[1]                      # a customer twice, another missing
+        empty = np.concatenate([tour[:1], tour])  # an empty route at the start
+        for bad in ([1, 2, 0], [0, 5, 9999, 0], tour[:-2].tolist() + [0], dup, empty):
             with pytest.raises(ca.AcoError):
                 g.set_ant(1, np.array(bad, np.int32))
         with pytest.raises(ca.AcoError):
