@@ -133,6 +133,7 @@ _SIGNATURES = {
     "aco_compute_nn_lists": (C.c_int, [C.c_int32, _P, C.c_int32, C.c_int32, _P]),
     "aco_compute_distances_batch": (C.c_int, [C.c_int32, _P, _P, _P, C.c_int32, _P]),
     "aco_get_setup_at": (C.c_int, [_P, C.c_int32, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "aco_set_candidates_at": (C.c_int, [_P, C.c_int32, _P, C.c_int32]),
     "aco_compute_nn_lists_batch": (C.c_int, [C.c_int32, _P, _P, _P, C.c_int32, _P]),
     "aco_compute_nn_lists_batch_noalloc": (C.c_int, [C.c_int32, _P, _P, _P, C.c_int32, _P, _P]),
     "aco_read_vrplib": (C.c_int, [C.c_char_p, C.c_int32, C.POINTER(C.c_int32), _P, _P, _P,

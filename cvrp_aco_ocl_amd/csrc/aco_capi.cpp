@@ -1137,6 +1137,33 @@ int aco_get_setup_at(aco_ctx *c, int32_t b, float *heur, float *d0, int32_t *nnk
     return ACO_OK;
 }
 
+int aco_set_candidates_at(aco_ctx *c, int32_t b, const int32_t *cand, int32_t width)
+{
+    int rc = need_inst(c, b, false);
+    if (rc) return rc;
+    const InstanceData &I = c->inst[b];
+    const DevProblem &dp = I.dp;
+    const int n = I.n, k = dp.k;
+    if (k < 1) return set_error(ACO_ERR_STATE, "aco_set_candidates_at: the context has no candidate rows (k = 0)");
+    if (!cand || width < k) return set_error(ACO_ERR_ARG, "aco_set_candidates_at: need cand with width >= k");
+    std::vector<int> seen(n, -1);
+    for (int i = 0; i < n; ++i)
+        for (int h = 0; h < k; ++h) {
+            const int v = cand[(size_t)i * width + h];
+            if (v < 1 || v >= n || v == i || seen[v] == i)
+                return set_error(ACO_ERR_ARG, "aco_set_candidates_at: row " + std::to_string(i) +
+                                                  " needs k distinct customers other than itself");
+            seen[v] = i;
+        }
+    HIP_TRY(hipMemcpy2DAsync(const_cast<int *>(dp.nnk), (size_t)k * 4, cand, (size_t)width * 4, (size_t)k * 4, n,
+                             hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(launch_candidate_dists(dp, c->stream));
+    // the candidates' total_info, when the pheromone exists already
+    if (c->inited && (rc = run_gather(c))) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return ACO_OK;
+}
+
 int aco_set_rng_state(aco_ctx *c, const int32_t *state)
 {
     int rc = need(c, false);

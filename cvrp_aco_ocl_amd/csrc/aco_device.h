@@ -256,6 +256,7 @@ int gpu_nn_lists_dev(hipStream_t st, int count, const int *n, const float *const
 // candidate and neighbour tables from the lists nnl (n x width)
 hipError_t launch_setup_rows(const DevProblem &p, int beta_code, int has_prior, int int16, hipStream_t st);
 hipError_t launch_setup_nn(const DevProblem &p, const int *nnl, int width, hipStream_t st);
+hipError_t launch_candidate_dists(const DevProblem &p, hipStream_t st);
 // sub-colony pheromone into a master instance (rows of all jobs in one launch)
 struct SubTauJob {
     const float *tau;   // sub tau, n rows of pitch P

@@ -9,6 +9,8 @@
 //   k_setup_nn    the candidate rows (nnk and their distances), the 2-opt
 //                 neighbour rows (nnls / nnd, padded), and nnpos (position
 //                 of j in i's neighbour row, 0xFFFF if absent)
+//   k_candidate_dists
+//                 distances of caller-given construction candidate rows
 //   k_distances   distance matrices from coordinates (aco_compute_distances
 //                 for the EUC_2D, nint, CEIL_2D and ATT conventions)
 //   k_tau_from_subs, k_ant_length
@@ -85,6 +87,24 @@ hipError_t launch_setup_rows(const DevProblem &p, int beta_code, int has_prior, 
 hipError_t launch_setup_nn(const DevProblem &p, const int *nnl, int width, hipStream_t st)
 {
     hipLaunchKernelGGL(k_setup_nn, dim3(p.n), dim3(256), 0, st, p, nnl, width);
+    return hipGetLastError();
+}
+
+// Distances of the construction candidate rows after the caller replaced
+// their ids (aco_set_candidates_at): nnkd[i][c] = d(i, nnk[i][c]); the 2-opt
+// rows and nnpos are not touched.
+__global__ __launch_bounds__(256) void k_candidate_dists(DevProblem p)
+{
+    const int i = blockIdx.x;
+    const float *drow = p.dist + (size_t)i * p.P;
+    const int *kk = p.nnk + (size_t)i * p.k;
+    float *kd = const_cast<float *>(p.nnkd) + (size_t)i * p.k;
+    for (int c = threadIdx.x; c < p.k; c += 256) kd[c] = drow[kk[c]];
+}
+
+hipError_t launch_candidate_dists(const DevProblem &p, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_candidate_dists, dim3(p.n), dim3(256), 0, st, p);
     return hipGetLastError();
 }
 

@@ -156,6 +156,18 @@ class PriorNet:
         check(lib.aco_prior_apply(g.h, self.h, C.cast(xp, C.c_void_p), C.cast(yp, C.c_void_p), k,
                                   exponent, floor), "aco_prior_apply")
 
+    def rank_candidates(self, inst, k, pool):
+        """Construction candidate rows chosen by the network: for each node,
+        the k of its ``pool`` nearest neighbours (distance-sorted, ties to the
+        lower index) with the highest logits, ties to the nearer, in order of
+        decreasing logit.  Features are computed over the pool (rank r/pool,
+        kd the distance to the pool-th neighbour), so a network trained on
+        k-wide lists sees them rescaled.  For GpuACO.set_candidates."""
+        nn = np.ascontiguousarray(inst.nn_lists(pool), np.int32)
+        z = self.forward(features(inst, nn, pool)).reshape(inst.n, pool)
+        order = np.argsort(-z, axis=1, kind="stable")[:, :k]
+        return np.ascontiguousarray(np.take_along_axis(nn, order, axis=1), np.int32)
+
 
 def solve_plain(insts, iterations, n_ants=64, k=16, seed=1, ls_flags=7):
     """Plain colony (roulette over k candidates, Philox, local search) on a
@@ -172,12 +184,15 @@ def solve_plain(insts, iterations, n_ants=64, k=16, seed=1, ls_flags=7):
 
 
 def run_colony(insts, iterations, net=None, n_ants=64, k=16, seed=1, ls_flags=7, beta=2.0,
-               floor=0.05, targets=None, chunk=25, prior_exponent=None, autotune=True):
+               floor=0.05, targets=None, chunk=25, prior_exponent=None, autotune=True, rerank_pool=0):
     """Batched colony over ``insts`` (the prior from ``net`` multiplies eta
     when given).  Stops early once every instance reached its target cost
     (``targets``).  The prior is applied on the device (aco_prior_apply)
     with exponent ``prior_exponent`` (default beta: the probability
-    multiplies eta).  Returns dict with per-instance best-so-far records
+    multiplies eta; 0: eta unchanged).  With ``rerank_pool`` > k the network
+    also chooses the construction candidates among each node's rerank_pool
+    nearest (rank_candidates), and eta is scaled on those pool edges.
+    Returns dict with per-instance best-so-far records
     [(iteration, length, seconds of search)], the prior's application time
     (seconds, host wall clock), the iterations run and the search's wall
     time."""
@@ -188,8 +203,12 @@ def run_colony(insts, iterations, net=None, n_ants=64, k=16, seed=1, ls_flags=7,
     try:
         t0 = time.perf_counter()
         if net is not None:
-            net.apply(g, k=k, exponent=prior_exponent if prior_exponent is not None else beta,
-                      floor=floor)
+            if rerank_pool > k:
+                for b, inst in enumerate(insts):
+                    g.set_candidates(net.rank_candidates(inst, k, rerank_pool), b)
+            expo = prior_exponent if prior_exponent is not None else beta
+            if expo != 0:
+                net.apply(g, k=rerank_pool if rerank_pool > k else k, exponent=expo, floor=floor)
         t_build = time.perf_counter() - t0
         g.init_aco()
         if autotune:

@@ -336,6 +336,12 @@ class GpuACO:
             out["nnkd"] = out["nnkd"][:, :0]
         return out
 
+    def set_candidates(self, cand, b: int = 0):
+        """Construction candidate rows of instance b (n x >= k ids; the first
+        k of each row are used), aco_set_candidates_at."""
+        cand = np.ascontiguousarray(cand, np.int32)
+        check(lib.aco_set_candidates_at(self.h, b, _ptr(cand), cand.shape[1]), "aco_set_candidates_at")
+
     def set_rng_state(self, state):
         st = np.ascontiguousarray(state, np.int32)
         check(lib.aco_set_rng_state(self.h, _ptr(st)), "aco_set_rng_state")

@@ -104,14 +104,15 @@ def cmd_train(a):
 
 
 def compare(insts, iterations, net, ants=64, k=16, seed=1, floor=0.05,
-            budgets=(0.25, 0.5, 1.0, 2.0), exponent=None):
+            budgets=(0.25, 0.5, 1.0, 2.0), exponent=None, rerank_pool=0):
     """Plain colony for ``iterations`` (C* = its final best), then the prior
-    colony until every instance is within 2% of C* or ``iterations``."""
+    colony until every instance is within 2% of C* or ``iterations``
+    (``rerank_pool``: the prior also chooses the candidates, run_colony)."""
     plain = PR.run_colony(insts, iterations, None, n_ants=ants, k=k, seed=seed)
     cstar = [r[-1][1] for r in plain["records"]]
     targets = [1.02 * c for c in cstar]
     pri = PR.run_colony(insts, iterations, net, n_ants=ants, k=k, seed=seed, floor=floor,
-                        targets=targets, prior_exponent=exponent)
+                        targets=targets, prior_exponent=exponent, rerank_pool=rerank_pool)
     t_plain = [PR.time_to_target(r, t) for r, t in zip(plain["records"], targets)]
     t_prior = [PR.time_to_target(r, t, pri["build_s"]) for r, t in zip(pri["records"], targets)]
     gaps = {}
@@ -133,6 +134,7 @@ def compare(insts, iterations, net, ants=64, k=16, seed=1, floor=0.05,
         "plain_wall_s": round(plain["wall_s"], 3),
         "plain_ant_solutions_per_s": round(plain["ant_solutions"] / plain["wall_s"], 1),
         "prior_exponent": exponent,
+        "rerank_pool": rerank_pool,
         "gap_at_budget": gaps,
         "mean_cstar": float(np.mean(cstar)),
     }
@@ -145,7 +147,8 @@ def compare(insts, iterations, net, ants=64, k=16, seed=1, floor=0.05,
 def cmd_eval(a):
     net = PR.PriorNet.load(a.weights)
     insts = I.evaluation(a.instances)
-    res = compare(insts, a.iterations, net, ants=a.ants, k=a.k, floor=a.floor, exponent=a.exponent)
+    res = compare(insts, a.iterations, net, ants=a.ants, k=a.k, floor=a.floor, exponent=a.exponent,
+                  rerank_pool=a.rerank_pool)
     print(json.dumps(res))
 
 
@@ -175,6 +178,8 @@ def main(argv=None):
     e.add_argument("--floor", type=float, default=0.05)
     e.add_argument("--exponent", type=float, default=PRIOR_EXPONENT)
     e.add_argument("--weights", default=PR.WEIGHTS_PATH)
+    e.add_argument("--rerank-pool", type=int, default=0,
+                   help="the prior also chooses the k candidates among this many nearest (0: off)")
     a = ap.parse_args(argv)
     {"train": cmd_train, "eval": cmd_eval}[a.cmd](a)
 

@@ -141,6 +141,17 @@ int aco_set_pheromone_at(aco_ctx *ctx, int32_t instance, const float *tau,
 int aco_get_setup_at(aco_ctx *ctx, int32_t instance, float *heur, float *d0, int32_t *nnk, float *nnkd,
                      int32_t *nnls, float *nnd, uint16_t *nnpos, uint16_t *dist16);
 
+/* Replaces the construction candidate rows of an instance (the first k
+ * entries of each row of cand, n rows of `width` >= k entries): the roulette
+ * over candidates then draws from these nodes.  The reference takes them as
+ * the first nn_ants entries of the distance-sorted nn_list (kernel.cl:
+ * 382-493 with compute_nn_lists, vrpHelper.cpp:188-236); a build-only
+ * addition lets a caller choose another set (e.g. ranked by a learned
+ * prior).  The 2-opt neighbour rows stay distance-sorted.  Each row holds k
+ * distinct customers other than its own node, else ACO_ERR_ARG.  Before or
+ * after aco_init; afterwards the candidates' total_info is gathered again. */
+int aco_set_candidates_at(aco_ctx *ctx, int32_t instance, const int32_t *cand, int32_t width);
+
 /* g_ACO::init_aco (g_aco.cpp:63-93) */
 int aco_init(aco_ctx *ctx);
 /* g_ACO::run_aco_iteration (g_aco.cpp:122-134): construct -> local search
