@@ -4,13 +4,22 @@
 // No counterpart in the reference (its eta is 1/(d+0.1)^beta from the
 // distance alone, kernel.cl:672-679,782-789).
 //
-// MLP 12 -> 64 -> 64 -> 1 with ReLU.  BF16 operands, FP32 accumulation with
-// v_mfma_f32_16x16x16_bf16.  One wave processes 16 edges per tile with the
-// edges on the N dimension of every product: the accumulator layout of one
-// layer (lane l holds rows 4*(l/16)+j, column l%16) is exactly the B-operand
+// MLP 12 -> 64 -> 64 -> 1 with ReLU.  BF16 operands, FP32 accumulation on
+// the matrix cores.  One wave processes 16 edges per tile with the edges on
+// the N dimension of every product: the accumulator layout of one layer
+// (lane l holds rows 4*(l/16)+j, column l%16) is exactly the B-operand
 // layout of the next, so the forward pass needs no data movement between
 // layers.  The weight gradients contract over edges, which needs the
 // transposed layout: 16x16 tiles go through a per-wave LDS buffer.
+//
+// Layer 1 (K = 12 features, padded to 16) and the weight gradients (K = the
+// 16 edges of a tile) use v_mfma_f32_16x16x16_bf16.  The 64-deep products
+// (layer 2 and dH1 = W2^T dz2) use gfx950's v_mfma_f32_16x16x32_bf16, two
+// per 16-row block: its A and B operands are two 4-element halves put side
+// by side, each half one 16-deep slice in the layout above.  Element i of a lane's A operand
+// and element i of its B operand carry the same index of the contraction,
+// so the product sums exactly the slices' terms, whatever order the unit
+// adds them in.
 //
 // Determinism: per-wave partial gradients (fixed tile assignment) are summed
 // by the reduce kernel in wave order; there are no atomics.
@@ -42,6 +51,8 @@ constexpr int kTilesPerWave = 4;    // training: 64 edges per wave per launch
 constexpr int kMaxBatch = 65536;
 
 typedef short v4s __attribute__((ext_vector_type(4)));
+typedef short v8s __attribute__((ext_vector_type(8)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
 __host__ __device__ inline unsigned short f2bf(float f)
@@ -60,6 +71,15 @@ __device__ __forceinline__ v4s ld4(const unsigned short *p) { return *reinterpre
 __device__ __forceinline__ v4f mfma(v4s a, v4s b, v4f c)
 {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// c + [a0 a1] x [b0; b1]: two 16-deep slices in one 32-deep product.
+__device__ __forceinline__ v4f mfma2(v4s a0, v4s a1, v4s b0, v4s b1, v4f c)
+{
+    const v8s a = {a0[0], a0[1], a0[2], a0[3], a1[0], a1[1], a1[2], a1[3]};
+    const v8s b = {b0[0], b0[1], b0[2], b0[3], b1[0], b1[1], b1[2], b1[3]};
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, a), __builtin_bit_cast(v8bf, b), c,
+                                                    0, 0, 0);
 }
 
 // Block-resident network: bf16 weights (W2 also transposed, for dH1),
@@ -105,14 +125,76 @@ struct Fwd {
     float z;
 };
 
+// The weights one lane feeds to the matrix cores (row r = l%16, slice
+// g = l/16 of every 16-deep operand), read from the block's LDS copy at
+// each use ...
+struct LdsWeights {
+    const NetLds &S;
+    int r, g;
+    __device__ v4s w1(int t) const { return ld4(&S.w1[16 * t + r][4 * g]); }
+    __device__ v4s w2(int mt, int kt) const { return ld4(&S.w2[16 * mt + r][16 * kt + 4 * g]); }
+    __device__ v4f b1(int t) const { return *reinterpret_cast<const v4f *>(&S.b1[16 * t + 4 * g]); }
+    __device__ v4f b2(int mt) const { return *reinterpret_cast<const v4f *>(&S.b2[16 * mt + 4 * g]); }
+    __device__ float w3(int mt, int j) const { return S.w3[16 * mt + 4 * g + j]; }
+    __device__ float b3() const { return S.b3; }
+};
+
+// ... or from the parameter buffers in global memory (bf16 weights, W1
+// rows of 12 features zero-padded to 16 here; fp32 biases; w3 widened from
+// its bf16 copy) ...
+struct GlobalWeights {
+    const unsigned short *wb;
+    const float *pf;
+    int r, g;
+    __device__ v4s w1(int t) const
+    {
+        if (g * 4 >= NF) return v4s{0, 0, 0, 0};
+        return ld4(wb + OFF_W1 + (16 * t + r) * NF + 4 * g);
+    }
+    __device__ v4s w2(int mt, int kt) const { return ld4(wb + OFF_W2 + (16 * mt + r) * H + 16 * kt + 4 * g); }
+    __device__ v4f b1(int t) const { return *reinterpret_cast<const v4f *>(pf + OFF_B1 + 16 * t + 4 * g); }
+    __device__ v4f b2(int mt) const { return *reinterpret_cast<const v4f *>(pf + OFF_B2 + 16 * mt + 4 * g); }
+    __device__ float w3(int mt, int j) const { return bf2f(wb[OFF_W3 + 16 * mt + 4 * g + j]); }
+    __device__ float b3() const { return pf[OFF_B3]; }
+};
+
+// ... or held in registers for a whole loop over tiles (72 VGPRs: the
+// inference kernel, whose waves run many tiles each).
+struct RegWeights {
+    v4s w1_[4], w2_[4][4];
+    v4f b1_[4], b2_[4];
+    float w3_[16], b3_;
+    template <typename Src>
+    __device__ explicit RegWeights(const Src &L)
+    {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            w1_[t] = L.w1(t);
+            b1_[t] = L.b1(t);
+            b2_[t] = L.b2(t);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                w2_[t][k] = L.w2(t, k);
+                w3_[4 * t + k] = L.w3(t, k);
+            }
+        }
+        b3_ = L.b3();
+    }
+    __device__ v4s w1(int t) const { return w1_[t]; }
+    __device__ v4s w2(int mt, int kt) const { return w2_[mt][kt]; }
+    __device__ v4f b1(int t) const { return b1_[t]; }
+    __device__ v4f b2(int mt) const { return b2_[mt]; }
+    __device__ float w3(int mt, int j) const { return w3_[4 * mt + j]; }
+    __device__ float b3() const { return b3_; }
+};
+
 // x: B operand of layer 1 (lane: features 4*(l/16)+j of edge l%16).
-__device__ __forceinline__ void fwd_tile(const NetLds &S, v4s x, int lane, Fwd &F)
+template <typename Wt>
+__device__ __forceinline__ void fwd_tile(const Wt &W, v4s x, Fwd &F)
 {
-    const int r = lane & 15, g = lane >> 4;
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-        const v4f c = *reinterpret_cast<const v4f *>(&S.b1[16 * t + 4 * g]);
-        F.z1[t] = mfma(ld4(&S.w1[16 * t + r][4 * g]), x, c);
+        F.z1[t] = mfma(W.w1(t), x, W.b1(t));
         v4s h;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -123,10 +205,8 @@ __device__ __forceinline__ void fwd_tile(const NetLds &S, v4s x, int lane, Fwd &
     }
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) {
-        v4f c = *reinterpret_cast<const v4f *>(&S.b2[16 * mt + 4 * g]);
-#pragma unroll
-        for (int kt = 0; kt < 4; ++kt) c = mfma(ld4(&S.w2[16 * mt + r][16 * kt + 4 * g]), F.h1[kt], c);
-        F.z2[mt] = c;
+        v4f c = mfma2(W.w2(mt, 0), W.w2(mt, 1), F.h1[0], F.h1[1], W.b2(mt));
+        F.z2[mt] = mfma2(W.w2(mt, 2), W.w2(mt, 3), F.h1[2], F.h1[3], c);
     }
     float s = 0.0f;
 #pragma unroll
@@ -134,11 +214,11 @@ __device__ __forceinline__ void fwd_tile(const NetLds &S, v4s x, int lane, Fwd &
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const float h = F.z2[mt][j] > 0.0f ? F.z2[mt][j] : 0.0f;
-            s += S.w3[16 * mt + 4 * g + j] * h;
+            s += W.w3(mt, j) * h;
         }
     s += __shfl_xor(s, 16);
     s += __shfl_xor(s, 32);
-    F.z = s + S.b3;
+    F.z = s + W.b3();
 }
 
 // Accumulator-layout tile (rows 4g+j, column r) -> LDS, read back as the
@@ -171,22 +251,29 @@ __device__ __forceinline__ v4s load_x(const unsigned short *xb, long long e, boo
 
 // ---------------------------------------------------------------- kernels --
 
+// Inference: each wave takes tiles tile, tile + waves, ... with its weight
+// fragments in registers (read once from global memory: no LDS, no block
+// barrier) and the next tile's features requested before the current
+// tile's products.
 __global__ __launch_bounds__(kThreads) void k_forward(const unsigned short *xb, long long E,
                                                       const unsigned short *wb, const float *pf,
                                                       float *logits)
 {
-    __shared__ NetLds S;
-    load_net(S, wb, pf);
     const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
     const long long tiles = (E + 15) / 16;
     const long long waves = (long long)gridDim.x * kWaves;
-    for (long long tile = (long long)blockIdx.x * kWaves + (threadIdx.x >> 6); tile < tiles;
-         tile += waves) {
+    long long tile = (long long)blockIdx.x * kWaves + (threadIdx.x >> 6);
+    if (tile >= tiles) return;   // wave-uniform
+    const RegWeights W(GlobalWeights{wb, pf, r, g});
+    v4s x = load_x(xb, tile * 16 + r, tile * 16 + r < E, g);
+    for (; tile < tiles; tile += waves) {
         const long long e = tile * 16 + r;
-        const bool valid = e < E;
+        const long long en = (tile + waves) * 16 + r;
+        const v4s xn = load_x(xb, en, en < E, g);
         Fwd F;
-        fwd_tile(S, load_x(xb, e, valid, g), lane, F);
-        if (g == 0 && valid) logits[e] = F.z;
+        fwd_tile(W, x, F);
+        if (g == 0 && e < E) logits[e] = F.z;
+        x = xn;
     }
 }
 
@@ -221,7 +308,7 @@ __global__ __launch_bounds__(kThreads) void k_train(const unsigned short *xb, co
         const v4s x = load_x(xb, e, valid, g);
         const float y = valid ? label[e] : 0.0f;
         Fwd F;
-        fwd_tile(S, x, lane, F);
+        fwd_tile(LdsWeights{S, r, g}, x, F);
         const float we = valid ? (y > 0.5f ? pos_weight : 1.0f) : 0.0f;
         const float sig = 1.0f / (1.0f + expf(-F.z));
         const float ge = we * (sig - y) * invE;
@@ -247,9 +334,10 @@ __global__ __launch_bounds__(kThreads) void k_train(const unsigned short *xb, co
         v4s dz1b[4];
 #pragma unroll
         for (int it = 0; it < 4; ++it) {
-            v4f c = zero4;
-#pragma unroll
-            for (int ot = 0; ot < 4; ++ot) c = mfma(ld4(&S.w2t[16 * it + r][16 * ot + 4 * g]), dz2b[ot], c);
+            v4f c = mfma2(ld4(&S.w2t[16 * it + r][4 * g]), ld4(&S.w2t[16 * it + r][16 + 4 * g]), dz2b[0],
+                          dz2b[1], zero4);
+            c = mfma2(ld4(&S.w2t[16 * it + r][32 + 4 * g]), ld4(&S.w2t[16 * it + r][48 + 4 * g]), dz2b[2],
+                      dz2b[3], c);
             v4f d;
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
