@@ -22,7 +22,7 @@
 // adds them in.
 //
 // Determinism: per-wave partial gradients (fixed tile assignment) are summed
-// by the reduce kernel in wave order; there are no atomics.
+// by the reduce kernel in a fixed order; there are no atomics.
 
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -406,20 +406,35 @@ __global__ __launch_bounds__(kThreads) void k_train(const unsigned short *xb, co
     }
 }
 
-// One thread per parameter: the wave-ordered sum of the partial rows, then
-// (update != 0) Adam in fp32.  Thread NP handles the loss.
+// Per parameter: the sum of the partial rows in a fixed order (runs of
+// consecutive waves, then the runs in order), then (update != 0) Adam in
+// fp32.  Parameter NP is the loss.
 struct AdamArgs {
     float lr, beta1, om_beta1, beta2, om_beta2, c1, c2, eps;
 };
 
-__global__ void k_reduce_adam(const float *partials, int W, float *pf, float *m, float *v,
-                              unsigned short *wb, float *grad_out, float *loss_out, int update,
-                              AdamArgs a)
+// A block takes kReduceParams parameters; each is summed by kReduceGroups
+// threads over consecutive runs of the partial rows, then the runs' sums are
+// added in run order (a fixed order: runs are deterministic).
+constexpr int kReduceParams = 32, kReduceGroups = 8;
+
+__global__ __launch_bounds__(kReduceParams * kReduceGroups) void k_reduce_adam(
+    const float *partials, int W, float *pf, float *m, float *v, unsigned short *wb, float *grad_out,
+    float *loss_out, int update, AdamArgs a)
 {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p > NP) return;
-    float s = 0.0f;
-    for (int w = 0; w < W; ++w) s += partials[(size_t)w * NPL + p];
+    __shared__ float red[kReduceGroups][kReduceParams];
+    const int i = threadIdx.x % kReduceParams, q = threadIdx.x / kReduceParams;
+    const int p = blockIdx.x * kReduceParams + i;
+    const int run = (W + kReduceGroups - 1) / kReduceGroups;
+    const int w0 = q * run, w1 = min(W, w0 + run);
+    float part = 0.0f;
+    if (p <= NP)
+        for (int w = w0; w < w1; ++w) part += partials[(size_t)w * NPL + p];
+    red[q][i] = part;
+    __syncthreads();
+    if (q != 0 || p > NP) return;
+    float s = red[0][i];
+    for (int g = 1; g < kReduceGroups && g * run < W; ++g) s += red[g][i];
     if (p == NP) {
         if (loss_out) *loss_out = s;
         return;
@@ -693,7 +708,8 @@ static int launch_batch(aco_prior *net, const unsigned short *xb, const float *l
     hipLaunchKernelGGL(k_train, dim3(blocks), dim3(kThreads), 0, net->stream, xb, label, idx, Eb,
                        net->wb, net->pf, pos_weight, 1.0f / (float)Eb, partials);
     PCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_reduce_adam, dim3((NPL + 255) / 256), dim3(256), 0, net->stream, partials,
+    hipLaunchKernelGGL(k_reduce_adam, dim3((NPL + kReduceParams - 1) / kReduceParams),
+                       dim3(kReduceParams * kReduceGroups), 0, net->stream, partials,
                        blocks * kWaves, net->pf, net->m, net->v, net->wb, grad_out, loss_out,
                        update ? 1 : 0, aa);
     PCHK(hipGetLastError());
@@ -910,7 +926,8 @@ int aco_prior_adam_step(aco_prior *net, const float *grad, float lr, float beta1
     PCHK(hipMemsetAsync(part.p, 0, NPL * 4, net->stream));
     PCHK(hipMemcpyAsync(part.p, grad, NP * 4, hipMemcpyHostToDevice, net->stream));
     net->step += 1;
-    hipLaunchKernelGGL(k_reduce_adam, dim3((NPL + 255) / 256), dim3(256), 0, net->stream, part.as<float>(), 1,
+    hipLaunchKernelGGL(k_reduce_adam, dim3((NPL + kReduceParams - 1) / kReduceParams),
+                       dim3(kReduceParams * kReduceGroups), 0, net->stream, part.as<float>(), 1,
                        net->pf, net->m, net->v, net->wb, nullptr, nullptr, 1,
                        adam_args(net->step, lr, beta1, beta2, eps));
     PCHK(hipGetLastError());
