@@ -82,6 +82,8 @@ def parse():
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes) and exit")
     ap.add_argument("--lds-probe", action="store_true",
                     help="only run the LDS bandwidth microbenchmark and exit")
+    ap.add_argument("--roof-probe", action="store_true",
+                    help="only run the roof microbenchmarks (LDS read, HBM copy, BF16 MFMA) and exit")
     ap.add_argument("--ls-phases", action="store_true",
                     help="only split the workload's local search into its phases (2-opt, swap, "
                          "relocate, all three) on the same constructed tours and exit")
@@ -116,6 +118,37 @@ def lds_peak():
     if rc != 0:
         raise RuntimeError(f"probe_lds_bandwidth failed ({rc})")
     return gbps.value, ms.value
+
+
+def _probe_lib():
+    lib = ctypes.CDLL(os.path.join(ROOT, "cvrp_aco_ocl_amd", "libcvrpprobe.so"))
+    D = ctypes.POINTER(ctypes.c_double)
+    lib.probe_hbm_copy.argtypes = [ctypes.c_int, ctypes.c_int, D, D]
+    lib.probe_mfma_bf16.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, D, D]
+    return lib
+
+
+def measured_peaks() -> dict:
+    """The device's roofs measured by libcvrpprobe.so in this run, beside the
+    published MI355X figures: LDS read bandwidth (the local search's roof),
+    HBM copy bandwidth (2 x 1 GiB buffers, read + write bytes; the pheromone
+    pass's roof) and the dense BF16 matrix-core rate of the two gfx950 BF16
+    MFMA shapes (4 waves per SIMD, 4 independent chains each; the prior MLP's
+    roof).  Best of 5 launches each, about a second in all."""
+    lib = _probe_lib()
+    lds, _ = lds_peak()
+    v, ms = ctypes.c_double(), ctypes.c_double()
+    if lib.probe_hbm_copy(5, 1024, ctypes.byref(v), ctypes.byref(ms)) != 0:
+        raise RuntimeError("probe_hbm_copy failed")
+    hbm, hbm_ms = v.value, ms.value
+    mfma = {}
+    for shape, name in ((0, "16x16x32"), (1, "32x32x16")):
+        if lib.probe_mfma_bf16(5, 2048, shape, 4, ctypes.byref(v), ctypes.byref(ms)) != 0:
+            raise RuntimeError("probe_mfma_bf16 failed")
+        mfma[name] = {"TFLOPs": round(v.value, 1), "best_launch_ms": round(ms.value, 4)}
+    return {"lds_read_GBps": round(lds, 1), "hbm_copy_GBps": round(hbm, 1),
+            "hbm_copy_best_launch_ms": round(hbm_ms, 4), "mfma_bf16": mfma,
+            "published": {"hbm_GBps": HBM_PEAK_GBS, "mfma_bf16_dense_TFLOPs": 2500.0}}
 
 
 def ls_phases(shape) -> None:
@@ -524,6 +557,9 @@ def main():
         gbps, ms = lds_peak()
         print(json.dumps({"lds_read_GBps": round(gbps, 1), "best_launch_ms": round(ms, 4)}))
         return
+    if args.roof_probe:
+        print(json.dumps({"peaks_measured": measured_peaks()}))
+        return
     if args.mlp_probe:
         mlp_probe(args.mlp_probe)
         return
@@ -564,7 +600,8 @@ def main():
     kern, per_kernel, ls_evals = profile_iterations(g, inst, m, k, args.profile_steps)
     g.close()
     dominant = max(kern, key=kern.get)
-    peak_lds, _ = lds_peak()
+    peaks = measured_peaks()
+    peak_lds = peaks["lds_read_GBps"]
     log(f"LDS read peak {peak_lds:.0f} GB/s; dominant kernel {dominant} {kern[dominant]:.3f} ms")
     if dominant == "local_search":
         # 16 B per evaluated move (the four distance / edge words of a 2-opt,
@@ -589,6 +626,7 @@ def main():
         hbm_pass = {"bound": "hbm", "kernel": "pheromone", "achieved": ph["GBps"],
                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(ph["GBps"] / HBM_PEAK_GBS, 5),
+                    "frac_of_measured_copy": round(ph["GBps"] / peaks["hbm_copy_GBps"], 5),
                     "note": "4 MB per matrix at this size: resident in the last-level cache"}
 
     quality = leg_quality(ca, inst, shape, args.full_iterations, args.seed) \
@@ -635,6 +673,7 @@ def main():
         },
         "roofline": roofline,
         "roofline_pheromone_pass": hbm_pass,
+        "peaks_measured": peaks,
         "kernels": per_kernel,
         "ls_move_evals_per_iter": int(ls_evals),
         "autotune_ms": {f"{STAGES[st]}{sh}": round(ms, 3) for sh, st, ms in tune},

@@ -16,7 +16,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(PKG, "libcvrpaco.so")
-PROBE = os.path.join(PKG, "libcvrpprobe.so")   # LDS microbenchmark (bench.py's LDS roof)
+PROBE = os.path.join(PKG, "libcvrpprobe.so")   # roof microbenchmarks: LDS, HBM copy, BF16 MFMA
 CLI = os.path.join(PKG, "cvrp_aco")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("CVRP_OFFLOAD_ARCH", "gfx950")
@@ -53,7 +53,8 @@ def build(verbose: bool = True) -> None:
     tmp = LIB + ".tmp"
     _run([HIPCC, *DEVICE_FLAGS, "-shared", "-o", tmp, *srcs], verbose)
     os.replace(tmp, LIB)
-    _run([HIPCC, *DEVICE_FLAGS, "-shared", "-o", PROBE + ".tmp", os.path.join(CSRC, "probe_lds.hip")], verbose)
+    _run([HIPCC, *DEVICE_FLAGS, "-shared", "-o", PROBE + ".tmp",
+          *[os.path.join(CSRC, s) for s in ("probe_lds.hip", "probe_roofs.hip")]], verbose)
     os.replace(PROBE + ".tmp", PROBE)
     cli_srcs = [os.path.join(CSRC, s) for s in CLI_SOURCES if os.path.exists(os.path.join(CSRC, s))]
     if len(cli_srcs) == len(CLI_SOURCES):
