@@ -5,8 +5,11 @@
 // the published MI355X figures.
 //
 //   k_copy       grid-stride copy of 16-byte elements from one buffer to
-//                another (each byte read once and written once); bytes =
-//                2 x buffer size per launch.
+//                another (each byte read once and written once), 4 blocks
+//                of 256 threads per CU; bytes = 2 x buffer size per launch.
+//                Of the variants tried (1, 2 or 4 elements per thread per
+//                trip, plain or streaming accesses, 4 to 32 blocks per CU,
+//                and hipMemcpy device to device) this one moved the most.
 //   k_mfma_bf16  every wave runs `trips` trips of 4 independent MFMA chains
 //                on register operands (no memory traffic in the loop):
 //                shape 0 = v_mfma_f32_16x16x32_bf16 (16384 FLOP each),
@@ -26,8 +29,9 @@ typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef short v8s __attribute__((ext_vector_type(8)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef float v16f __attribute__((ext_vector_type(16)));
+typedef unsigned v4u __attribute__((ext_vector_type(4)));
 
-__global__ __launch_bounds__(kThreads) void k_copy(const uint4 *src, uint4 *dst, size_t count)
+__global__ __launch_bounds__(kThreads) void k_copy(const v4u *src, v4u *dst, size_t count)
 {
     const size_t stride = (size_t)gridDim.x * kThreads;
     for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < count; i += stride) dst[i] = src[i];
@@ -125,15 +129,15 @@ extern "C" int probe_hbm_copy(int reps, int mib, double *gbps, double *ms_best)
     if (reps < 1 || mib < 1 || mib > 16384 || !gbps) return -1;
     int cus = 0;
     if (device_cus(cus)) return -2;
-    const size_t bytes = (size_t)mib << 20, count = bytes / sizeof(uint4);
-    uint4 *src = nullptr, *dst = nullptr;
+    const size_t bytes = (size_t)mib << 20, count = bytes / sizeof(v4u);
+    v4u *src = nullptr, *dst = nullptr;
     if (hipMalloc(&src, bytes) != hipSuccess) return -2;
     if (hipMalloc(&dst, bytes) != hipSuccess) {
         (void)hipFree(src);
         return -2;
     }
     int rc = hipMemset(src, 1, bytes) == hipSuccess ? 0 : -2;
-    const dim3 grid(cus * 8), block(kThreads);
+    const dim3 grid(cus * 4), block(kThreads);
     const float ms = rc ? -1.0f : best_ms(reps, [&] { hipLaunchKernelGGL(k_copy, grid, block, 0, 0, src, dst, count); });
     (void)hipFree(src);
     (void)hipFree(dst);
