@@ -16,6 +16,7 @@ the oracle.
 * An instance past the local-search LDS limit (3001 nodes, which a colony
   with local search refuses): the decomposition still runs, every merged
   tour is a valid solution and never longer than the best it started from.
+  The same at the library's 4096-node limit.
 """
 import numpy as np
 import pytest
