@@ -38,6 +38,14 @@ def test_every_declared_symbol_is_exported():
     assert set(names) == set(_lib.EXPORTED)
 
 
+def test_probe_library_exports_the_roof_microbenchmarks():
+    """libcvrpprobe.so (bench.py's measured roofs) loads and exports its
+    three entry points; they are only called on a GPU."""
+    lib = C.CDLL(os.path.join(ROOT, "cvrp_aco_ocl_amd", "libcvrpprobe.so"))
+    for name in ("probe_lds_bandwidth", "probe_hbm_copy", "probe_mfma_bf16"):
+        assert hasattr(lib, name), name
+
+
 def test_default_params_are_the_references():
     p = _lib.AcoParams()
     _lib.lib.aco_default_params(C.byref(p))
