@@ -355,13 +355,13 @@ __global__ __launch_bounds__(64) void k_construct_iroulette(DevBatch bt)
     s.pm[a * kLanes + lane] = z;
 }
 
-// The same construction with four waves per ant, for launches with at most
-// one ant per CU (otherwise the single-wave kernel keeps the CU's other SIMDs
-// busy with other ants).  A lane's draws stay its own Park-Miller stream:
-// node l + 64u uses the stream's (u+1)-th draw of the step, z * 16807^(u+1)
-// mod (2^31 - 1), so wave w takes u = w, w+4, ... starting from the jump
-// 16807^(w+1) and stepping by 16807^4; after the step every wave advances
-// the lane's state by the lane's node count.  The same keys as the
+// The same construction with W = 4 or 8 waves per ant, for launches with at
+// most one ant per CU (otherwise the single-wave kernel keeps the CU's other
+// SIMDs busy with other ants).  A lane's draws stay its own Park-Miller
+// stream: node l + 64u uses the stream's (u+1)-th draw of the step,
+// z * 16807^(u+1) mod (2^31 - 1), so wave w takes u = w, w+W, ... starting
+// from the jump 16807^(w+1) and stepping by 16807^W; after the step every
+// wave advances the lane's state by the lane's node count.  The same keys as the
 // single-wave kernel; the waves' maxima meet in LDS (one barrier per step,
 // two buffers by step parity), and every wave sets the visited flag itself,
 // so it reads its own write in the next step.
@@ -375,10 +375,14 @@ __device__ __forceinline__ int pm_mulmod(int z, unsigned a)
     return (int)r;
 }
 
-constexpr int kIrWaves = 4;
+// Waves per ant: 4, or 8 when the instance has more than 512 nodes (two
+// waves per SIMD hide more of each step's latency; on smaller rows most of
+// an eighth-wave's lanes would have no node).
+constexpr int kIrWavesMax = 8;
+__host__ __device__ constexpr int iroulette_waves(int n) { return n > 512 ? 8 : 4; }
 
-template <bool HAS_D>
-__global__ __launch_bounds__(256) void k_construct_iroulette4(DevBatch bt)
+template <bool HAS_D, int kIrWaves>
+__global__ __launch_bounds__(kIrWaves * 64) void k_construct_iroulette4(DevBatch bt)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     ANT_PROLOGUE
@@ -778,7 +782,8 @@ static hipError_t max_static_lds(const void *const *fns, int count, size_t &byte
 hipError_t construct_static_lds(size_t &bytes)
 {
     const void *fns[] = {(const void *)k_construct_iroulette<false>, (const void *)k_construct_iroulette<true>,
-                         (const void *)k_construct_iroulette4<false>, (const void *)k_construct_iroulette4<true>,
+                         (const void *)k_construct_iroulette4<false, 4>, (const void *)k_construct_iroulette4<true, 4>,
+                         (const void *)k_construct_iroulette4<false, 8>, (const void *)k_construct_iroulette4<true, 8>,
                          (const void *)k_construct_knn<false, kRngPhilox, true>,
                          (const void *)k_construct_knn<true, kRngPhilox, true>,
                          (const void *)k_construct_knn<false, kRngParkMiller, true>,
@@ -816,10 +821,16 @@ hipError_t launch_construct(const DevBatch &b, int selection, int rng, int apb, 
     if (selection == kSelIRoulette) {
         if (b.ants <= b.cus) {
             // at most one ant per CU: four waves per ant
-            const size_t lds4 = iroulette4_exchange_offset(b.max_n) + 2 * kIrWaves * sizeof(u64);
-            const dim3 block4(kIrWaves * kLanes);
-            if (b.has_d) hipLaunchKernelGGL(k_construct_iroulette4<true>, grid, block4, lds4, st, b);
-            else hipLaunchKernelGGL(k_construct_iroulette4<false>, grid, block4, lds4, st, b);
+            const size_t lds4 = iroulette4_exchange_offset(b.max_n) + 2 * kIrWavesMax * sizeof(u64);
+            const int nw = iroulette_waves(b.max_n);
+            const dim3 block4(nw * kLanes);
+            if (nw == 8) {
+                if (b.has_d) hipLaunchKernelGGL((k_construct_iroulette4<true, 8>), grid, block4, lds4, st, b);
+                else hipLaunchKernelGGL((k_construct_iroulette4<false, 8>), grid, block4, lds4, st, b);
+            } else {
+                if (b.has_d) hipLaunchKernelGGL((k_construct_iroulette4<true, 4>), grid, block4, lds4, st, b);
+                else hipLaunchKernelGGL((k_construct_iroulette4<false, 4>), grid, block4, lds4, st, b);
+            }
         } else if (b.has_d) {
             hipLaunchKernelGGL(k_construct_iroulette<true>, grid, block, lds, st, b);
         } else {
