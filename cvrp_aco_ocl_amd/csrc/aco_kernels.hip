@@ -266,6 +266,43 @@ __device__ __forceinline__ float tour_length_in_order(const DevProblem &p, const
 
 constexpr int kIrChunk = 16;   // I-Roulette: nodes per lane whose loads are issued together
 
+// C nodes of one lane for an I-Roulette pass: their row values (and
+// distances), visited flags, demands, ranks (and depot distances), all
+// requested before any is used (clamped in-row indices; nodes >= n are
+// skipped by the caller), then the key of kernel.cl's reduction,
+// (prob, 4095 - rank, node), with the feasibility mask of kernel.cl:445-452
+// evaluated without branches.
+template <bool HAS_D, int C>
+struct IrNodes {
+    float tv[C], dv[C], d0[C];
+    int vis[C], dem[C], rank[C];
+    __device__ __forceinline__ void load(const DevProblem &p, const AntLds &L, const float *tirow,
+                                         const float *drow, int j, int i)
+    {
+        const int ic = i < p.n ? i : p.n - 1;
+        tv[j] = gload(tirow + ic);
+        dv[j] = HAS_D ? gload(drow + ic) : 0.0f;
+        d0[j] = HAS_D ? L.d0[ic] : 0.0f;
+        vis[j] = L.vis[ic];
+        dem[j] = L.dem[ic];
+        rank[j] = L.rank[ic];
+    }
+    __device__ __forceinline__ u64 key(const DevProblem &p, int j, int i, float r, int load, float rdist,
+                                       u64 best) const
+    {
+        const float pp = tv[j] * 100.0f;
+        bool c = (vis[j] == 0) & (load + dem[j] <= p.Q);
+        if (HAS_D) {
+            float t = rdist + (dv[j] + p.svc);
+            t = t + d0[j];
+            c = c & (t <= p.D);
+        }
+        const float prob = c ? r * pp : 0.0f;
+        const u64 k = ((u64)f2u(prob) << 32) | ((u64)(4095 - rank[j]) << 12) | (u64)i;
+        return k > best ? k : best;
+    }
+};
+
 template <bool HAS_D>
 __global__ __launch_bounds__(64) void k_construct_iroulette(DevBatch bt)
 {
@@ -288,28 +325,19 @@ __global__ __launch_bounds__(64) void k_construct_iroulette(DevBatch bt)
         const float *tirow = s.tinfo + (size_t)cur * p.P;
         const float *drow = p.dist + (size_t)cur * p.P;
         // the row's values (and distances) for 16 nodes per lane are loaded
-        // together, then the lane draws for them in node order: one memory
-        // latency per 1024 nodes instead of one per node
+        // together with their LDS fields, then the lane draws for them in node
+        // order: one memory latency per 1024 nodes instead of one per node
         u64 best = 0;
         for (int base = 0; base < n; base += kIrChunk * kLanes) {
-            float tv[kIrChunk], dv[kIrChunk];
+            IrNodes<HAS_D, kIrChunk> F;
 #pragma unroll
-            for (int u = 0; u < kIrChunk; ++u) {
-                const int i = base + u * kLanes + lane;
-                const int ic = i < n ? i : n - 1;
-                tv[u] = gload(tirow + ic);
-                dv[u] = HAS_D ? gload(drow + ic) : 0.0f;
-            }
+            for (int u = 0; u < kIrChunk; ++u) F.load(p, L, tirow, drow, u, base + u * kLanes + lane);
 #pragma unroll
             for (int u = 0; u < kIrChunk; ++u) {
                 const int i = base + u * kLanes + lane;
                 if (i < n) {
                     const float r = pm_ran01(z) + 0.001f;
-                    const float pp = tv[u] * 100.0f;
-                    const bool c = feasible<HAS_D>(p, L, i, dv[u], load, rdist);
-                    const float prob = c ? r * pp : 0.0f;
-                    const u64 key = ((u64)f2u(prob) << 32) | ((u64)(4095 - L.rank[i]) << 12) | (u64)i;
-                    best = key > best ? key : best;
+                    best = F.key(p, u, i, r, load, rdist, best);
                 }
             }
         }
@@ -327,7 +355,9 @@ __global__ __launch_bounds__(64) void k_construct_iroulette(DevBatch bt)
             w = (int)(best & 0xFFFull);
         }
         w = uniform(w);
-        const float dcw = drow[w];
+        // without a length limit the loop carries no edge lengths (summed
+        // afterwards in the same order), so no step waits for one
+        const float dcw = HAS_D ? drow[w] : 0.0f;
         if (w == 0) {
             load = 0;
             rdist = 0.0f;
@@ -345,6 +375,7 @@ __global__ __launch_bounds__(64) void k_construct_iroulette(DevBatch bt)
     tdist = tdist + L.d0[cur];
     step++;
     tour_close(tour, tbuf, step, lane);
+    if (!HAS_D) tdist = tour_length_in_order(p, tour, step, lane);
     if (lane == 0) {
         s.sizes[a] = step + 1;
         s.lens[a] = tdist;
@@ -410,27 +441,18 @@ __global__ __launch_bounds__(kIrWaves * 64) void k_construct_iroulette4(DevBatch
         const float *drow = p.dist + (size_t)cur * p.P;
         u64 best = 0;
         int zz = pm_mulmod(z, aw);
-        // pass over u = base + w + 4j (node lane + 64u), j < C
+        // pass over u = base + w + Wj (node lane + 64u), j < C
         for (int ub = 0; ub * kLanes < n; ub += kIrChunk) {
-            float tv[C], dv[C];
+            IrNodes<HAS_D, C> F;
 #pragma unroll
-            for (int j = 0; j < C; ++j) {
-                const int i = lane + kLanes * (ub + w + kIrWaves * j);
-                const int ic = i < n ? i : n - 1;
-                tv[j] = gload(tirow + ic);
-                dv[j] = HAS_D ? gload(drow + ic) : 0.0f;
-            }
+            for (int j = 0; j < C; ++j) F.load(p, L, tirow, drow, j, lane + kLanes * (ub + w + kIrWaves * j));
 #pragma unroll
             for (int j = 0; j < C; ++j) {
                 const int i = lane + kLanes * (ub + w + kIrWaves * j);
                 if (i < n) {
                     const float r = (float)zz * 0x1p-31f + 0.001f;
                     zz = pm_mulmod(zz, a4);
-                    const float pp = tv[j] * 100.0f;
-                    const bool c = feasible<HAS_D>(p, L, i, dv[j], load, rdist);
-                    const float prob = c ? r * pp : 0.0f;
-                    const u64 key = ((u64)f2u(prob) << 32) | ((u64)(4095 - L.rank[i]) << 12) | (u64)i;
-                    best = key > best ? key : best;
+                    best = F.key(p, j, i, r, load, rdist, best);
                 }
             }
         }
@@ -457,7 +479,7 @@ __global__ __launch_bounds__(kIrWaves * 64) void k_construct_iroulette4(DevBatch
             wn = (int)(best & 0xFFFull);
         }
         wn = uniform(wn);
-        const float dcw = drow[wn];
+        const float dcw = HAS_D ? drow[wn] : 0.0f;   // without a limit: summed afterwards
         if (wn == 0) {
             load = 0;
             rdist = 0.0f;
@@ -475,6 +497,7 @@ __global__ __launch_bounds__(kIrWaves * 64) void k_construct_iroulette4(DevBatch
     step++;
     if (w == 0) {
         tour_close(tour, tbuf, step, lane);
+        if (!HAS_D) tdist = tour_length_in_order(p, tour, step, lane);
         if (lane == 0) {
             s.sizes[a] = step + 1;
             s.lens[a] = tdist;
