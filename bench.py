@@ -247,9 +247,11 @@ def construct_bytes(inst, sizes, m, k, fallbacks):
     return steps * (8 * k + 8) + m * 8 * n + fallbacks * 4 * n
 
 
-def pheromone_bytes(n):
-    """Fused pass: read tau and heur, write tau and total_info (16 n^2 B)."""
-    return 16 * n * n
+def pheromone_bytes(n, k):
+    """Fused pass: read tau and heur, write tau and total_info (16 n^2 B),
+    and the candidate rows' total_info: read the ids, write the values
+    (8 n k B)."""
+    return 16 * n * n + 8 * n * k
 
 
 def cpu_share():
@@ -298,8 +300,8 @@ def profile_iterations(g, inst, m, k, steps):
             e["algorithmic_bytes"] = int(cons_bytes / steps)
             e["GBps"] = round(cons_bytes / steps / (ms * 1e-3) / 1e9, 2)
         elif name == "pheromone":
-            e["algorithmic_bytes"] = pheromone_bytes(inst.n)
-            e["GBps"] = round(pheromone_bytes(inst.n) / (ms * 1e-3) / 1e9, 2)
+            e["algorithmic_bytes"] = pheromone_bytes(inst.n, k)
+            e["GBps"] = round(pheromone_bytes(inst.n, k) / (ms * 1e-3) / 1e9, 2)
         elif name == "local_search":
             e["move_evals_per_launch"] = int(ls_evals)
             e["move_evals_per_s"] = round(ls_evals / (ms * 1e-3), 1)

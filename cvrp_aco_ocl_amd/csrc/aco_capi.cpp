@@ -296,10 +296,11 @@ static int run_gather(aco_ctx *c)
     return ACO_OK;
 }
 
+// The pheromone pass also writes the candidate rows' total_info (no gather).
 static int run_pheromone(aco_ctx *c)
 {
     HIP_TRY(timed(c, "pheromone", [&] { return launch_pheromone_update(c->bt, c->stream); }));
-    return run_gather(c);
+    return ACO_OK;
 }
 
 static int run_iteration_steps(aco_ctx *c)
@@ -1406,7 +1407,6 @@ int aco_autotune(aco_ctx *c, int32_t reps, aco_tune_result *results, int32_t cap
             if (e == hipSuccess) e = launch_local_search(c->bt, c->prm.ls_flags, sh, c->stream, &c->lss);
             if (e == hipSuccess) e = launch_statistics(c->bt, c->tick_to_s, c->stream);
             if (e == hipSuccess) e = launch_pheromone_update(c->bt, c->stream);
-            if (e == hipSuccess) e = launch_gather(c->bt, c->stream);
             return e;
         }, ti[q]);
         if (err == hipSuccess && (best_i < 0 || ti[q] < ti[best_i])) best_i = q;

@@ -1,5 +1,6 @@
 // HIP kernels (gfx950) for one ACO iteration: solution construction,
-// iteration statistics, the fused pheromone pass and candidate-row gather.
+// iteration statistics, the fused pheromone pass (candidate rows included)
+// and the candidate-row gather used where no pass ran.
 // Local search lives in aco_local_search.hip.
 //
 // Arithmetic contract: compiled with -ffp-contract=off, IEEE division and
@@ -1052,8 +1053,8 @@ constexpr int kPhF4 = 128;   // float4 per work item (row segment)
 
 __device__ __forceinline__ int pheromone_items(const DevProblem &p) { return p.n * ((p.P / 4 + kPhF4 - 1) / kPhF4); }
 
-__device__ __forceinline__ void pheromone_f4(const DevProblem &p, const DevState &s, int j, int c4, float evap,
-                                             const int *sc, const float *dl, float4 t, float4 h, size_t off)
+__device__ __forceinline__ float4 pheromone_f4(const DevProblem &p, const DevState &s, int j, int c4, float evap,
+                                               const int *sc, const float *dl, float4 t, float4 h, size_t off)
 {
     u32 ff[kMaxRas];
 #pragma unroll
@@ -1079,7 +1080,14 @@ __device__ __forceinline__ void pheromone_f4(const DevProblem &p, const DevState
         ti[c] = (p.alpha == 1.0f ? x : powf(x, p.alpha)) * hv[c];
     }
     *reinterpret_cast<float4 *>(s.tau + off) = make_float4(v[0], v[1], v[2], v[3]);
-    *reinterpret_cast<float4 *>(s.tinfo + off) = make_float4(ti[0], ti[1], ti[2], ti[3]);
+    const float4 out = make_float4(ti[0], ti[1], ti[2], ti[3]);
+    *reinterpret_cast<float4 *>(s.tinfo + off) = out;
+    return out;
+}
+
+__device__ __forceinline__ float f4_get(const float4 &v, int c)
+{
+    return c == 0 ? v.x : c == 1 ? v.y : c == 2 ? v.z : v.w;
 }
 
 // Work item `wave` (row j, kPhF4 float4 of it) of one instance's pass.
@@ -1112,8 +1120,25 @@ __device__ __forceinline__ void pheromone_item(const DevProblem &p, const DevSta
         tb = *reinterpret_cast<const float4 *>(s.tau + offb);
         hb = *reinterpret_cast<const float4 *>(p.heur + offb);
     }
-    if (a_in) pheromone_f4(p, s, j, c4a, evap, sc, dl, ta, ha, offa);
-    if (b_in) pheromone_f4(p, s, j, c4b, evap, sc, dl, tb, hb, offb);
+    float4 ia = make_float4(0, 0, 0, 0), ib = ia;
+    if (a_in) ia = pheromone_f4(p, s, j, c4a, evap, sc, dl, ta, ha, offa);
+    if (b_in) ib = pheromone_f4(p, s, j, c4b, evap, sc, dl, tb, hb, offb);
+    // the candidate row's total_info (tiknn[j][c] = total_info[j][nnk[j][c]])
+    // for the candidates whose column lies in this segment: lane l holds
+    // columns c0 + 4l.. and c0 + 256 + 4l.., so the value comes from lane
+    // ((col - c0) mod 256) / 4 by a cross-lane read
+    const int c0 = (c4a - lane) * 4;
+    for (int cb = 0; cb < p.k; cb += kLanes) {
+        const int c = cb + lane;
+        const int col = c < p.k ? p.nnk[(size_t)j * p.k + c] : -1;
+        const int rel = col - c0;
+        const int src = (rel & 255) >> 2;
+        float4 va, vb;
+        va.x = __shfl(ia.x, src); va.y = __shfl(ia.y, src); va.z = __shfl(ia.z, src); va.w = __shfl(ia.w, src);
+        vb.x = __shfl(ib.x, src); vb.y = __shfl(ib.y, src); vb.z = __shfl(ib.z, src); vb.w = __shfl(ib.w, src);
+        if (c < p.k && rel >= 0 && rel < 4 * kPhF4)
+            s.tiknn[(size_t)j * p.k + c] = f4_get(rel < 256 ? va : vb, rel & 3);
+    }
 }
 
 __global__ __launch_bounds__(256) void k_pheromone_update(DevBatch bt)
@@ -1246,9 +1271,7 @@ __global__ __launch_bounds__(W * 64, W == 4 ? 3 : 4) void k_iteration_fused(DevB
             __syncthreads();
             const int np = pheromone_items(p);
 #pragma unroll 1
-            for (int w = wv; w < np; w += W) pheromone_item(p, s, w, lane);
-            __syncthreads();
-            for (int e = tid; e < p.n * p.k; e += NT) s.tiknn[e] = s.tinfo[(size_t)(e / p.k) * p.P + p.nnk[e]];
+            for (int w = wv; w < np; w += W) pheromone_item(p, s, w, lane);   // candidate rows included
             if (tid == 0) s.ctrl[kCtrlDone] = 0;
         }
         __syncthreads();
