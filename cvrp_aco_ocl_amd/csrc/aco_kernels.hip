@@ -1085,10 +1085,6 @@ __device__ __forceinline__ float4 pheromone_f4(const DevProblem &p, const DevSta
     return out;
 }
 
-__device__ __forceinline__ float f4_get(const float4 &v, int c)
-{
-    return c == 0 ? v.x : c == 1 ? v.y : c == 2 ? v.z : v.w;
-}
 
 // Work item `wave` (row j, kPhF4 float4 of it) of one instance's pass.
 __device__ __forceinline__ void pheromone_item(const DevProblem &p, const DevState &s, int wave, int lane)
@@ -1133,11 +1129,19 @@ __device__ __forceinline__ void pheromone_item(const DevProblem &p, const DevSta
         const int col = c < p.k ? p.nnk[(size_t)j * p.k + c] : -1;
         const int rel = col - c0;
         const int src = (rel & 255) >> 2;
-        float4 va, vb;
-        va.x = __shfl(ia.x, src); va.y = __shfl(ia.y, src); va.z = __shfl(ia.z, src); va.w = __shfl(ia.w, src);
-        vb.x = __shfl(ib.x, src); vb.y = __shfl(ib.y, src); vb.z = __shfl(ib.z, src); vb.w = __shfl(ib.w, src);
-        if (c < p.k && rel >= 0 && rel < 4 * kPhF4)
-            s.tiknn[(size_t)j * p.k + c] = f4_get(rel < 256 ? va : vb, rel & 3);
+        // every lane takes part in all eight cross-lane reads (a read inside a
+        // conditional would run with the other lanes inactive), then plain
+        // selects pick the value (an indexed pick from a float4 would go
+        // through scratch memory)
+        const float a0 = __shfl(ia.x, src), a1 = __shfl(ia.y, src), a2 = __shfl(ia.z, src),
+                    a3 = __shfl(ia.w, src);
+        const float b0 = __shfl(ib.x, src), b1 = __shfl(ib.y, src), b2 = __shfl(ib.z, src),
+                    b3 = __shfl(ib.w, src);
+        const bool hi = rel >= 256;
+        const int e = rel & 3;
+        const float x0 = hi ? b0 : a0, x1 = hi ? b1 : a1, x2 = hi ? b2 : a2, x3 = hi ? b3 : a3;
+        const float x = e == 0 ? x0 : e == 1 ? x1 : e == 2 ? x2 : x3;
+        if (c < p.k && rel >= 0 && rel < 4 * kPhF4) s.tiknn[(size_t)j * p.k + c] = x;
     }
 }
 
