@@ -42,7 +42,7 @@ STAGES = {0: "construct", 1: "local_search", 2: "iteration"}
 HBM_PEAK_GBS = 8000.0
 WORKLOAD = dict(m=256, k=32, ls=7, iterations_full=2000)
 CLUSTERED = dict(customers=480, clusters=8, sigma=6.0, routes=30, m=480, k=30, ls=7)
-KERNELS = ("construct", "local_search", "statistics", "pheromone", "gather")
+KERNELS = ("construct", "local_search", "statistics", "pheromone")
 
 
 def parse():
