@@ -16,10 +16,10 @@
 // 16 edges of a tile) use v_mfma_f32_16x16x16_bf16.  The 64-deep products
 // (layer 2 and dH1 = W2^T dz2) use gfx950's v_mfma_f32_16x16x32_bf16, two
 // per 16-row block: its A and B operands are two 4-element halves put side
-// by side, each half one 16-deep slice in the layout above.  Element i of a lane's A operand
-// and element i of its B operand carry the same index of the contraction,
-// so the product sums exactly the slices' terms, whatever order the unit
-// adds them in.
+// by side, each half one 16-deep slice in the layout above.  Element i of a
+// lane's A operand and element i of its B operand carry the same index of
+// the contraction, so the product sums exactly the slices' terms, whatever
+// order the unit adds them in.
 //
 // Determinism: per-wave partial gradients (fixed tile assignment) are summed
 // by the reduce kernel in a fixed order; there are no atomics.
@@ -158,7 +158,7 @@ struct GlobalWeights {
     __device__ float b3() const { return pf[OFF_B3]; }
 };
 
-// ... or held in registers for a whole loop over tiles (72 VGPRs: the
+// ... or held in registers for a whole loop over tiles (about 90 VGPRs: the
 // inference kernel, whose waves run many tiles each).
 struct RegWeights {
     v4s w1_[4], w2_[4][4];
