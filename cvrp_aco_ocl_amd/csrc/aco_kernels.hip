@@ -121,23 +121,8 @@ __host__ __device__ __forceinline__ size_t iroulette4_exchange_offset(int n)
 }
 
 // kernel.cl:445-452 mask: not visited, load + d_i <= Q and, with a route
-// length limit, rdist + (d(cur,i) + svc) + d(i,0) <= D.
-template <bool HAS_D>
-__device__ __forceinline__ bool feasible(const DevProblem &p, const AntLds &L, int i, float dci,
-                                         int load, float rdist)
-{
-    if (L.vis[i]) return false;
-    if (load + L.dem[i] > p.Q) return false;
-    if (HAS_D) {
-        float t = rdist + (dci + p.svc);
-        t = t + L.d0[i];
-        return t <= p.D;
-    }
-    return true;
-}
-
-// The same mask without branches (every condition evaluated), so the loads
-// feeding it are issued together.
+// length limit, rdist + (d(cur,i) + svc) + d(i,0) <= D; every condition
+// evaluated, without branches, so the loads feeding it are issued together.
 template <bool HAS_D>
 __device__ __forceinline__ bool feasible_mask(const DevProblem &p, const AntLds &L, int i, int demi, float dci,
                                               int load, float rdist)
@@ -702,7 +687,7 @@ __global__ __launch_bounds__(64 * A) void k_construct_knn_lds(DevBatch bt, int r
             tj = gload(s.tiknn + e);
             dj = dem[j];
         }
-        const bool ok = (lane < k) & (vis[j] == 0) & (load + dj <= p.Q);   // the mask of feasible<false>
+        const bool ok = (lane < k) & (vis[j] == 0) & (load + dj <= p.Q);   // feasible_mask<false>
         // total_info is finite and >= 0, so x * 1 = x and x * 0 = 0 exactly; the
         // product keeps its load unconditional (one 8-byte LDS read with the id)
         const float pj = tj * (ok ? 1.0f : 0.0f);
