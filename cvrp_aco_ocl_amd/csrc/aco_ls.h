@@ -790,7 +790,6 @@ __device__ __forceinline__ int swap_scan(const DevProblem &p, const LsLds &L, in
     const float a1 = rget(rowA, n1), b1 = rget(rowB, sn1);
     const int dn1 = L.dem[n1];
     for (int jb = i + 1; jb < S; jb += U * kLanes) {
-        const unsigned key = read_key(stop.key);
         int tp[U], t[U], tn[U];
         float dm[U], d0[U];
 #pragma unroll
@@ -810,6 +809,8 @@ __device__ __forceinline__ int swap_scan(const DevProblem &p, const LsLds &L, in
             rBp[u] = rowB[tp[u]];
             rBs[u] = rowB[tn[u]];
         }
+        // after the row reads, so its latency overlaps theirs
+        const unsigned key = read_key(stop.key);
         bool imp[U], any = false;
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -987,7 +988,6 @@ __device__ __forceinline__ int relocate_scan(const DevProblem &p, const LsLds &L
         room = __ballot(lane < R && (lane == r1 || rl + dn1 <= p.Q));
     }
     for (int qb = 0; qb < S - 1; qb += U * kLanes) {
-        const unsigned key = read_key(stop.key);
         int t0[U], t1[U], r2[U];
         float rm[U];
 #pragma unroll
@@ -1005,6 +1005,8 @@ __device__ __forceinline__ int relocate_scan(const DevProblem &p, const LsLds &L
             v0[u] = rowB[t0[u]];
             v1[u] = rowB[t1[u]];
         }
+        // after the row reads, so its latency overlaps theirs
+        const unsigned key = read_key(stop.key);
         float add[U];
         bool imp[U], any = false;
 #pragma unroll
