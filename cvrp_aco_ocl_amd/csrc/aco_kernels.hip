@@ -866,6 +866,7 @@ hipError_t launch_construct(const DevBatch &b, int selection, int rng, int apb, 
 // --------------------------------------------------------- statistics ----
 
 constexpr int kStatThreads = 1024;
+constexpr int kStatUnroll = 4;   // tour positions per thread whose loads are issued together
 
 // Block reductions over NT threads (NT / 64 waves); red holds NT/64 + 1.
 template <int NT>
@@ -936,7 +937,22 @@ __device__ void statistics_instance(const DevProblem &p, const DevState &s, floa
         const int sz = s.sizes[best];
         const int *src = s.tours + (size_t)best * T;
         int *dst = s.tours + (size_t)m * T;
-        for (int q = tid; q < sz; q += kStatThreads) dst[q] = src[q];
+        // a trip's loads are issued before its stores (source and
+        // destination are rows of one buffer, which the compiler cannot tell
+        // apart, so it would otherwise wait for each load in turn)
+        for (int q0 = tid; q0 < sz; q0 += kStatUnroll * kStatThreads) {
+            int v[kStatUnroll];
+#pragma unroll
+            for (int u = 0; u < kStatUnroll; ++u) {
+                const int q = q0 + u * kStatThreads;
+                v[u] = src[q < sz ? q : 0];
+            }
+#pragma unroll
+            for (int u = 0; u < kStatUnroll; ++u) {
+                const int q = q0 + u * kStatThreads;
+                if (q < sz) dst[q] = v[u];
+            }
+        }
         if (tid == 0) {
             s.sizes[m] = sz;
             s.lens[m] = best_len;
@@ -987,10 +1003,24 @@ __device__ void statistics_instance(const DevProblem &p, const DevState &s, floa
         const int e = sh_elite[g];
         const int sz = s.sizes[e];
         const int *t = s.tours + (size_t)e * T;
-        for (int q = tid; q < sz - 1; q += kStatThreads) {
-            const int u = t[q], v = t[q + 1];
-            if (u != 0) s.succ[g * p.n + u] = v;
-            else if (v != 0) s.firstf[(size_t)g * p.P + v] = 1;
+        // loads first, then the stores (each node is written once per elite)
+        for (int q0 = tid; q0 < sz - 1; q0 += kStatUnroll * kStatThreads) {
+            int tu[kStatUnroll], tv[kStatUnroll];
+#pragma unroll
+            for (int u = 0; u < kStatUnroll; ++u) {
+                const int q = q0 + u * kStatThreads;
+                const int qc = q < sz - 1 ? q : 0;
+                tu[u] = t[qc];
+                tv[u] = t[qc + 1];
+            }
+#pragma unroll
+            for (int u = 0; u < kStatUnroll; ++u) {
+                const int q = q0 + u * kStatThreads;
+                if (q < sz - 1) {
+                    if (tu[u] != 0) s.succ[g * p.n + tu[u]] = tv[u];
+                    else if (tv[u] != 0) s.firstf[(size_t)g * p.P + tv[u]] = 1;
+                }
+            }
         }
     }
 
@@ -1015,7 +1045,12 @@ __device__ void statistics_instance(const DevProblem &p, const DevState &s, floa
 
 __global__ __launch_bounds__(kStatThreads) void k_statistics(DevBatch bt, float tick_to_s)
 {
-    statistics_instance<kStatThreads>(bt.probs[blockIdx.x], bt.states[blockIdx.x], tick_to_s);
+    // copies in registers: fields read through the global records would be
+    // loaded again after every barrier, each load waiting for all
+    // outstanding memory operations
+    const DevProblem p = bt.probs[blockIdx.x];
+    const DevState s = bt.states[blockIdx.x];
+    statistics_instance<kStatThreads>(p, s, tick_to_s);
 }
 
 hipError_t launch_statistics(const DevBatch &b, float tick_to_s, hipStream_t st)
