@@ -885,46 +885,50 @@ __device__ __forceinline__ void ls_swap(const DevProblem &p, LsLds &L, const Row
         }
         const int is = i0 + wstar;
         if (tid == 0) {
+            // every value read before any write and without branches, so the
+            // reads go out in two batches (positions' entries, then rows,
+            // demands and route tables); the branches below only select
             const RT *rowA = rr.at(is - 1), *rowB = rr.at(is), *rowC = rr.at(is + 1);
             const int n1 = L.tour[is], sn1 = L.tour[is + 1], r1 = L.ridx[is];
             const int n2 = L.tour[js], pn2 = L.tour[js - 1], sn2 = L.tour[js + 1], r2 = L.ridx[js];
+            const float dm = L.dtw[js - 1], d0 = L.dtw[js];
             const float a1 = rget(rowA, n1), b1 = rget(rowB, sn1);
-            const int dn1 = L.dem[n1];
+            // row values of the new edges: (pn1, n2), (n2, sn1), (pn2, n1),
+            // (n1, sn2) and, for adjacent positions, (n2, n1) (symmetric
+            // distances: the same values as the distance matrix)
+            const float an2 = rget(rowA, n2), cn2 = rget(rowC, n2), bpn2 = rget(rowB, pn2);
+            const float bsn2 = rget(rowB, sn2), bn2 = rget(rowB, n2);
+            const int dn1 = L.dem[n1], dn2 = L.dem[n2], rl1 = L.rl[r1], rl2 = L.rl[r2];
+            const float rd1 = L.rd[r1], rd2 = L.rd[r2];
+            const bool adj = js == is + 1;
             int load1, load2;
             float dist1, dist2;
             if (r1 != r2) {
-                const int dn2 = L.dem[n2];
-                load1 = L.rl[r1] - dn1 + dn2;
-                load2 = L.rl[r2] - dn2 + dn1;
-                dist1 = (L.rd[r1] - (a1 + b1)) + (rget(rowA, n2) + rget(rowC, n2));
-                dist2 = (L.rd[r2] - (L.dtw[js - 1] + L.dtw[js])) + (rget(rowB, pn2) + rget(rowB, sn2));
+                load1 = rl1 - dn1 + dn2;
+                load2 = rl2 - dn2 + dn1;
+                dist1 = (rd1 - (a1 + b1)) + (an2 + cn2);
+                dist2 = (rd2 - (dm + d0)) + (bpn2 + bsn2);
             } else {
-                float gain;
-                if (js == is + 1)
-                    gain = -(a1 + L.dtw[js]) + (rget(rowA, n2) + rget(rowB, sn2));
-                else
-                    gain = -(a1 + b1 + L.dtw[js - 1] + L.dtw[js]) +
-                           (rget(rowA, n2) + rget(rowC, n2) + rget(rowB, pn2) + rget(rowB, sn2));
-                load1 = L.rl[r1];
+                const float gain = adj ? -(a1 + d0) + (an2 + bsn2)
+                                       : -(a1 + b1 + dm + d0) + (an2 + cn2 + bpn2 + bsn2);
+                load1 = rl1;
                 load2 = load1;
-                dist1 = L.rd[r1] + gain;
+                dist1 = rd1 + gain;
                 dist2 = dist1;
             }
             L.tour[is] = n2;
             L.tour[js] = n1;
             L.rl[r1] = load1; L.rl[r2] = load2;
             L.rd[r1] = dist1; L.rd[r2] = dist2;
-            // the four new edges, from the rows of pn1, n1, sn1 (symmetric
-            // distances: the same values as the distance matrix)
-            if (js == is + 1) {
-                L.dtw[is - 1] = rget(rowA, n2);   // (pn1, n2)
-                L.dtw[is] = rget(rowB, n2);       // (n2, n1)
-                L.dtw[js] = rget(rowB, sn2);      // (n1, sn2)
+            if (adj) {
+                L.dtw[is - 1] = an2;   // (pn1, n2)
+                L.dtw[is] = bn2;       // (n2, n1)
+                L.dtw[js] = bsn2;      // (n1, sn2)
             } else {
-                L.dtw[is - 1] = rget(rowA, n2);   // (pn1, n2)
-                L.dtw[is] = rget(rowC, n2);       // (n2, sn1)
-                L.dtw[js - 1] = rget(rowB, pn2);  // (pn2, n1)
-                L.dtw[js] = rget(rowB, sn2);      // (n1, sn2)
+                L.dtw[is - 1] = an2;   // (pn1, n2)
+                L.dtw[is] = cn2;       // (n2, sn1)
+                L.dtw[js - 1] = bpn2;  // (pn2, n1)
+                L.dtw[js] = bsn2;      // (n1, sn2)
             }
         }
         __syncthreads();
@@ -1087,35 +1091,36 @@ __device__ __forceinline__ void ls_relocate(const DevProblem &p, LsLds &L, const
             continue;
         }
         const int is = i0 + wstar;
-        const int n1 = L.tour[is];
         const int lo = is < js ? is : js, hi = is < js ? js : is;
+        // Every value the move needs, read by every thread before any write
+        // and without branches, so the reads go out in two batches: the
+        // positions' entries, then the rows, the demand and the route
+        // tables.  x is the insertion's other neighbour (pn2 before j, sn2
+        // after it); the new edges are (pn1, sn1) = c1s and (n1, n2), (n1,
+        // x) = e_n2, e_x.
+        const int n1 = L.tour[is], sn1 = L.tour[is + 1], n2 = L.tour[js];
+        const int xs = L.tour[is > js ? js - 1 : js + 1];
+        const int r1 = L.ridx[is], r2 = L.ridx[js];
+        const float a1 = L.dtw[is - 1], b1 = L.dtw[is], rm = L.dtw[is > js ? js - 1 : js];
+        const RT *rowPs = rr.at(is - 1), *rowBs = rr.at(is);
+        const float c1s = rget(rowPs, sn1), e_n2 = rget(rowBs, n2), e_x = rget(rowBs, xs);
+        const int dn1 = L.dem[n1], rl1 = L.rl[r1], rl2 = L.rl[r2];
+        const float rd1 = L.rd[r1], rd2 = L.rd[r2];
+        const int rc1 = L.rc[r1], rc2 = L.rc[r2];
         if (tid == 0) {
-            const RT *rowP = rr.at(is - 1), *rowB = rr.at(is);
-            const int r1 = L.ridx[is], sn1 = L.tour[is + 1];
-            const int n2 = L.tour[js], pn2 = L.tour[js - 1], sn2 = L.tour[js + 1], r2 = L.ridx[js];
-            const float a1 = L.dtw[is - 1], b1 = L.dtw[is], c1 = rget(rowP, sn1);
-            const int dn1 = L.dem[n1];
-            float rm, add;
-            if (is > js) { rm = L.dtw[js - 1]; add = rget(rowB, n2) + rget(rowB, pn2); }
-            else { rm = L.dtw[js]; add = rget(rowB, n2) + rget(rowB, sn2); }
-            const float gain = -(a1 + b1 + rm) + (add + c1);
+            const float add = e_n2 + e_x;
+            const float gain = -(a1 + b1 + rm) + (add + c1s);
             if (r1 != r2) {
-                L.rl[r1] -= dn1;
-                L.rl[r2] += dn1;
-                L.rd[r1] = ((L.rd[r1] - p.svc) - (a1 + b1)) + c1;
-                L.rd[r2] = ((L.rd[r2] + p.svc) - rm) + add;
-                L.rc[r1] = (short)(L.rc[r1] - 1);
-                L.rc[r2] = (short)(L.rc[r2] + 1);
+                L.rl[r1] = rl1 - dn1;
+                L.rl[r2] = rl2 + dn1;
+                L.rd[r1] = ((rd1 - p.svc) - (a1 + b1)) + c1s;
+                L.rd[r2] = ((rd2 + p.svc) - rm) + add;
+                L.rc[r1] = (short)(rc1 - 1);
+                L.rc[r2] = (short)(rc2 + 1);
             } else {
-                L.rd[r1] = L.rd[r1] + gain;
+                L.rd[r1] = rd1 + gain;
             }
         }
-        // the three new edges (removal point, both sides of the insertion),
-        // read by every thread before the shift
-        const RT *rowPs = rr.at(is - 1), *rowBs = rr.at(is);
-        const int sn1s = L.tour[is + 1], n2s = L.tour[js];
-        const int xs = is > js ? L.tour[js - 1] : L.tour[js + 1];
-        const float c1s = rget(rowPs, sn1s), e_n2 = rget(rowBs, n2s), e_x = rget(rowBs, xs);
         // shift, in place: positions [lo, hi] move by one towards is, n1
         // lands at js (in the route of the node that was at js); route
         // indices and the edge weights of [lo-1, hi] move with them (no
