@@ -108,6 +108,38 @@ __device__ __forceinline__ AntLds ant_lds(unsigned char *smem, int n, bool with_
     return L;
 }
 
+// The instance's demands, depot distances and (with rank) reduction ranks
+// copied into LDS by one wave, visited flags set to the depot alone:
+// kCopyUnroll nodes per lane have their loads issued before their stores
+// (a loop storing each trip would wait for that trip's loads in turn).
+constexpr int kCopyUnroll = 8;
+template <bool RANK>
+__device__ __forceinline__ void ant_lds_fill(const DevProblem &p, const AntLds &L, int n, int lane)
+{
+    for (int i0 = lane; i0 < n; i0 += kCopyUnroll * kLanes) {
+        int dm[kCopyUnroll], rk[kCopyUnroll];
+        float dz[kCopyUnroll];
+#pragma unroll
+        for (int u = 0; u < kCopyUnroll; ++u) {
+            const int i = i0 + u * kLanes, ic = i < n ? i : n - 1;
+            dm[u] = p.dem[ic];
+            dz[u] = p.d0[ic];
+            rk[u] = RANK ? p.rank[ic] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < kCopyUnroll; ++u) {
+            const int i = i0 + u * kLanes;
+            if (i < n) {
+                L.dem[i] = dm[u];
+                L.d0[i] = dz[u];
+                if (RANK) L.rank[i] = rk[u];
+                L.vis[i] = i == 0;
+            }
+        }
+    }
+}
+
+
 size_t construct_lds_bytes(int n, int selection)
 {
     return (size_t)n * (4 + 4 + (selection == kSelIRoulette ? 4 : 0) + 1) + 16;
@@ -228,12 +260,23 @@ __device__ __forceinline__ float tour_length_in_order(const DevProblem &p, const
                                                       int lane)
 {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");   // this wave's tour stores, fresh L1
+    // the loads of kLenTrips chunks of 64 edges are issued together (a
+    // position past the last reads edge 1 instead; its value is not added)
+    constexpr int kLenTrips = 4;
     float t = 0.0f;
-    for (int base = 1; base <= last; base += kLanes) {
-        const int q = base + lane;
-        const float d = q <= last ? p.dist[(size_t)tour[q - 1] * p.P + tour[q]] : 0.0f;
-        const int cnt = last - base + 1 < kLanes ? last - base + 1 : kLanes;
-        for (int l = 0; l < cnt; ++l) t = t + lane_f32(d, l);
+    for (int base0 = 1; base0 <= last; base0 += kLenTrips * kLanes) {
+        float d[kLenTrips];
+#pragma unroll
+        for (int g = 0; g < kLenTrips; ++g) {
+            const int q = base0 + g * kLanes + lane, qc = q <= last ? q : 1;
+            d[g] = p.dist[(size_t)tour[qc - 1] * p.P + tour[qc]];
+        }
+#pragma unroll
+        for (int g = 0; g < kLenTrips; ++g) {
+            const int base = base0 + g * kLanes;
+            const int cnt = last - base + 1 < kLanes ? last - base + 1 : kLanes;
+            for (int l = 0; l < cnt; ++l) t = t + lane_f32(d[g], l);
+        }
     }
     return t;
 }
@@ -296,12 +339,7 @@ __global__ __launch_bounds__(64) void k_construct_iroulette(DevBatch bt)
     ANT_PROLOGUE
     const int n = p.n, lane = threadIdx.x;
     AntLds L = ant_lds(smem, n, true);
-    for (int i = lane; i < n; i += kLanes) {
-        L.dem[i] = p.dem[i];
-        L.d0[i] = p.d0[i];
-        L.rank[i] = p.rank[i];
-        L.vis[i] = i == 0;
-    }
+    ant_lds_fill<true>(p, L, n, lane);
     __syncthreads();
     int z = s.pm[a * kLanes + lane];
     int *tour = s.tours + (size_t)a * p.T;
@@ -600,11 +638,7 @@ __global__ __launch_bounds__(64) void k_construct_knn(DevBatch bt)
     ANT_PROLOGUE
     const int n = p.n, lane = threadIdx.x;
     AntLds L = ant_lds(smem, n, false);
-    for (int i = lane; i < n; i += kLanes) {
-        L.dem[i] = p.dem[i];
-        L.d0[i] = p.d0[i];
-        L.vis[i] = i == 0;
-    }
+    ant_lds_fill<false>(p, L, n, lane);
     __syncthreads();
     construct_knn_ant<HAS_D, RNG, WIDE>(p, s, a, L, lane);
 }
