@@ -10,8 +10,12 @@ size_t local_search_lds_bytes(int n, int ring, int row16, int flags)
     return ls_offsets(n, ring, row16 ? 2 : 4, (flags & kLs2OptRandom) != 0).total;
 }
 
+// With 4 or 8 waves per ant several ants share a CU: registers are held to
+// 128 per lane so that four waves fit each SIMD (not for the random-order
+// 2-opt instantiations, which would spill).
 template <int W, typename RT, bool R2>
-__global__ __launch_bounds__(W * 64) void k_local_search(DevBatch bt, int first, int flags, int ring, int prefetch)
+__global__ __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(W <= 8 && !R2 ? 4 : 1)))
+void k_local_search(DevBatch bt, int first, int flags, int ring, int prefetch)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const int g = bt.ls_order[first + blockIdx.x];   // this launch's size class

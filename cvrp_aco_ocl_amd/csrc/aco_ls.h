@@ -344,6 +344,31 @@ __device__ __forceinline__ int scan_pos(const LsLds &L, int rbeg, int l)
     return RANDOM ? rbeg + L.perm[rbeg + l] : rbeg + l;
 }
 
+template <bool RANDOM>
+__device__ __forceinline__ bool two_opt_skip(const LsLds &L, int rbeg, int l, bool dlb0)
+{
+    const int v = L.tour[scan_pos<RANDOM>(L, rbeg, l)];
+    return v == 0 ? dlb0 : (L.dlb[v] != 0);
+}
+
+// First scan index in [l, num) whose position is not skipped, or num: the
+// GW lanes of the calling group (gl = 0 .. GW-1, an aligned run of lanes)
+// test GW indices at once and take the lowest from a ballot.  No don't-look
+// bit changes between the tests of a sequential walk, so this is the index
+// that walk stops at.
+template <bool RANDOM, int GW>
+__device__ __forceinline__ int two_opt_next(const LsLds &L, int rbeg, int l, int num, bool dlb0, int gl)
+{
+    const int gshift = (int)(threadIdx.x & 63) & ~(GW - 1);
+    for (int base = l; base < num; base += GW) {
+        const int q = base + gl;
+        const bool ok = q < num && !two_opt_skip<RANDOM>(L, rbeg, q < num ? q : base, dlb0);
+        const unsigned gm = (unsigned)(__ballot(ok) >> gshift) & ((1u << GW) - 1u);
+        if (gm) return base + __builtin_ctz(gm);
+    }
+    return num;
+}
+
 // kernel.cl:126-277 for route r at positions [rbeg, rend] (rbeg holds the
 // depot), run by the kOptGroup lanes gl = 0.. of one group; RANDOM scans
 // the positions in the order of perm (localSearch.cpp:180-294).
@@ -358,9 +383,10 @@ __device__ __forceinline__ void two_opt_route(const DevProblem &p, LsLds &L, int
     while (imp) {
         imp = false;
         for (int l = 0; l < num; ++l) {
+            l = two_opt_next<RANDOM, kOptGroup>(L, rbeg, l, num, dlb0, gl);
+            if (l >= num) break;
             const int pos1 = scan_pos<RANDOM>(L, rbeg, l);
             const int n1 = L.tour[pos1];
-            if (n1 == 0 ? dlb0 : (L.dlb[n1] != 0)) continue;
             int h1 = 0, h2 = 0, h3 = 0, h4 = 0;
             const int s1 = pos1 == rend ? 0 : L.tour[pos1 + 1];
             const int p1 = pos1 == rbeg ? L.tour[rend] : L.tour[pos1 - 1];
@@ -422,6 +448,12 @@ __device__ __forceinline__ void two_opt_route(const DevProblem &p, LsLds &L, int
     if (gl == 0) moves_total += moves;
 }
 
+// kind (2 bits) and the move's four nodes (13 bits each; n <= 4096)
+__device__ __forceinline__ u64 two_opt_pack(int kind, int h1, int h2, int h3, int h4)
+{
+    return ((u64)kind << 52) | ((u64)h1 << 39) | ((u64)h2 << 26) | ((u64)h3 << 13) | (u64)h4;
+}
+
 // The same search with two positions at a time (16 lanes per route): the
 // halves of the group scan the next two positions whose don't-look bits are
 // clear.  The second result stands only when the first position has no
@@ -430,19 +462,6 @@ __device__ __forceinline__ void two_opt_route(const DevProblem &p, LsLds &L, int
 // sequential search would see.  After a move at the first position the
 // search continues right after it and scans the second again.  Halves
 // exchange their results with one DPP swap (16-lane groups are DPP rows).
-template <bool RANDOM>
-__device__ __forceinline__ bool two_opt_skip(const LsLds &L, int rbeg, int l, bool dlb0)
-{
-    const int v = L.tour[scan_pos<RANDOM>(L, rbeg, l)];
-    return v == 0 ? dlb0 : (L.dlb[v] != 0);
-}
-
-// kind (2 bits) and the move's four nodes (13 bits each; n <= 4096)
-__device__ __forceinline__ u64 two_opt_pack(int kind, int h1, int h2, int h3, int h4)
-{
-    return ((u64)kind << 52) | ((u64)h1 << 39) | ((u64)h2 << 26) | ((u64)h3 << 13) | (u64)h4;
-}
-
 template <bool RANDOM>
 __device__ __forceinline__ void two_opt_route_pair(const DevProblem &p, LsLds &L, int r, int rbeg, int rend,
                                                    int gl16, long long &evals, int &moves_total)
@@ -457,11 +476,9 @@ __device__ __forceinline__ void two_opt_route_pair(const DevProblem &p, LsLds &L
         imp = false;
         int l = 0;
         for (;;) {
-            int la = l;
-            while (la < num && two_opt_skip<RANDOM>(L, rbeg, la, dlb0)) ++la;
+            const int la = two_opt_next<RANDOM, G>(L, rbeg, l, num, dlb0, gl16);
             if (la >= num) break;
-            int lb = la + 1;
-            while (lb < num && two_opt_skip<RANDOM>(L, rbeg, lb, dlb0)) ++lb;
+            const int lb = two_opt_next<RANDOM, G>(L, rbeg, la + 1, num, dlb0, gl16);
             const int na = L.tour[scan_pos<RANDOM>(L, rbeg, la)];
             const int nb = lb < num ? L.tour[scan_pos<RANDOM>(L, rbeg, lb)] : -1;
             const int lm = half == 0 ? la : lb;
