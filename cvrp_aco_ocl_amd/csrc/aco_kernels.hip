@@ -952,10 +952,13 @@ __device__ void statistics_instance(const DevProblem &p, const DevState &s, floa
     __shared__ int sh_elite[kMaxRas];
     const int tid = threadIdx.x, m = p.m, T = p.T;
 
-    // 1. iteration best: key (length bits, group, rank-in-group)
+    // 1. iteration best: key (length bits, group, rank-in-group).  The
+    //    length of the thread's first ant is kept for the elite rounds.
+    const float len_own = tid < m ? s.lens[tid] : 0.0f;
     u64 kmin = ~0ull;
     for (int a = tid; a < m; a += kStatThreads) {
-        const u64 key = ((u64)f2u(s.lens[a]) << 32) | (u64)((a & ~63) | p.rank64[a & 63]);
+        const float len = a == tid ? len_own : s.lens[a];
+        const u64 key = ((u64)f2u(len) << 32) | (u64)((a & ~63) | p.rank64[a & 63]);
         kmin = key < kmin ? key : kmin;
     }
     kmin = block_min_u64<NT>(kmin, red);
@@ -1017,7 +1020,7 @@ __device__ void statistics_instance(const DevProblem &p, const DevState &s, floa
         for (int a = tid; a < m; a += kStatThreads) {
             bool taken = false;
             for (int q = 1; q < g; ++q) taken |= sh_elite[q] == a;
-            const float v = taken ? __int_as_float(0x7f800000) : s.lens[a];
+            const float v = taken ? __int_as_float(0x7f800000) : a == tid ? len_own : s.lens[a];
             const u64 key = ((u64)f2u(v) << 32) | (u64)a;
             km = key < km ? key : km;
         }
@@ -1033,9 +1036,13 @@ __device__ void statistics_instance(const DevProblem &p, const DevState &s, floa
     }
     for (int q = tid; q < p.ras * p.P; q += kStatThreads) s.firstf[q] = 0;
     __syncthreads();
+    // the elites' tour sizes, requested together
+    int szs[kMaxRas];
+#pragma unroll
+    for (int g = 0; g < kMaxRas; ++g) szs[g] = g < p.ras ? s.sizes[sh_elite[g]] : 0;
     for (int g = 0; g < p.ras; ++g) {
         const int e = sh_elite[g];
-        const int sz = s.sizes[e];
+        const int sz = szs[g];
         const int *t = s.tours + (size_t)e * T;
         // loads first, then the stores (each node is written once per elite)
         for (int q0 = tid; q0 < sz - 1; q0 += kStatUnroll * kStatThreads) {
